@@ -1,0 +1,142 @@
+/*
+ * sphadj.h -- C ABI of libsphadj.so, the MI355X (gfx950) implementation of
+ * SPHAdjoint's differentiable 2D WCSPH step.
+ *
+ * The ABI mirrors the reference's public Julia API (src/SPHAdjoint.jl:39-40):
+ * a Julia `ccall` wrapper (or Python ctypes) keeps the reference's names and
+ * argument meaning and calls these entry points instead of launching
+ * KernelAbstractions kernels.  Every entry point returns an sph_status; the
+ * message of the last failure on the calling thread is sph_last_error().
+ *
+ * Conventions (the reference's):
+ *   - X, V, a, seeds, gradients: (2, N) column-major doubles, i.e. interleaved
+ *     x,y per particle, in particle-id order.
+ *   - theta / gtheta: (ngy, ngx) column-major doubles.
+ *   - cell ids, particle ids and `order` are 1-based; `starts` is the exclusive
+ *     0-based prefix of `counts`.
+ * Device memory is owned by the library behind opaque handles; host pointers
+ * are borrowed for the duration of a call.  Calls are asynchronous on the
+ * context's stream except where they return host data (get/objective/flags)
+ * or where the reference synchronises (simulate, backward).
+ */
+#ifndef SPHADJ_H
+#define SPHADJ_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sph_status {
+  SPH_OK = 0,
+  SPH_E_ARG = 1,       /* invalid argument (reference: ArgumentError on layout etc.) */
+  SPH_E_CAPACITY = 2,  /* tape too small (reference: ArgumentError, src/driver.jl:96-98) */
+  SPH_E_HIP = 3,       /* HIP runtime failure */
+  SPH_E_STATE = 4,     /* call not valid in the current state */
+  SPH_E_NOMEM = 5      /* device allocation failed */
+} sph_status;
+
+/* Arithmetic form of a state / tape / workspace. */
+typedef enum sph_precision {
+  SPH_F64 = 0,    /* plain FP64: the reference's Float64 arithmetic, operation for operation */
+  SPH_F32C = 1    /* FP32 state and gathers with compensated sums; FP64 x += dt*v, J, gtheta */
+} sph_precision;
+
+typedef enum sph_layout { SPH_LAYOUT_SLOT = 0, SPH_LAYOUT_PARTICLE = 1 } sph_layout;
+
+typedef enum sph_field_id {
+  SPH_X = 0, SPH_V = 1, SPH_A = 2, SPH_RHO = 3, SPH_PTERM = 4, SPH_INVRHO = 5, SPH_ALPHA = 6
+} sph_field_id;
+
+typedef enum sph_grad_id { SPH_GX = 0, SPH_GV = 1, SPH_GTHETA = 2 } sph_grad_id;
+
+/* SPHParams{T} (src/params.jl:14-28); gx defaults to 0 and gy to -9.81 there. */
+typedef struct sph_params {
+  double h, m, rho0, c, mu, gx, gy, dt, Lx, Ly, kw;
+  int64_t ngx, ngy;
+} sph_params;
+
+typedef struct sph_ctx sph_ctx;
+typedef struct sph_field sph_field;
+typedef struct sph_state sph_state;
+typedef struct sph_tape sph_tape;
+typedef struct sph_workspace sph_workspace;
+
+/* --- context: replaces the KernelAbstractions `backend` object (src/driver.jl:24) --- */
+const char *sph_last_error(void);
+int sph_ctx_create(int device, sph_ctx **out);
+int sph_ctx_destroy(sph_ctx *ctx);
+/* KernelAbstractions.synchronize(backend) (src/driver.jl:110,204) */
+int sph_sync(sph_ctx *ctx);
+
+/* --- design field theta, a device array of shape (ngy, ngx) (src/params.jl:70-82) --- */
+int sph_field_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, const double *host, sph_field **out);
+int sph_field_set(sph_field *f, const double *host);
+int sph_field_get(const sph_field *f, double *host);
+int sph_field_destroy(sph_field *f);
+
+/* --- State (src/driver.jl:5-36) with its CellList and NeighborList (src/neighbors.jl:28-36,174-190) --- */
+int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
+                     const double *X0, const double *V0, double skin, int64_t interval,
+                     int64_t maxnb, int layout, sph_state **out);
+int sph_state_destroy(sph_state *st);
+/* st.X / st.V / st.a / st.rho / st.pterm / st.invrho / st.alpha, in particle-id order */
+int sph_state_get(sph_state *st, int field, double *host);
+/* copyto!(st.X, ...) / copyto!(st.V, ...) */
+int sph_state_set(sph_state *st, int field, const double *host);
+/* out8 = {nx, ny, ncell, maxnb, sm, si, interval, age} */
+int sph_state_info(const sph_state *st, int64_t *out8);
+
+/* build!(cl, X, p, backend) (src/neighbors.jl:97-114) */
+int sph_build(sph_state *st);
+/* cl.cellof, cl.counts, cl.starts, cl.order of the last build (any pointer may be NULL) */
+int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *starts,
+                      int32_t *order);
+/* build_neighbors!(nl, cl, X, p, backend) (src/neighbors.jl:252-263) */
+int sph_build_neighbors(sph_state *st);
+/* nl.indices (N*maxnb, with the state's (sm, si) strides) and nl.counts of the last build */
+int sph_neighbors_get(sph_state *st, int32_t *indices, int32_t *counts);
+/* nl.flags: [0] row overflow, [1] displacement beyond skin*h (src/neighbors.jl:282-290) */
+int sph_flags_get(sph_state *st, int32_t *flags2);
+int sph_flags_clear(sph_state *st);
+
+/* step!(st, theta, p, backend) (src/forward.jl:161-173) */
+int sph_step(sph_state *st, const sph_field *theta);
+/* simulate!(st, theta, p, backend, nsteps; tape) (src/driver.jl:92-115); tape may be NULL */
+int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape *tape);
+
+/* --- Tape (src/driver.jl:45-83) --- */
+int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out);
+int sph_tape_destroy(sph_tape *tp);
+int sph_tape_reset(sph_tape *tp);                            /* reset!(tape) */
+int sph_tape_length(const sph_tape *tp, int64_t *n);         /* length(tape) */
+int sph_tape_capacity(const sph_tape *tp, int64_t *cap);     /* capacity(tape) */
+/* tape.X[:, :, k], tape.V[:, :, k] (1-based k), particle-id order */
+int sph_tape_get(sph_tape *tp, int64_t k, double *X, double *V);
+
+/* --- AdjointWorkspace (src/driver.jl:117-133) and backward! (src/driver.jl:141-207) --- */
+int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
+                         sph_workspace **out);
+int sph_workspace_destroy(sph_workspace *ws);
+/* ws.gX / ws.gV ((2,N), id order) and ws.gtheta ((ngy,ngx)) */
+int sph_workspace_get(sph_workspace *ws, int which, double *host);
+int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
+                 const double *seedX, const double *seedV, double skin, int64_t interval,
+                 int64_t maxnb, int layout);
+
+/* target_objective(st.X, xc, yc, sigma) (src/driver.jl:219-234): J in FP64 and,
+ * when seed != NULL, dJ/dX_T as (2,N) in id order. */
+int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
+                         double *seed);
+
+/* --- measurement: per-kernel GPU time with events on the context's stream.
+ * No counterpart in the reference (its stage timing lives in
+ * scripts/06_metal_profile.jl:158-290); used by bench.py for the roofline. */
+int sph_profile_enable(sph_ctx *ctx, int on);     /* also resets the counters */
+int sph_profile_get(sph_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+const char *sph_profile_name(int kernel);          /* NULL past the last kernel id */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,418 @@
+"""Python mirror of the reference's public API over libsphadj.so.
+
+Names, argument order and meaning follow src/SPHAdjoint.jl:39-40 with Julia's
+``!`` spelled as a trailing underscore (``simulate!`` -> ``simulate_``):
+
+    SPHParams                       src/params.jl:14-28
+    State(backend, X0, V0, p; ...)  src/driver.jl:24-36
+    CellList / NeighborList views   src/neighbors.jl:9-36, 150-190
+    build_(cl, X, p, backend)       src/neighbors.jl:97-114
+    build_neighbors_(nl, cl, X, p, backend)   src/neighbors.jl:252-263
+    step_(st, theta, p, backend)    src/forward.jl:161-173
+    simulate_(st, theta, p, backend, nsteps; tape)   src/driver.jl:92-115
+    Tape(backend, N, nsteps, p), reset_(tape)        src/driver.jl:62-71
+    AdjointWorkspace(backend, N, p)                  src/driver.jl:128-133
+    backward_(ws, tape, theta, p, backend; seedX, seedV, ...)  src/driver.jl:141-207
+    target_objective(X, xc, yc, sigma)               src/driver.jl:219-234
+
+``backend`` is an :class:`MI355XBackend` (the KernelAbstractions backend
+object's counterpart).  Arrays are numpy: X/V/a and seeds have shape (2, N),
+theta has shape (ngy, ngx).  Reading ``st.X`` copies device data to the host,
+as ``Array(st.X)`` does in Julia.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _lib as L
+
+_PREC = {"f64": L.SPH_F64, "float64": L.SPH_F64, "f32c": L.SPH_F32C}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class MI355XBackend:
+    """One HIP device and one in-order stream (the KA backend's role)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        L.call("sph_ctx_create", int(device), C.byref(h))
+        self._h = h
+        self.device = device
+
+    def synchronize(self):
+        L.call("sph_sync", self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_ctx_destroy(h)
+            self._h = None
+
+
+class SPHParams:
+    """SPHParams{T} (src/params.jl:14-28).  ``T`` is kept for interface parity;
+    the arithmetic form is chosen per State (``precision``)."""
+
+    def __init__(self, *, h, m, rho0, c, mu, dt, Lx, Ly, kw, ngx, ngy, gx=0.0, gy=-9.81,
+                 T=np.float64):
+        self.h, self.m, self.rho0, self.c, self.mu = float(h), float(m), float(rho0), float(c), float(mu)
+        self.gx, self.gy, self.dt = float(gx), float(gy), float(dt)
+        self.Lx, self.Ly, self.kw = float(Lx), float(Ly), float(kw)
+        self.ngx, self.ngy = int(ngx), int(ngy)
+        self.T = np.dtype(T)
+
+    def c_struct(self):
+        return L.sph_params(self.h, self.m, self.rho0, self.c, self.mu, self.gx, self.gy, self.dt,
+                            self.Lx, self.Ly, self.kw, self.ngx, self.ngy)
+
+    def support(self):
+        return 2 * self.h
+
+    def dxg(self):
+        return self.Lx / (self.ngx - 1)
+
+    def dyg(self):
+        return self.Ly / (self.ngy - 1)
+
+
+class Field:
+    """Device-resident design field theta of shape (ngy, ngx)."""
+
+    def __init__(self, backend: MI355XBackend, theta):
+        th = np.asfortranarray(np.asarray(theta, dtype=np.float64))
+        if th.ndim != 2:
+            raise ValueError("theta must be 2-D (ngy, ngx)")
+        self.shape = th.shape
+        self.backend = backend
+        h = C.c_void_p()
+        L.call("sph_field_create", backend._h, th.shape[0], th.shape[1], _ptr(th), C.byref(h))
+        self._h = h
+
+    def set(self, theta):
+        th = np.asfortranarray(np.asarray(theta, dtype=np.float64))
+        if th.shape != self.shape:
+            raise ValueError("shape mismatch")
+        L.call("sph_field_set", self._h, _ptr(th))
+
+    def __array__(self, dtype=None, copy=None):
+        out = np.empty(self.shape[::-1], np.float64)
+        L.call("sph_field_get", self._h, _ptr(out))
+        out = out.T
+        return out.astype(dtype) if dtype is not None else out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_field_destroy(h)
+            self._h = None
+
+
+def _field(theta, p: SPHParams, backend: MI355XBackend) -> Field:
+    if isinstance(theta, Field):
+        f = theta
+    else:
+        f = Field(backend, theta)
+    if f.shape != (p.ngy, p.ngx):
+        raise ValueError(f"theta must be (ngy, ngx) = {(p.ngy, p.ngx)}, got {f.shape}")
+    return f
+
+
+def _as_2n(A, N=None):
+    A = np.asarray(A, dtype=np.float64)
+    if A.ndim != 2 or A.shape[0] != 2:
+        raise ValueError("expected an array of shape (2, N)")
+    if N is not None and A.shape[1] != N:
+        raise ValueError(f"expected (2, {N}), got {A.shape}")
+    return np.ascontiguousarray(A.T)       # interleaved x,y per particle
+
+
+class CellList:
+    """View of a state's cell list (src/neighbors.jl:9-19)."""
+
+    def __init__(self, st):
+        self._st = st
+        info = st._info()
+        self.nx, self.ny = int(info[0]), int(info[1])
+        self.cs = st.rc
+
+    def arrays(self):
+        st = self._st
+        n, nc = st.N, self.nx * self.ny
+        cellof = np.zeros(n, np.int32)
+        counts = np.zeros(nc, np.int32)
+        starts = np.zeros(nc, np.int32)
+        order = np.zeros(n, np.int32)
+        L.call("sph_cell_list_get", st._h, _ptr(cellof), _ptr(counts), _ptr(starts), _ptr(order))
+        return cellof, counts, starts, order
+
+    cellof = property(lambda s: s.arrays()[0])
+    counts = property(lambda s: s.arrays()[1])
+    starts = property(lambda s: s.arrays()[2])
+    order = property(lambda s: s.arrays()[3])
+
+
+class NeighborList:
+    """View of a state's neighbour list (src/neighbors.jl:150-162).  The list
+    is materialised from the positions of the last build on request."""
+
+    def __init__(self, st):
+        self._st = st
+        info = st._info()
+        self.maxnb, self.sm, self.si = int(info[3]), int(info[4]), int(info[5])
+        self.rc = st.rc
+        self.dmax = st.p.h * st.skin
+
+    @property
+    def interval(self):
+        return int(self._st._info()[6])
+
+    @property
+    def age(self):
+        return int(self._st._info()[7])
+
+    def arrays(self):
+        st = self._st
+        idx = np.zeros(st.N * self.maxnb, np.int32)
+        cnt = np.zeros(st.N, np.int32)
+        L.call("sph_neighbors_get", st._h, _ptr(idx), _ptr(cnt))
+        return idx, cnt
+
+    indices = property(lambda s: s.arrays()[0])
+    counts = property(lambda s: s.arrays()[1])
+
+    @property
+    def flags(self):
+        f = np.zeros(2, np.int32)
+        L.call("sph_flags_get", self._st._h, _ptr(f))
+        return f
+
+    def rows(self):
+        idx, cnt = self.arrays()
+        return [[int(idx[m * self.sm + i * self.si]) for m in range(int(cnt[i]))]
+                for i in range(self._st.N)]
+
+
+class State:
+    """State (src/driver.jl:5-36).  ``precision`` selects the arithmetic form:
+    "f64" (the reference's Float64 arithmetic) or "f32c"."""
+
+    def __init__(self, backend: MI355XBackend, X0, V0, p: SPHParams, *, skin=0.2, interval=4,
+                 maxnb=64, layout="auto", precision="f64"):
+        if layout == "auto":
+            layout = "slot"                        # GPU backends pick slot-major
+        if layout not in ("slot", "particle"):
+            raise ValueError("layout must be :slot / :particle / :auto")
+        self.backend, self.p = backend, p
+        self.skin = float(skin)
+        self.rc = (2 * p.h) * (1 + self.skin)
+        self.precision = _PREC[precision]
+        x0 = _as_2n(X0)
+        self.N = x0.shape[0]
+        v0 = _as_2n(V0, self.N)
+        cp = p.c_struct()
+        h = C.c_void_p()
+        L.call("sph_state_create", backend._h, C.byref(cp), self.precision, self.N, _ptr(x0),
+               _ptr(v0), self.skin, int(interval), int(maxnb),
+               L.SPH_LAYOUT_SLOT if layout == "slot" else L.SPH_LAYOUT_PARTICLE, C.byref(h))
+        self._h = h
+        self.cl = CellList(self)
+        self.nl = NeighborList(self)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_state_destroy(h)
+            self._h = None
+
+    def _info(self):
+        o = np.zeros(8, np.int64)
+        L.call("sph_state_info", self._h, _ptr(o))
+        return o
+
+    def _get(self, which, two):
+        out = np.empty((self.N, 2) if two else self.N, np.float64)
+        L.call("sph_state_get", self._h, which, _ptr(out))
+        return np.ascontiguousarray(out.T) if two else out
+
+    def _set(self, which, A):
+        a = _as_2n(A, self.N)
+        L.call("sph_state_set", self._h, which, _ptr(a))
+
+    X = property(lambda s: s._get(L.SPH_X, True), lambda s, v: s._set(L.SPH_X, v))
+    V = property(lambda s: s._get(L.SPH_V, True), lambda s, v: s._set(L.SPH_V, v))
+    a = property(lambda s: s._get(L.SPH_A, True))
+    rho = property(lambda s: s._get(L.SPH_RHO, False))
+    pterm = property(lambda s: s._get(L.SPH_PTERM, False))
+    invrho = property(lambda s: s._get(L.SPH_INVRHO, False))
+    alpha = property(lambda s: s._get(L.SPH_ALPHA, False))
+
+
+def check_neighbor_flags(nl: NeighborList):
+    """check_neighbor_flags (src/neighbors.jl:282-290): warn, then reset."""
+    f = nl.flags
+    if f[0] != 0:
+        warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
+    if f[1] != 0:
+        warnings.warn("neighbour-list rebuild interval too long (a displacement exceeded "
+                      "skin*h); reduce interval or increase skin")
+    L.call("sph_flags_clear", nl._st._h)
+    return f
+
+
+def _state_of(obj):
+    if isinstance(obj, State):
+        return obj
+    if isinstance(obj, (CellList, NeighborList)):
+        return obj._st
+    raise TypeError("expected a State, CellList or NeighborList of this backend")
+
+
+def build_(cl, X=None, p=None, backend=None):
+    """build!(cl, X, p, backend): rebuild the cell list from the state's X."""
+    st = _state_of(cl)
+    L.call("sph_build", st._h)
+    return cl
+
+
+def build_neighbors_(nl, cl=None, X=None, p=None, backend=None):
+    """build_neighbors!(nl, cl, X, p, backend): cell list + neighbour list; age = 0."""
+    st = _state_of(nl)
+    L.call("sph_build_neighbors", st._h)
+    return nl
+
+
+def step_(st: State, theta, p: SPHParams = None, backend: MI355XBackend = None):
+    f = _field(theta, st.p, st.backend)
+    L.call("sph_step", st._h, f._h)
+    return st
+
+
+class Tape:
+    """Tape (src/driver.jl:45-83): the state before every step, on the device."""
+
+    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, precision="f64"):
+        self.backend, self.N = backend, int(N)
+        h = C.c_void_p()
+        L.call("sph_tape_create", backend._h, self.N, int(nsteps), _PREC[precision], C.byref(h))
+        self._h = h
+
+    def __len__(self):
+        n = C.c_int64()
+        L.call("sph_tape_length", self._h, C.byref(n))
+        return int(n.value)
+
+    def capacity(self):
+        n = C.c_int64()
+        L.call("sph_tape_capacity", self._h, C.byref(n))
+        return int(n.value)
+
+    def get(self, k):
+        """(tape.X[:, :, k], tape.V[:, :, k]) for 1-based k."""
+        x = np.empty((self.N, 2))
+        v = np.empty((self.N, 2))
+        L.call("sph_tape_get", self._h, int(k), _ptr(x), _ptr(v))
+        return np.ascontiguousarray(x.T), np.ascontiguousarray(v.T)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_tape_destroy(h)
+            self._h = None
+
+
+def reset_(tape: Tape):
+    L.call("sph_tape_reset", tape._h)
+    return tape
+
+
+def capacity(tape: Tape):
+    return tape.capacity()
+
+
+def simulate_(st: State, theta, p: SPHParams, backend: MI355XBackend, nsteps, tape: Tape = None):
+    """simulate!: nsteps steps; with a tape, the state before each step is kept."""
+    f = _field(theta, st.p, st.backend)
+    L.call("sph_simulate", st._h, f._h, int(nsteps), tape._h if tape is not None else None)
+    check_neighbor_flags(st.nl)
+    return st
+
+
+class AdjointWorkspace:
+    """AdjointWorkspace (src/driver.jl:117-133): gX, gV, gtheta after backward_."""
+
+    def __init__(self, backend: MI355XBackend, N, p: SPHParams, precision="f64"):
+        self.backend, self.N, self.p = backend, int(N), p
+        cp = p.c_struct()
+        h = C.c_void_p()
+        L.call("sph_workspace_create", backend._h, self.N, C.byref(cp), _PREC[precision], C.byref(h))
+        self._h = h
+
+    def _get(self, which):
+        if which == L.SPH_GTHETA:
+            out = np.empty((self.p.ngx, self.p.ngy))
+            L.call("sph_workspace_get", self._h, which, _ptr(out))
+            return np.ascontiguousarray(out.T)
+        out = np.empty((self.N, 2))
+        L.call("sph_workspace_get", self._h, which, _ptr(out))
+        return np.ascontiguousarray(out.T)
+
+    gX = property(lambda s: s._get(L.SPH_GX))
+    gV = property(lambda s: s._get(L.SPH_GV))
+    gtheta = property(lambda s: s._get(L.SPH_GTHETA))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_workspace_destroy(h)
+            self._h = None
+
+
+def backward_(ws: AdjointWorkspace, tape: Tape, theta, p: SPHParams, backend: MI355XBackend, *,
+              seedX, seedV, scratch=None, skin=0.2, interval=4, maxnb=64, layout="auto"):
+    """backward!: reverse sweep over the tape; fills ws.gX, ws.gV, ws.gtheta."""
+    if layout == "auto":
+        layout = "slot"
+    f = _field(theta, ws.p, ws.backend)
+    sx = _as_2n(seedX, ws.N)
+    sv = _as_2n(seedV, ws.N)
+    L.call("sph_backward", ws._h, tape._h, f._h, _ptr(sx), _ptr(sv), float(skin), int(interval),
+           int(maxnb), L.SPH_LAYOUT_SLOT if layout == "slot" else L.SPH_LAYOUT_PARTICLE)
+    return ws
+
+
+def target_objective(st: State, xc, yc, sigma):
+    """target_objective(st.X, xc, yc, sigma) -> (J, dJ/dX_T of shape (2, N)), on the device."""
+    if not isinstance(st, State):
+        raise TypeError("target_objective takes the State whose X is evaluated on the device")
+    J = C.c_double()
+    seed = np.empty((st.N, 2))
+    L.call("sph_target_objective", st._h, float(xc), float(yc), float(sigma), C.byref(J), _ptr(seed))
+    return J.value, np.ascontiguousarray(seed.T)
+
+
+def profile_enable(backend: MI355XBackend, on: bool = True):
+    """Per-kernel GPU timing with events on the backend's stream (resets counters)."""
+    L.call("sph_profile_enable", backend._h, 1 if on else 0)
+
+
+def profile_stats(backend: MI355XBackend):
+    """{kernel name: (launches, total ms)} since profile_enable."""
+    out = {}
+    k = 0
+    lib = L.load()
+    while True:
+        name = lib.sph_profile_name(k)
+        if name is None:
+            break
+        n = C.c_int64()
+        ms = C.c_double()
+        L.call("sph_profile_get", backend._h, k, C.byref(n), C.byref(ms))
+        out[name.decode()] = (int(n.value), float(ms.value))
+        k += 1
+    return out

@@ -1,0 +1,1071 @@
+// sph_api.cpp -- host side of libsphadj: the C ABI of include/sphadj.h.
+//
+// Maps the reference's drivers (src/driver.jl, src/neighbors.jl:252-290) onto
+// one in-order HIP stream per context.  Host logic that the reference keeps on
+// the host (rebuild age counter, tape capacity, flag reporting) stays here;
+// everything per particle runs in the kernels of sph_cells.hip / sph_f64.hip.
+#include "sphadj.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "sph_internal.h"
+
+using namespace sph;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(SPH_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return SPH_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return SPH_E_NOMEM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return SPH_E_HIP;
+  }
+}
+
+template <class T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) fail(SPH_E_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return static_cast<T *>(p);
+}
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+constexpr int64_t kAgeNever = INT64_MAX;
+
+// Derived FP64 constants with the reference's operation order in T = Float64.
+PhysF64 make_phys(const sph_params &p) {
+  PhysF64 P{};
+  const double h = p.h;
+  P.m = p.m;
+  P.rho0 = p.rho0;
+  P.c2 = p.c * p.c;                                   // p.c^2
+  P.mu = p.mu;
+  P.gx = p.gx;
+  P.gy = p.gy;
+  P.dt = p.dt;
+  P.Lx = p.Lx;
+  P.Ly = p.Ly;
+  P.kw = p.kw;
+  P.A = 7.0 / ((4.0 * M_PI) * (h * h));               // wnorm (src/kernels.jl:15)
+  P.invh = 1.0 / h;
+  P.r2max = (4.0 * h) * h;                            // T(4)*h*h
+  P.r2min = (DBL_EPSILON * h) * h;                    // eps(T)*p.h*p.h
+  P.Fc = ((-5.0 * P.A) * P.invh) * P.invh;            // src/forward.jl:87
+  P.Gc = ((7.5 * P.A) * P.invh) * P.invh;             // src/adjoint.jl:44
+  P.twomumass = (2.0 * p.m) * p.mu;
+  P.hx = p.Lx / (double)(p.ngx - 1);                  // dxg (src/params.jl:31)
+  P.hy = p.Ly / (double)(p.ngy - 1);
+  P.gxmax = (double)(p.ngx - 1);
+  P.gymax = (double)(p.ngy - 1);
+  P.gxclamp = (double)(p.ngx - 1) - 1e-6;
+  P.gyclamp = (double)(p.ngy - 1) - 1e-6;
+  P.ngx = (int)p.ngx;
+  P.ngy = (int)p.ngy;
+  return P;
+}
+
+void check_params(const sph_params *p) {
+  if (!p) fail(SPH_E_ARG, "params is NULL");
+  if (!(p->h > 0) || !(p->dt > 0) || !(p->Lx > 0) || !(p->Ly > 0))
+    fail(SPH_E_ARG, "params: h, dt, Lx, Ly must be positive");
+  if (p->ngx < 2 || p->ngy < 2) fail(SPH_E_ARG, "params: ngx, ngy must be >= 2");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+// Kernels timed by the profiler (sph_profile_*).
+enum KernelId {
+  K_ASSIGN, K_SCAN, K_FILL, K_RANK, K_PERMUTE, K_DENSITY, K_FORCE, K_ADJ_R, K_ADJ_P1, K_ADJ_P2,
+  K_REMAP, K_COUNT
+};
+const char *const kKernelNames[K_COUNT] = {
+  "cell_assign", "cell_scan", "cell_fill", "cell_rank", "slot_permute", "density", "force_integrate",
+  "adjoint_r", "adjoint_p1", "adjoint_p2", "layout_remap"};
+
+struct ProfRec {
+  int kid;
+  hipEvent_t a, b;
+};
+
+struct sph_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // optional per-kernel timing with events on this stream
+  bool prof = false;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> evpool;
+  double tot_ms[K_COUNT] = {};
+  int64_t launches[K_COUNT] = {};
+};
+
+namespace {
+
+hipEvent_t prof_event(sph_ctx *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPC(hipEventCreate(&e));
+  return e;
+}
+
+// Runs f (one or more launches on c->stream); with profiling on, brackets it
+// with events and books the interval under kernel id kid.
+template <class F>
+void timed(sph_ctx *c, int kid, F &&f) {
+  if (!c->prof) {
+    f();
+    return;
+  }
+  hipEvent_t a = prof_event(c), b = prof_event(c);
+  HIPC(hipEventRecord(a, c->stream));
+  f();
+  HIPC(hipEventRecord(b, c->stream));
+  c->pending.push_back({kid, a, b});
+}
+
+// after a stream synchronisation
+void prof_collect(sph_ctx *c) {
+  for (auto &r : c->pending) {
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, r.a, r.b));
+    c->tot_ms[r.kid] += ms;
+    c->launches[r.kid] += 1;
+    c->evpool.push_back(r.a);
+    c->evpool.push_back(r.b);
+  }
+  c->pending.clear();
+}
+
+}  // namespace
+
+struct sph_field {
+  sph_ctx *ctx = nullptr;
+  int64_t ngy = 0, ngx = 0;
+  double *d = nullptr;
+};
+
+struct sph_state {
+  sph_ctx *ctx = nullptr;
+  sph_params p{};
+  int precision = SPH_F64;
+  int64_t N = 0;
+  Grid g{};
+  PhysF64 P{};
+  CellConstF64 kc{};
+  double skin = 0.2;
+  int64_t interval = 4, maxnb = 64, sm = 0, si = 0;
+  int layout = SPH_LAYOUT_SLOT;
+  int64_t age = kAgeNever;
+  uint64_t epoch = 0;  // serial number of the current slot layout
+  // current layout
+  int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
+  double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
+  double2 *Xref = nullptr, *Xref2 = nullptr;
+  double2 *PI = nullptr;
+  // build! buffers (cell list of the last build)
+  int *cellof = nullptr, *counts = nullptr, *cl_cstart = nullptr, *cursor = nullptr;
+  int *tmp_id = nullptr, *tmp_slot = nullptr, *perm = nullptr, *new_ids = nullptr;
+  int *bsum = nullptr;
+  int cl_view = 0;  // where the last build's (starts, order) live: 0 none, 1 cl buffers, 2 layout
+  int *flags = nullptr;
+  // auxiliary outputs: 0 = zeros (no step yet), 1 = recomputable from the
+  // pre-step buffers (Xn, Vn) of the last step, 2 = materialised in id order
+  int aux_state = 0;
+  double2 *a_id = nullptr, *pi_id = nullptr, *a_slot = nullptr;
+  double *alpha_id = nullptr, *rho_id = nullptr, *alpha_slot = nullptr, *rho_slot = nullptr;
+  double *theta_last = nullptr;  // copy of the design field of the last step (ngy*ngx)
+  // materialised neighbour list (built on request)
+  int *nl_idx = nullptr, *nl_cnt = nullptr;
+  // staging
+  double2 *tmp2 = nullptr, *seed_slot = nullptr;
+  double *tmp1 = nullptr;
+  double *obj_part = nullptr, *obj_J = nullptr;
+  int obj_nb = 0;
+};
+
+struct TapeEpoch {
+  uint64_t serial = 0;
+  const sph_state *owner = nullptr;
+  int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
+};
+
+struct sph_tape {
+  sph_ctx *ctx = nullptr;
+  int64_t N = 0, cap = 0, n = 0;
+  int precision = SPH_F64;
+  double2 *X = nullptr, *V = nullptr;  // (cap+1) entries of N; entry k (1-based) at (k-1)*N
+  std::vector<int> entry_epoch;         // epoch record of each entry (cap+1)
+  std::vector<TapeEpoch> epochs;        // records in use
+  std::vector<TapeEpoch> pool;          // retained allocations
+  bool has_geom = false;
+  Grid g{};
+  PhysF64 P{};
+  int ncell = 0;
+};
+
+struct sph_workspace {
+  sph_ctx *ctx = nullptr;
+  int64_t N = 0;
+  sph_params p{};
+  int precision = SPH_F64;
+  double2 *gX = nullptr, *gV = nullptr, *gXs = nullptr, *abar = nullptr, *PI = nullptr;
+  double2 *Vtmp = nullptr, *t2 = nullptr, *gX_id = nullptr, *gV_id = nullptr;
+  double *galpha = nullptr, *grho = nullptr, *gtheta = nullptr;
+  int *inv = nullptr;
+};
+
+namespace {
+
+void state_free(sph_state *st) {
+  int **ints[] = {&st->ids, &st->scell, &st->cstart, &st->cellof, &st->counts, &st->cl_cstart,
+                  &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
+                  &st->flags, &st->nl_idx, &st->nl_cnt};
+  for (auto pp : ints) dfree(*pp);
+  double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
+                    &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
+  for (auto pp : d2) dfree(*pp);
+  double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
+                   &st->tmp1, &st->obj_part, &st->obj_J};
+  for (auto pp : d1) dfree(*pp);
+}
+
+// build! (src/neighbors.jl:97-114): cell list of the current X into the cl buffers.
+void do_build(sph_state *st) {
+  hipStream_t s = st->ctx->stream;
+  const int N = (int)st->N;
+  sph_ctx *c = st->ctx;
+  timed(c, K_ASSIGN, [&] {
+    launch_fill_i32(s, st->counts, st->g.ncell, 0);
+    launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts);
+  });
+  timed(c, K_SCAN, [&] {
+    launch_scan_excl(s, st->counts, st->g.ncell, st->cl_cstart, st->cursor, st->bsum);
+  });
+  timed(c, K_FILL, [&] {
+    launch_fill(s, st->cellof, st->ids, N, st->cursor, st->tmp_id, st->tmp_slot);
+  });
+  timed(c, K_RANK, [&] {
+    launch_rank(s, st->cellof, st->cl_cstart, st->tmp_id, st->tmp_slot, N, st->perm, st->new_ids);
+  });
+  st->cl_view = 1;  // the cell list of this build is (cl_cstart, new_ids)
+}
+
+// st.a, st.alpha, st.rho, st.pterm, st.invrho of the last step, recomputed
+// from that step's input state (still held in the spare buffers Xn, Vn) and
+// stored in particle-id order.
+void materialise_aux(sph_state *st) {
+  if (st->aux_state != 1) return;
+  hipStream_t s = st->ctx->stream;
+  const int N = (int)st->N;
+  launch_density_f64(s, st->Xn, st->scell, st->cstart, st->g, st->P, st->PI, st->rho_slot, nullptr);
+  launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->scell, st->cstart, st->g, st->P,
+                   nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
+  launch_scatter2(s, st->a_slot, st->ids, N, st->a_id);
+  launch_scatter2(s, st->PI, st->ids, N, st->pi_id);
+  launch_scatter1(s, st->alpha_slot, st->ids, N, st->alpha_id);
+  launch_scatter1(s, st->rho_slot, st->ids, N, st->rho_id);
+  st->aux_state = 2;
+}
+
+// build_neighbors! (src/neighbors.jl:252-263): build, then move every slot to
+// its (cell, id) position and start a new layout epoch.
+void do_build_neighbors(sph_state *st) {
+  hipStream_t s = st->ctx->stream;
+  do_build(st);
+  int check = (st->age != kAgeNever) ? 1 : 0;
+  timed(st->ctx, K_PERMUTE, [&] {
+    launch_permute(s, st->perm, st->X, st->V, st->Xref, check, st->g, st->kc, st->Xn, st->Vn,
+                   st->Xref2, st->scell, st->flags);
+  });
+  std::swap(st->X, st->Xn);
+  std::swap(st->V, st->Vn);
+  std::swap(st->Xref, st->Xref2);
+  std::swap(st->ids, st->new_ids);
+  std::swap(st->cstart, st->cl_cstart);
+  st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
+  st->epoch += 1;
+  st->age = 0;
+}
+
+}  // namespace
+
+namespace {
+
+TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
+  if (!tp->epochs.empty() && tp->epochs.back().owner == st && tp->epochs.back().serial == st->epoch)
+    return tp->epochs.back();
+  if (!tp->has_geom) {
+    tp->g = st->g;
+    tp->P = st->P;
+    tp->ncell = st->g.ncell;
+    tp->has_geom = true;
+  } else if (tp->ncell != st->g.ncell || tp->g.nx != st->g.nx) {
+    fail(SPH_E_ARG, "tape already holds steps of a state with another cell grid");
+  }
+  TapeEpoch e;
+  if (!tp->pool.empty()) {
+    e = tp->pool.back();
+    tp->pool.pop_back();
+  } else {
+    e.ids = dalloc<int>(tp->N);
+    e.scell = dalloc<int>(tp->N);
+    e.cstart = dalloc<int>(tp->ncell + 1);
+  }
+  e.owner = st;
+  e.serial = st->epoch;
+  hipStream_t s = st->ctx->stream;
+  HIPC(hipMemcpyAsync(e.ids, st->ids, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
+  HIPC(hipMemcpyAsync(e.scell, st->scell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
+  HIPC(hipMemcpyAsync(e.cstart, st->cstart, sizeof(int) * (tp->ncell + 1), hipMemcpyDeviceToDevice, s));
+  tp->epochs.push_back(e);
+  return tp->epochs.back();
+}
+
+// step! (src/forward.jl:161-173); with a tape, entry k (1-based) receives the
+// state before the step (written by the kernels themselves).
+void do_step(sph_state *st, const double *theta, sph_tape *tp) {
+  hipStream_t s = st->ctx->stream;
+  if (st->age >= st->interval) do_build_neighbors(st);   // maybe_rebuild!
+  st->age += 1;
+  double2 *tX = nullptr, *tV = nullptr;
+  if (tp) {
+    tape_epoch_for(tp, st);
+    tp->n += 1;
+    tp->entry_epoch[tp->n] = (int)tp->epochs.size() - 1;
+    tX = tp->X + (size_t)(tp->n - 1) * tp->N;
+    tV = tp->V + (size_t)(tp->n - 1) * tp->N;
+  }
+  timed(st->ctx, K_DENSITY, [&] {
+    launch_density_f64(s, st->X, st->scell, st->cstart, st->g, st->P, st->PI, nullptr, tX);
+  });
+  timed(st->ctx, K_FORCE, [&] {
+    launch_force_f64(s, st->X, st->V, st->PI, theta, st->scell, st->cstart, st->g, st->P, st->Xn,
+                     st->Vn, tV, nullptr, nullptr);
+  });
+  std::swap(st->X, st->Xn);
+  std::swap(st->V, st->Vn);
+  st->aux_state = 1;
+}
+
+void need_f64(int precision) {
+  if (precision != SPH_F64) fail(SPH_E_ARG, "precision %d is not available for this entry point", precision);
+}
+
+void copy_theta(sph_state *st, const sph_field *th) {
+  if (!th) fail(SPH_E_ARG, "theta is NULL");
+  if (th->ngy != st->p.ngy || th->ngx != st->p.ngx)
+    fail(SPH_E_ARG, "theta is (%lld, %lld) but params want (%lld, %lld)", (long long)th->ngy,
+         (long long)th->ngx, (long long)st->p.ngy, (long long)st->p.ngx);
+  HIPC(hipMemcpyAsync(st->theta_last, th->d, sizeof(double) * th->ngy * th->ngx,
+                      hipMemcpyDeviceToDevice, st->ctx->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sph_last_error(void) { return g_err.c_str(); }
+
+int sph_ctx_create(int device, sph_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(SPH_E_ARG, "out is NULL");
+    int n = 0;
+    HIPC(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(SPH_E_ARG, "device %d not present (%d devices)", device, n);
+    HIPC(hipSetDevice(device));
+    auto *c = new sph_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      hip_check(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+  });
+}
+
+int sph_ctx_destroy(sph_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->evpool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int sph_sync(sph_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) fail(SPH_E_ARG, "ctx is NULL");
+    HIPC(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+  });
+}
+
+int sph_profile_enable(sph_ctx *ctx, int on) {
+  return guarded([&] {
+    if (!ctx) fail(SPH_E_ARG, "ctx is NULL");
+    HIPC(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    ctx->prof = on != 0;
+    for (int k = 0; k < K_COUNT; ++k) {
+      ctx->tot_ms[k] = 0.0;
+      ctx->launches[k] = 0;
+    }
+  });
+}
+
+int sph_profile_get(sph_ctx *ctx, int kernel, int64_t *launches, double *total_ms) {
+  return guarded([&] {
+    if (!ctx || !launches || !total_ms) fail(SPH_E_ARG, "NULL argument");
+    if (kernel < 0 || kernel >= K_COUNT) fail(SPH_E_ARG, "kernel id %d out of range", kernel);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    *launches = ctx->launches[kernel];
+    *total_ms = ctx->tot_ms[kernel];
+  });
+}
+
+const char *sph_profile_name(int kernel) {
+  return (kernel >= 0 && kernel < K_COUNT) ? kKernelNames[kernel] : nullptr;
+}
+
+int sph_field_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, const double *host, sph_field **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    if (ngy < 2 || ngx < 2) fail(SPH_E_ARG, "field must be at least 2x2");
+    HIPC(hipSetDevice(ctx->device));
+    auto *f = new sph_field;
+    f->ctx = ctx;
+    f->ngy = ngy;
+    f->ngx = ngx;
+    f->d = dalloc<double>(ngy * ngx);
+    if (host) HIPC(hipMemcpy(f->d, host, sizeof(double) * ngy * ngx, hipMemcpyHostToDevice));
+    else HIPC(hipMemset(f->d, 0, sizeof(double) * ngy * ngx));
+    *out = f;
+  });
+}
+
+int sph_field_set(sph_field *f, const double *host) {
+  return guarded([&] {
+    if (!f || !host) fail(SPH_E_ARG, "NULL argument");
+    HIPC(hipStreamSynchronize(f->ctx->stream));
+    HIPC(hipMemcpy(f->d, host, sizeof(double) * f->ngy * f->ngx, hipMemcpyHostToDevice));
+  });
+}
+
+int sph_field_get(const sph_field *f, double *host) {
+  return guarded([&] {
+    if (!f || !host) fail(SPH_E_ARG, "NULL argument");
+    HIPC(hipStreamSynchronize(f->ctx->stream));
+    HIPC(hipMemcpy(host, f->d, sizeof(double) * f->ngy * f->ngx, hipMemcpyDeviceToHost));
+  });
+}
+
+int sph_field_destroy(sph_field *f) {
+  return guarded([&] {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    dfree(f->d);
+    delete f;
+  });
+}
+
+int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, const double *X0,
+                     const double *V0, double skin, int64_t interval, int64_t maxnb, int layout,
+                     sph_state **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    check_params(p);
+    need_f64(precision);
+    if (N < 1 || N > (int64_t)1 << 30) fail(SPH_E_ARG, "N=%lld out of range", (long long)N);
+    if (!X0 || !V0) fail(SPH_E_ARG, "X0 and V0 are required");
+    if (!(skin >= 0)) fail(SPH_E_ARG, "skin must be >= 0");
+    if (interval < 1) fail(SPH_E_ARG, "interval must be >= 1");
+    if (maxnb < 1) fail(SPH_E_ARG, "maxnb must be >= 1");
+    if (layout != SPH_LAYOUT_SLOT && layout != SPH_LAYOUT_PARTICLE)
+      fail(SPH_E_ARG, "layout must be :slot or :particle");
+    HIPC(hipSetDevice(ctx->device));
+    auto *st = new sph_state;
+    try {
+      st->ctx = ctx;
+      st->p = *p;
+      st->precision = precision;
+      st->N = N;
+      st->skin = skin;
+      st->interval = interval;
+      st->maxnb = maxnb;
+      st->layout = layout;
+      if (layout == SPH_LAYOUT_SLOT) { st->sm = N; st->si = 1; }
+      else { st->sm = 1; st->si = maxnb; }
+      st->P = make_phys(*p);
+      // NeighborList ctor: rc = Float64(support(p))*(1+skin), dmax = Float64(p.h)*skin;
+      // CellList with cs = rc (src/neighbors.jl:28-36,174-190; src/driver.jl:33-34)
+      const double rc = (2.0 * p->h) * (1.0 + skin);
+      const double dmax = p->h * skin;
+      long long nx = (long long)std::ceil(p->Lx / rc) + 2;
+      long long ny = (long long)std::ceil(p->Ly / rc) + 2;
+      nx = nx < 1 ? 1 : nx;
+      ny = ny < 1 ? 1 : ny;
+      if (nx * ny > (1LL << 30)) fail(SPH_E_ARG, "cell grid %lldx%lld too large", nx, ny);
+      st->g.nx = (int)nx;
+      st->g.ny = (int)ny;
+      st->g.ncell = (int)(nx * ny);
+      st->g.N = (int)N;
+      st->kc.cs = rc;
+      st->kc.invcs = 1.0 / rc;
+      st->kc.rc2 = rc * rc;
+      st->kc.r2min = (DBL_EPSILON * p->h) * p->h;
+      st->kc.dmax2 = dmax * dmax;
+      const size_t n = (size_t)N, nc = (size_t)st->g.ncell;
+      st->ids = dalloc<int>(n);
+      st->scell = dalloc<int>(n);
+      st->cstart = dalloc<int>(nc + 1);
+      st->X = dalloc<double2>(n);
+      st->V = dalloc<double2>(n);
+      st->Xn = dalloc<double2>(n);
+      st->Vn = dalloc<double2>(n);
+      st->Xref = dalloc<double2>(n);
+      st->Xref2 = dalloc<double2>(n);
+      st->PI = dalloc<double2>(n);
+      st->cellof = dalloc<int>(n);
+      st->counts = dalloc<int>(nc);
+      st->cl_cstart = dalloc<int>(nc + 1);
+      st->cursor = dalloc<int>(nc);
+      st->tmp_id = dalloc<int>(n);
+      st->tmp_slot = dalloc<int>(n);
+      st->perm = dalloc<int>(n);
+      st->new_ids = dalloc<int>(n);
+      st->bsum = dalloc<int>(nc / 1024 + 2);
+      st->flags = dalloc<int>(2);
+      st->a_id = dalloc<double2>(n);
+      st->pi_id = dalloc<double2>(n);
+      st->a_slot = dalloc<double2>(n);
+      st->alpha_id = dalloc<double>(n);
+      st->rho_id = dalloc<double>(n);
+      st->alpha_slot = dalloc<double>(n);
+      st->rho_slot = dalloc<double>(n);
+      st->theta_last = dalloc<double>((size_t)(p->ngx * p->ngy));
+      st->tmp2 = dalloc<double2>(n);
+      st->seed_slot = dalloc<double2>(n);
+      st->tmp1 = dalloc<double>(n);
+      st->obj_nb = objective_blocks((int)N);
+      st->obj_part = dalloc<double>(st->obj_nb + 2);
+      st->obj_J = dalloc<double>(1);
+      hipStream_t s = ctx->stream;
+      // State ctor: X, V copied in; a, rho, pterm, invrho, alpha start at zero
+      std::vector<int> iota(n);
+      for (size_t i = 0; i < n; ++i) iota[i] = (int)i;
+      HIPC(hipMemcpy(st->ids, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+      HIPC(hipMemcpy(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice));
+      HIPC(hipMemcpy(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
+      HIPC(hipMemset(st->flags, 0, sizeof(int) * 2));
+      HIPC(hipMemset(st->PI, 0, sizeof(double2) * n));
+      HIPC(hipMemset(st->a_id, 0, sizeof(double2) * n));
+      HIPC(hipMemset(st->pi_id, 0, sizeof(double2) * n));
+      HIPC(hipMemset(st->alpha_id, 0, sizeof(double) * n));
+      HIPC(hipMemset(st->rho_id, 0, sizeof(double) * n));
+      HIPC(hipMemset(st->counts, 0, sizeof(int) * nc));
+      HIPC(hipStreamSynchronize(s));
+    } catch (...) {
+      state_free(st);
+      delete st;
+      throw;
+    }
+    *out = st;
+  });
+}
+
+int sph_state_destroy(sph_state *st) {
+  return guarded([&] {
+    if (!st) return;
+    (void)hipStreamSynchronize(st->ctx->stream);
+    state_free(st);
+    delete st;
+  });
+}
+
+int sph_state_info(const sph_state *st, int64_t *o) {
+  return guarded([&] {
+    if (!st || !o) fail(SPH_E_ARG, "NULL argument");
+    o[0] = st->g.nx; o[1] = st->g.ny; o[2] = st->g.ncell; o[3] = st->maxnb;
+    o[4] = st->sm; o[5] = st->si; o[6] = st->interval; o[7] = st->age;
+  });
+}
+
+int sph_state_get(sph_state *st, int field, double *host) {
+  return guarded([&] {
+    if (!st || !host) fail(SPH_E_ARG, "NULL argument");
+    hipStream_t s = st->ctx->stream;
+    const int N = (int)st->N;
+    const size_t n = (size_t)N;
+    std::vector<double2> buf;
+    switch (field) {
+      case SPH_X:
+      case SPH_V:
+        launch_scatter2(s, field == SPH_X ? st->X : st->V, st->ids, N, st->tmp2);
+        HIPC(hipMemcpyAsync(host, st->tmp2, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        break;
+      case SPH_A:
+        materialise_aux(st);
+        HIPC(hipMemcpyAsync(host, st->a_id, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        break;
+      case SPH_RHO:
+      case SPH_ALPHA:
+        materialise_aux(st);
+        HIPC(hipMemcpyAsync(host, field == SPH_RHO ? st->rho_id : st->alpha_id, sizeof(double) * n,
+                            hipMemcpyDeviceToHost, s));
+        break;
+      case SPH_PTERM:
+      case SPH_INVRHO:
+        materialise_aux(st);
+        buf.resize(n);
+        HIPC(hipMemcpyAsync(buf.data(), st->pi_id, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (size_t i = 0; i < n; ++i) host[i] = field == SPH_PTERM ? buf[i].x : buf[i].y;
+        break;
+      default:
+        fail(SPH_E_ARG, "unknown field %d", field);
+    }
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int sph_state_set(sph_state *st, int field, const double *host) {
+  return guarded([&] {
+    if (!st || !host) fail(SPH_E_ARG, "NULL argument");
+    if (field != SPH_X && field != SPH_V) fail(SPH_E_ARG, "only X and V can be set");
+    hipStream_t s = st->ctx->stream;
+    materialise_aux(st);  // the spare buffers may be reused below
+    HIPC(hipMemcpyAsync(st->tmp2, host, sizeof(double2) * st->N, hipMemcpyHostToDevice, s));
+    launch_gather2(s, st->tmp2, st->ids, (int)st->N, field == SPH_X ? st->X : st->V);
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int sph_build(sph_state *st) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    do_build(st);
+  });
+}
+
+int sph_build_neighbors(sph_state *st) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    materialise_aux(st);
+    do_build_neighbors(st);
+  });
+}
+
+int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *starts,
+                      int32_t *order) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (st->cl_view == 0) fail(SPH_E_STATE, "no cell list has been built");
+    hipStream_t s = st->ctx->stream;
+    const size_t n = (size_t)st->N, nc = (size_t)st->g.ncell;
+    std::vector<int> cnt(nc), stt(nc + 1), ord(n);
+    const int *dstart = st->cl_view == 1 ? st->cl_cstart : st->cstart;
+    const int *dorder = st->cl_view == 1 ? st->new_ids : st->ids;
+    HIPC(hipMemcpyAsync(cnt.data(), st->counts, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(stt.data(), dstart, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(ord.data(), dorder, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (counts) std::memcpy(counts, cnt.data(), sizeof(int) * nc);
+    if (starts) std::memcpy(starts, stt.data(), sizeof(int) * nc);
+    if (order)
+      for (size_t t = 0; t < n; ++t) order[t] = ord[t] + 1;
+    if (cellof)
+      for (size_t c = 0; c < nc; ++c)
+        for (int t = stt[c]; t < stt[c] + cnt[c]; ++t) cellof[ord[t]] = (int)c + 1;
+  });
+}
+
+int sph_neighbors_get(sph_state *st, int32_t *indices, int32_t *counts) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (st->age == kAgeNever) fail(SPH_E_STATE, "the neighbour list has not been built");
+    hipStream_t s = st->ctx->stream;
+    const size_t n = (size_t)st->N;
+    if (!st->nl_idx) {
+      st->nl_idx = dalloc<int>(n * st->maxnb);
+      st->nl_cnt = dalloc<int>(n);
+    }
+    HIPC(hipMemsetAsync(st->nl_idx, 0, sizeof(int) * n * st->maxnb, s));
+    launch_materialise(s, st->Xref, st->ids, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb,
+                       st->sm, st->si, st->nl_idx, st->nl_cnt, st->flags);
+    if (indices)
+      HIPC(hipMemcpyAsync(indices, st->nl_idx, sizeof(int) * n * st->maxnb, hipMemcpyDeviceToHost, s));
+    if (counts) HIPC(hipMemcpyAsync(counts, st->nl_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int sph_flags_get(sph_state *st, int32_t *flags2) {
+  return guarded([&] {
+    if (!st || !flags2) fail(SPH_E_ARG, "NULL argument");
+    HIPC(hipMemcpyAsync(flags2, st->flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st->ctx->stream));
+    HIPC(hipStreamSynchronize(st->ctx->stream));
+  });
+}
+
+int sph_flags_clear(sph_state *st) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    launch_fill_i32(st->ctx->stream, st->flags, 2, 0);
+  });
+}
+
+int sph_step(sph_state *st, const sph_field *theta) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    copy_theta(st, theta);
+    do_step(st, theta->d, nullptr);
+  });
+}
+
+int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape *tape) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (nsteps < 0) fail(SPH_E_ARG, "nsteps must be >= 0");
+    if (tape) {
+      if (tape->N != st->N) fail(SPH_E_ARG, "tape holds N=%lld, state has N=%lld",
+                                 (long long)tape->N, (long long)st->N);
+      if (tape->precision != st->precision) fail(SPH_E_ARG, "tape and state precision differ");
+      if (tape->n + nsteps > tape->cap)
+        fail(SPH_E_CAPACITY, "tape capacity %lld steps, %lld needed; allocate a larger Tape or reset!",
+             (long long)tape->cap, (long long)(tape->n + nsteps));
+    }
+    copy_theta(st, theta);
+    for (int64_t k = 0; k < nsteps; ++k) do_step(st, theta->d, tape);
+    if (tape && nsteps > 0) {
+      // entry n+1 holds the final state: the reverse sweep reads V_{n+1} from it
+      hipStream_t s = st->ctx->stream;
+      const size_t off = (size_t)tape->n * tape->N;
+      HIPC(hipMemcpyAsync(tape->X + off, st->X, sizeof(double2) * tape->N, hipMemcpyDeviceToDevice, s));
+      HIPC(hipMemcpyAsync(tape->V + off, st->V, sizeof(double2) * tape->N, hipMemcpyDeviceToDevice, s));
+      tape->entry_epoch[tape->n + 1] = tape->entry_epoch[tape->n];
+    }
+    HIPC(hipStreamSynchronize(st->ctx->stream));
+    prof_collect(st->ctx);
+  });
+}
+
+int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    need_f64(precision);
+    if (N < 1 || capacity < 0) fail(SPH_E_ARG, "bad tape size");
+    HIPC(hipSetDevice(ctx->device));
+    auto *tp = new sph_tape;
+    try {
+      tp->ctx = ctx;
+      tp->N = N;
+      tp->cap = capacity;
+      tp->precision = precision;
+      const size_t tot = (size_t)(capacity + 1) * (size_t)N;
+      tp->X = dalloc<double2>(tot);
+      tp->V = dalloc<double2>(tot);
+      tp->entry_epoch.assign((size_t)capacity + 2, -1);
+    } catch (...) {
+      dfree(tp->X);
+      dfree(tp->V);
+      delete tp;
+      throw;
+    }
+    *out = tp;
+  });
+}
+
+int sph_tape_destroy(sph_tape *tp) {
+  return guarded([&] {
+    if (!tp) return;
+    (void)hipStreamSynchronize(tp->ctx->stream);
+    for (auto *v : {&tp->epochs, &tp->pool})
+      for (auto &e : *v) {
+        dfree(e.ids);
+        dfree(e.scell);
+        dfree(e.cstart);
+      }
+    dfree(tp->X);
+    dfree(tp->V);
+    delete tp;
+  });
+}
+
+int sph_tape_reset(sph_tape *tp) {
+  return guarded([&] {
+    if (!tp) fail(SPH_E_ARG, "NULL argument");
+    for (auto &e : tp->epochs) tp->pool.push_back(e);
+    tp->epochs.clear();
+    tp->n = 0;
+    std::fill(tp->entry_epoch.begin(), tp->entry_epoch.end(), -1);
+  });
+}
+
+int sph_tape_length(const sph_tape *tp, int64_t *n) {
+  return guarded([&] {
+    if (!tp || !n) fail(SPH_E_ARG, "NULL argument");
+    *n = tp->n;
+  });
+}
+
+int sph_tape_capacity(const sph_tape *tp, int64_t *cap) {
+  return guarded([&] {
+    if (!tp || !cap) fail(SPH_E_ARG, "NULL argument");
+    *cap = tp->cap;
+  });
+}
+
+int sph_tape_get(sph_tape *tp, int64_t k, double *X, double *V) {
+  return guarded([&] {
+    if (!tp) fail(SPH_E_ARG, "NULL argument");
+    if (k < 1 || k > tp->n) fail(SPH_E_ARG, "tape entry %lld outside 1..%lld", (long long)k, (long long)tp->n);
+    hipStream_t s = tp->ctx->stream;
+    const TapeEpoch &e = tp->epochs[tp->entry_epoch[k]];
+    const size_t off = (size_t)(k - 1) * tp->N;
+    double2 *tmp = dalloc<double2>(tp->N);
+    try {
+      if (X) {
+        launch_scatter2(s, tp->X + off, e.ids, (int)tp->N, tmp);
+        HIPC(hipMemcpyAsync(X, tmp, sizeof(double2) * tp->N, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+      }
+      if (V) {
+        launch_scatter2(s, tp->V + off, e.ids, (int)tp->N, tmp);
+        HIPC(hipMemcpyAsync(V, tmp, sizeof(double2) * tp->N, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+      }
+    } catch (...) {
+      dfree(tmp);
+      throw;
+    }
+    dfree(tmp);
+  });
+}
+
+int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
+                         sph_workspace **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    check_params(p);
+    need_f64(precision);
+    if (N < 1) fail(SPH_E_ARG, "bad N");
+    HIPC(hipSetDevice(ctx->device));
+    auto *ws = new sph_workspace;
+    try {
+      ws->ctx = ctx;
+      ws->N = N;
+      ws->p = *p;
+      ws->precision = precision;
+      const size_t n = (size_t)N;
+      ws->gX = dalloc<double2>(n);
+      ws->gV = dalloc<double2>(n);
+      ws->gXs = dalloc<double2>(n);
+      ws->abar = dalloc<double2>(n);
+      ws->PI = dalloc<double2>(n);
+      ws->Vtmp = dalloc<double2>(n);
+      ws->t2 = dalloc<double2>(n);
+      ws->gX_id = dalloc<double2>(n);
+      ws->gV_id = dalloc<double2>(n);
+      ws->galpha = dalloc<double>(n);
+      ws->grho = dalloc<double>(n);
+      ws->gtheta = dalloc<double>((size_t)(p->ngx * p->ngy));
+      ws->inv = dalloc<int>(n);
+      HIPC(hipMemset(ws->gX_id, 0, sizeof(double2) * n));
+      HIPC(hipMemset(ws->gV_id, 0, sizeof(double2) * n));
+      HIPC(hipMemset(ws->gtheta, 0, sizeof(double) * p->ngx * p->ngy));
+    } catch (...) {
+      double2 **d2[] = {&ws->gX, &ws->gV, &ws->gXs, &ws->abar, &ws->PI, &ws->Vtmp, &ws->t2,
+                        &ws->gX_id, &ws->gV_id};
+      for (auto pp : d2) dfree(*pp);
+      dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
+      delete ws;
+      throw;
+    }
+    *out = ws;
+  });
+}
+
+int sph_workspace_destroy(sph_workspace *ws) {
+  return guarded([&] {
+    if (!ws) return;
+    (void)hipStreamSynchronize(ws->ctx->stream);
+    double2 **d2[] = {&ws->gX, &ws->gV, &ws->gXs, &ws->abar, &ws->PI, &ws->Vtmp, &ws->t2,
+                      &ws->gX_id, &ws->gV_id};
+    for (auto pp : d2) dfree(*pp);
+    dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
+    delete ws;
+  });
+}
+
+int sph_workspace_get(sph_workspace *ws, int which, double *host) {
+  return guarded([&] {
+    if (!ws || !host) fail(SPH_E_ARG, "NULL argument");
+    hipStream_t s = ws->ctx->stream;
+    switch (which) {
+      case SPH_GX:
+        HIPC(hipMemcpyAsync(host, ws->gX_id, sizeof(double2) * ws->N, hipMemcpyDeviceToHost, s));
+        break;
+      case SPH_GV:
+        HIPC(hipMemcpyAsync(host, ws->gV_id, sizeof(double2) * ws->N, hipMemcpyDeviceToHost, s));
+        break;
+      case SPH_GTHETA:
+        HIPC(hipMemcpyAsync(host, ws->gtheta, sizeof(double) * ws->p.ngx * ws->p.ngy,
+                            hipMemcpyDeviceToHost, s));
+        break;
+      default:
+        fail(SPH_E_ARG, "unknown gradient %d", which);
+    }
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+// backward! (src/driver.jl:141-207).  The reverse sweep reuses the neighbour
+// epochs the tape recorded: the reference's own comments note that any
+// rebuild schedule gives the same interaction set while displacements stay
+// within skin*h (src/neighbors.jl:133-144; src/driver.jl:162-167), and reusing
+// the forward's keeps the recomputed densities bit-identical to the forward's.
+// skin, interval, maxnb and layout are validated for interface parity.
+int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
+                 const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout) {
+  return guarded([&] {
+    if (!ws || !tape || !theta || !seedX || !seedV) fail(SPH_E_ARG, "NULL argument");
+    if (tape->N != ws->N) fail(SPH_E_ARG, "tape and workspace N differ");
+    if (tape->precision != ws->precision) fail(SPH_E_ARG, "tape and workspace precision differ");
+    if (theta->ngy != ws->p.ngy || theta->ngx != ws->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
+    if (!(skin >= 0) || interval < 1 || maxnb < 1) fail(SPH_E_ARG, "bad neighbour-list settings");
+    if (layout != SPH_LAYOUT_SLOT && layout != SPH_LAYOUT_PARTICLE) fail(SPH_E_ARG, "bad layout");
+    hipStream_t s = ws->ctx->stream;
+    const int N = (int)ws->N;
+    const size_t n = (size_t)N;
+    const PhysF64 P = make_phys(ws->p);
+    HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
+    const int64_t L = tape->n;
+    if (L == 0) {
+      HIPC(hipMemcpyAsync(ws->gX_id, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(ws->gV_id, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+      HIPC(hipStreamSynchronize(s));
+      return;
+    }
+    const Grid g = tape->g;
+    int cur = tape->entry_epoch[L];
+    // seeds (id order) into the layout of the final entry
+    HIPC(hipMemcpyAsync(ws->t2, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    launch_gather2(s, ws->t2, tape->epochs[cur].ids, N, ws->gX);
+    HIPC(hipStreamSynchronize(s));
+    HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    launch_gather2(s, ws->t2, tape->epochs[cur].ids, N, ws->gV);
+    for (int64_t k = L; k >= 1; --k) {
+      const int e = tape->entry_epoch[k];
+      const TapeEpoch &E = tape->epochs[e];
+      const double2 *Xk = tape->X + (size_t)(k - 1) * n;
+      const double2 *Vk = tape->V + (size_t)(k - 1) * n;
+      const double2 *Vnext = tape->V + (size_t)k * n;
+      if (e != cur) {
+        // entry k+1 was stored in the next epoch's slot order: move the
+        // adjoint state and V_{k+1} into this epoch's order
+        timed(ws->ctx, K_REMAP, [&] {
+          launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
+          launch_gather_remap2(s, ws->gX, ws->inv, E.ids, N, ws->t2);
+          std::swap(ws->gX, ws->t2);
+          launch_gather_remap2(s, ws->gV, ws->inv, E.ids, N, ws->t2);
+          std::swap(ws->gV, ws->t2);
+          launch_gather_remap2(s, Vnext, ws->inv, E.ids, N, ws->Vtmp);
+        });
+        Vnext = ws->Vtmp;
+        cur = e;
+      }
+      sph_ctx *c = ws->ctx;
+      timed(c, K_ADJ_R, [&] {
+        launch_adj_r_f64(s, Xk, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI, ws->gX, ws->gV,
+                         ws->abar, ws->galpha);
+      });
+      timed(c, K_ADJ_P1, [&] {
+        launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, E.scell, E.cstart, g,
+                          P, ws->gXs, ws->gV, ws->grho, ws->gtheta);
+      });
+      timed(c, K_ADJ_P2, [&] {
+        launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, E.scell, E.cstart, g, P, ws->gX);
+      });
+    }
+    launch_scatter2(s, ws->gX, tape->epochs[cur].ids, N, ws->gX_id);
+    launch_scatter2(s, ws->gV, tape->epochs[cur].ids, N, ws->gV_id);
+    HIPC(hipStreamSynchronize(s));
+    prof_collect(ws->ctx);
+  });
+}
+
+int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
+                         double *seed) {
+  return guarded([&] {
+    if (!st || !J) fail(SPH_E_ARG, "NULL argument");
+    hipStream_t s = st->ctx->stream;
+    const int N = (int)st->N;
+    launch_objective_f64(s, st->X, N, xc, yc, sigma, st->obj_part, st->obj_nb, st->obj_J,
+                         seed ? st->seed_slot : nullptr);
+    HIPC(hipMemcpyAsync(J, st->obj_J, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (seed) {
+      launch_scatter2(s, st->seed_slot, st->ids, N, st->tmp2);
+      HIPC(hipMemcpyAsync(seed, st->tmp2, sizeof(double2) * N, hipMemcpyDeviceToHost, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,326 @@
+// sph_cells.hip -- cell list, slot reordering and layout remaps for gfx950.
+//
+// Rebuild pipeline (the reference's build! + _nl_build!, src/neighbors.jl:97-114,192-245):
+//   assign   cell of every slot by floor(x/cs) (a division, as cell_index does),
+//            counts by wave-aggregated atomics (one atomic per run of equal cells)
+//   scan     exclusive prefix of counts -> cstart (LDS block scan, shuffle wave scan)
+//   fill     slot -> position in its cell (wave-aggregated atomic cursor)
+//   rank     rank inside the cell by particle id: the result equals the reference's
+//            _cl_sort_cells! insertion sort, without a serial per-cell loop
+//   permute  gather X, V into the new slot order, check displacement against Xref,
+//            record the sweep cell by floor(x*(1/cs)) (a multiplication, as _nl_build! does)
+// All of it is integer/byte work bound by HBM and atomics; no MFMA.
+#include "sph_internal.h"
+
+namespace sph {
+
+namespace {
+
+inline int nblocks(long long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+__global__ void k_fill_i32(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Run heads of equal keys within a wavefront.  Lanes whose key is -1 are
+// inactive.  Returns the ballot of heads; `head_lane` is the head of this
+// lane's run and `run_len` is set on heads.
+__device__ __forceinline__ unsigned long long wave_runs(int key, int &head_lane, int &run_len) {
+  const int lane = threadIdx.x & 63;
+  int prev = __shfl_up(key, 1, 64);
+  bool head = (lane == 0) || (prev != key);
+  unsigned long long heads = __ballot(head);
+  // next head strictly above this lane (or 64)
+  unsigned long long above = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+  int next = above ? __ffsll((long long)above) - 1 : 64;
+  unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  head_lane = 63 - __clzll((long long)below);
+  run_len = next - lane;
+  return heads;
+}
+
+__global__ void k_assign(const double2 *__restrict__ X, Grid g, double cs, int *__restrict__ cellof,
+                         int *__restrict__ counts) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = -1;
+  if (s < g.N) {
+    double2 x = X[s];
+    int cx = cell_coord_div(x.x / cs, g.nx);
+    int cy = cell_coord_div(x.y / cs, g.ny);
+    c = cx + cy * g.nx;
+    cellof[s] = c;
+  }
+  int head_lane, run_len;
+  wave_runs(c, head_lane, run_len);
+  const int lane = threadIdx.x & 63;
+  if (c >= 0 && head_lane == lane) atomicAdd(&counts[c], run_len);
+}
+
+// ---- exclusive scan: 256 threads x 4 items per block ---------------------------
+
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and sets `total`.
+__device__ __forceinline__ int block_excl_scan(int v, int &total) {
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    int ws = wsum[w];
+    off += (w < wid) ? ws : 0;
+    tot += ws;
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
+
+__global__ void k_scan_partial(const int *__restrict__ in, int n, int *__restrict__ bsum) {
+  const int base = blockIdx.x * kScanTile;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + threadIdx.x * kScanItems + k;
+    v += (i < n) ? in[i] : 0;
+  }
+  int total;
+  block_excl_scan(v, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// one block scans the block sums in place (exclusive), chunk by chunk
+__global__ void k_scan_bsum(int *__restrict__ bsum, int nb) {
+  int carry = 0;
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    int v = (i < nb) ? bsum[i] : 0;
+    int total;
+    int ex = block_excl_scan(v, total);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void k_scan_final(const int *__restrict__ in, int n, const int *__restrict__ bsum,
+                             int *__restrict__ out, int *__restrict__ cursor) {
+  const int base = blockIdx.x * kScanTile;
+  int vals[kScanItems];
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + threadIdx.x * kScanItems + k;
+    vals[k] = (i < n) ? in[i] : 0;
+    v += vals[k];
+  }
+  int total;
+  int ex = block_excl_scan(v, total) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int i = base + threadIdx.x * kScanItems + k;
+    if (i < n) {
+      out[i] = ex;
+      if (cursor) cursor[i] = ex;
+    }
+    ex += vals[k];
+    if (i == n - 1) out[n] = ex;
+  }
+}
+
+__global__ void k_fill(const int *__restrict__ cellof, const int *__restrict__ ids, int N,
+                       int *__restrict__ cursor, int *__restrict__ tmp_id,
+                       int *__restrict__ tmp_slot) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = (s < N) ? cellof[s] : -1;
+  int head_lane, run_len;
+  wave_runs(c, head_lane, run_len);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (c >= 0 && head_lane == lane) base = atomicAdd(&cursor[c], run_len);
+  base = __shfl(base, head_lane, 64);
+  if (c >= 0) {
+    int pos = base + (lane - head_lane);
+    tmp_id[pos] = ids[s];
+    tmp_slot[pos] = s;
+  }
+}
+
+__global__ void k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
+                       const int *__restrict__ tmp_id, const int *__restrict__ tmp_slot, int N,
+                       int *__restrict__ perm, int *__restrict__ new_ids) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  int so = tmp_slot[p];
+  int id = tmp_id[p];
+  int c = cellof[so];
+  int b = cstart[c], e = cstart[c + 1];
+  int rank = 0;
+  for (int q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1 : 0;
+  int t = b + rank;
+  perm[t] = so;
+  new_ids[t] = id;
+}
+
+__global__ void k_permute(const int *__restrict__ perm, const double2 *__restrict__ X,
+                          const double2 *__restrict__ V, const double2 *__restrict__ Xref,
+                          int check, Grid g, CellConstF64 k, double2 *__restrict__ Xn,
+                          double2 *__restrict__ Vn, double2 *__restrict__ Xrefn,
+                          int *__restrict__ scell, int *__restrict__ flags) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int o = perm[t];
+  double2 x = X[o];
+  if (check) {
+    double2 xr = Xref[o];
+    double ddx = x.x - xr.x;
+    double ddy = x.y - xr.y;
+    if (ddx * ddx + ddy * ddy > k.dmax2) flags[1] = 1;
+  }
+  Xn[t] = x;
+  Vn[t] = V[o];
+  Xrefn[t] = x;
+  int cx = cell_coord_i32(x.x * k.invcs, g.nx);
+  int cy = cell_coord_i32(x.y * k.invcs, g.ny);
+  scell[t] = cx + cy * g.nx;
+}
+
+__global__ void k_materialise(const double2 *__restrict__ Xref, const int *__restrict__ ids,
+                              const int *__restrict__ scell, const int *__restrict__ cstart,
+                              Grid g, CellConstF64 k, int maxnb, long long sm, long long si,
+                              int *__restrict__ indices, int *__restrict__ nbcount,
+                              int *__restrict__ flags) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const int id = ids[t];
+  const double2 xi = Xref[t];
+  Runs r = sweep_runs(cstart, scell[t], g.nx, g.ny);
+  int m = 0;
+  for (int w = 0; w < 3; ++w) {
+    for (int j = r.b[w]; j < r.e[w]; ++j) {
+      double2 xj = Xref[j];
+      double dx = xi.x - xj.x;
+      double dy = xi.y - xj.y;
+      double r2 = dx * dx + dy * dy;
+      if (k.r2min < r2 && r2 < k.rc2) {
+        if (m < maxnb) {
+          indices[(long long)m * sm + (long long)id * si] = ids[j] + 1;
+          m += 1;
+        } else {
+          flags[0] = 1;
+        }
+      }
+    }
+  }
+  nbcount[id] = m;
+}
+
+__global__ void k_scatter_i32(const int *__restrict__ src, const int *__restrict__ ids, int N,
+                              int *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[ids[s]] = src[s];
+}
+__global__ void k_inverse(const int *__restrict__ ids, int N, int *__restrict__ inv) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) inv[ids[s]] = s;
+}
+__global__ void k_gather_remap2(const double2 *__restrict__ src, const int *__restrict__ inv_src,
+                                const int *__restrict__ ids_dst, int N, double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[inv_src[ids_dst[s]]];
+}
+__global__ void k_scatter2(const double2 *__restrict__ src, const int *__restrict__ ids, int N,
+                           double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[ids[s]] = src[s];
+}
+__global__ void k_gather2(const double2 *__restrict__ src, const int *__restrict__ ids, int N,
+                          double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[ids[s]];
+}
+__global__ void k_scatter1(const double *__restrict__ src, const int *__restrict__ ids, int N,
+                           double *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[ids[s]] = src[s];
+}
+__global__ void k_copy2(const double2 *__restrict__ src, int N, double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[s];
+}
+
+}  // namespace
+
+void launch_fill_i32(hipStream_t s, int *p, int n, int v) {
+  if (n > 0) k_fill_i32<<<nblocks(n), kBlock, 0, s>>>(p, n, v);
+}
+void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts) {
+  if (g.N > 0) k_assign<<<nblocks(g.N), kBlock, 0, s>>>(X, g, cs, cellof, counts);
+}
+void launch_scan_excl(hipStream_t s, const int *in, int n, int *out, int *cursor, int *bsum) {
+  int nb = (n + kScanTile - 1) / kScanTile;
+  k_scan_partial<<<nb, kBlock, 0, s>>>(in, n, bsum);
+  k_scan_bsum<<<1, kBlock, 0, s>>>(bsum, nb);
+  k_scan_final<<<nb, kBlock, 0, s>>>(in, n, bsum, out, cursor);
+}
+void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
+                 int *tmp_id, int *tmp_slot) {
+  if (N > 0) k_fill<<<nblocks(N), kBlock, 0, s>>>(cellof, ids, N, cursor, tmp_id, tmp_slot);
+}
+void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
+                 const int *tmp_slot, int N, int *perm, int *new_ids) {
+  if (N > 0) k_rank<<<nblocks(N), kBlock, 0, s>>>(cellof, cstart, tmp_id, tmp_slot, N, perm, new_ids);
+}
+void launch_permute(hipStream_t s, const int *perm, const double2 *X, const double2 *V,
+                    const double2 *Xref, int check, Grid g, CellConstF64 k, double2 *Xn,
+                    double2 *Vn, double2 *Xrefn, int *scell, int *flags) {
+  if (g.N > 0)
+    k_permute<<<nblocks(g.N), kBlock, 0, s>>>(perm, X, V, Xref, check, g, k, Xn, Vn, Xrefn, scell,
+                                              flags);
+}
+void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, const int *scell,
+                        const int *cstart, Grid g, CellConstF64 k, int maxnb, long long sm,
+                        long long si, int *indices, int *nbcount, int *flags) {
+  if (g.N > 0)
+    k_materialise<<<nblocks(g.N), kBlock, 0, s>>>(Xref, ids, scell, cstart, g, k, maxnb, sm, si,
+                                                  indices, nbcount, flags);
+}
+void launch_scatter_i32(hipStream_t s, const int *src, const int *ids, int N, int *dst) {
+  if (N > 0) k_scatter_i32<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_inverse(hipStream_t s, const int *ids, int N, int *inv) {
+  if (N > 0) k_inverse<<<nblocks(N), kBlock, 0, s>>>(ids, N, inv);
+}
+void launch_gather_remap2(hipStream_t s, const double2 *src, const int *inv_src,
+                          const int *ids_dst, int N, double2 *dst) {
+  if (N > 0) k_gather_remap2<<<nblocks(N), kBlock, 0, s>>>(src, inv_src, ids_dst, N, dst);
+}
+void launch_scatter2(hipStream_t s, const double2 *src, const int *ids, int N, double2 *dst) {
+  if (N > 0) k_scatter2<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_gather2(hipStream_t s, const double2 *src, const int *ids, int N, double2 *dst) {
+  if (N > 0) k_gather2<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_scatter1(hipStream_t s, const double *src, const int *ids, int N, double *dst) {
+  if (N > 0) k_scatter1<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_copy2(hipStream_t s, const double2 *src, int N, double2 *dst) {
+  if (N > 0) k_copy2<<<nblocks(N), kBlock, 0, s>>>(src, N, dst);
+}
+
+}  // namespace sph

@@ -1,0 +1,383 @@
+// sph_f64.hip -- plain FP64 physics kernels for gfx950.
+//
+// Forward (src/forward.jl) and gather-form adjoint (src/adjoint.jl) over the
+// slot layout: each thread owns one slot and sweeps the three contiguous slot
+// runs of its 3x3 cell block, in the reference's neighbour order (ddy outer,
+// ddx inner, ascending particle id inside a cell).  That is exactly the order
+// of the reference's neighbour list, so every sum is accumulated in the same
+// order with the same operations.  This file is compiled with
+// -ffp-contract=off: a*b+c stays two rounded operations as in Julia, and the
+// FP64 results equal the reference's CPU Float64 arithmetic.
+//
+// Kernels per step:
+//   forward:  density (pterm, invrho)         -> force + implicit-drag integrate
+//   adjoint:  R  (density recompute, integrate and drag adjoints)
+//             P1 (pair adjoint, design term and theta-bar atomics)
+//             P2 (density adjoint, accumulate gX)
+// All are HBM/L2-bound gathers with ~10 flop/B; there is no dense contraction.
+#pragma clang fp contract(off)
+#include "sph_internal.h"
+
+namespace sph {
+
+namespace {
+
+inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ double wall_accel(double x, double L, double kw) {
+  double lo = (-x > 0.0) ? -x : 0.0;
+  double hi = (x - L > 0.0) ? x - L : 0.0;
+  return kw * (lo * lo - hi * hi);
+}
+__device__ __forceinline__ double wall_daccel(double x, double L, double kw) {
+  double lo = (-x > 0.0) ? -x : 0.0;
+  double hi = (x - L > 0.0) ? x - L : 0.0;
+  return kw * (-2.0 * lo - 2.0 * hi);
+}
+
+struct Interp {
+  int l00;            // 0-based linear index of theta[j, i] (column-major, ny rows)
+  double fx, fy, ax, ay;
+};
+
+// interp_cell (src/params.jl:53-68)
+__device__ __forceinline__ Interp interp_cell(double x, double y, const PhysF64 &P) {
+  Interp r;
+  double gx = x / P.hx;
+  double gy = y / P.hy;
+  r.ax = (gx > 0.0 && gx < P.gxmax) ? 1.0 : 0.0;
+  r.ay = (gy > 0.0 && gy < P.gymax) ? 1.0 : 0.0;
+  double gxc = (gx > P.gxclamp) ? P.gxclamp : ((gx < 0.0) ? 0.0 : gx);
+  double gyc = (gy > P.gyclamp) ? P.gyclamp : ((gy < 0.0) ? 0.0 : gy);
+  double fxf = floor(gxc), fyf = floor(gyc);
+  long long i0 = (fxf > -1e15 && fxf < 1e15) ? (long long)fxf : -(1LL << 60);
+  long long j0 = (fyf > -1e15 && fyf < 1e15) ? (long long)fyf : -(1LL << 60);
+  i0 = i0 < 0 ? 0 : (i0 > P.ngx - 2 ? P.ngx - 2 : i0);
+  j0 = j0 < 0 ? 0 : (j0 > P.ngy - 2 ? P.ngy - 2 : j0);
+  r.fx = gxc - (double)i0;
+  r.fy = gyc - (double)j0;
+  r.l00 = (int)(j0 + i0 * P.ngy);
+  return r;
+}
+
+// interp_alpha (src/params.jl:70-82)
+__device__ __forceinline__ double interp_alpha(const double *__restrict__ th, const Interp &c,
+                                               const PhysF64 &P, double *dadx, double *dady) {
+  double t00 = th[c.l00];
+  double t01 = th[c.l00 + P.ngy];
+  double t10 = th[c.l00 + 1];
+  double t11 = th[c.l00 + P.ngy + 1];
+  double omx = 1.0 - c.fx, omy = 1.0 - c.fy;
+  double a = (((omx * omy) * t00 + (c.fx * omy) * t01) + (omx * c.fy) * t10) + (c.fx * c.fy) * t11;
+  if (dadx) {
+    *dadx = ((omy * (t01 - t00) + c.fy * (t11 - t10)) / P.hx) * c.ax;
+    *dady = ((omx * (t10 - t00) + c.fx * (t11 - t01)) / P.hy) * c.ay;
+  }
+  return a;
+}
+
+// density sum of slot s (density_kernel!, src/forward.jl:25-65)
+__device__ __forceinline__ double density_at(const double2 *__restrict__ X, double2 xi,
+                                             const Runs &r, const PhysF64 &P) {
+  double acc = 0.0;
+  for (int w = 0; w < 3; ++w) {
+    for (int j = r.b[w]; j < r.e[w]; ++j) {
+      double2 xj = X[j];
+      double dx = xi.x - xj.x;
+      double dy = xi.y - xj.y;
+      double r2 = dx * dx + dy * dy;
+      if (r2 < P.r2max && r2 > P.r2min) {
+        double q = sqrt(r2) * P.invh;
+        double u = 1.0 - 0.5 * q;
+        double u2 = u * u;
+        acc += P.m * (((P.A * u2) * u2) * (2.0 * q + 1.0));
+      }
+    }
+  }
+  return acc + P.m * P.A;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_density(const double2 *__restrict__ X, const int *__restrict__ scell,
+          const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ PI,
+          double *__restrict__ rho_out, double2 *__restrict__ tapeX) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s];
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  double rho = density_at(X, xi, r, P);
+  // eos_kernel! (src/forward.jl:68-75)
+  double inv = 1.0 / rho;
+  PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
+  if (rho_out) rho_out[s] = rho;
+  if (tapeX) tapeX[s] = xi;
+}
+
+// accel_kernel! + integrate_kernel! (src/forward.jl:77-147), fused: the new
+// state goes to separate buffers because neighbours still read the old one.
+__global__ void __launch_bounds__(kBlock)
+k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
+        const double2 *__restrict__ PI, const double *__restrict__ theta,
+        const int *__restrict__ scell, const int *__restrict__ cstart, Grid g, PhysF64 P,
+        double2 *__restrict__ Xn, double2 *__restrict__ Vn, double2 *__restrict__ tapeV,
+        double2 *__restrict__ a_out, double *__restrict__ alpha_out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s], vi = V[s], pii = PI[s];
+  double pti = pii.x, iri = pii.y;
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  double ax = 0.0, ay = 0.0;
+  for (int w = 0; w < 3; ++w) {
+    for (int j = r.b[w]; j < r.e[w]; ++j) {
+      double2 xj = X[j];
+      double dx = xi.x - xj.x;
+      double dy = xi.y - xj.y;
+      double r2 = dx * dx + dy * dy;
+      if (r2 < P.r2max && r2 > P.r2min) {
+        double q = sqrt(r2) * P.invh;
+        double u = 1.0 - 0.5 * q;
+        double F = ((P.Fc * u) * u) * u;
+        double2 pij = PI[j];
+        double Pij = P.m * (pti + pij.x);
+        ax -= (Pij * F) * dx;
+        ay -= (Pij * F) * dy;
+        double C = ((P.twomumass * iri) * pij.y) * F;
+        double2 vj = V[j];
+        ax += C * (vi.x - vj.x);
+        ay += C * (vi.y - vj.y);
+      }
+    }
+  }
+  Interp ic = interp_cell(xi.x, xi.y, P);
+  double al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  ax += P.gx + wall_accel(xi.x, P.Lx, P.kw);
+  ay += P.gy + wall_accel(xi.y, P.Ly, P.kw);
+  if (a_out) {  // auxiliary recompute of st.a / st.alpha, no integration
+    a_out[s] = make_double2(ax, ay);
+    alpha_out[s] = al;
+    return;
+  }
+  double D = 1.0 / (1.0 + P.dt * al);
+  double v1 = (vi.x + P.dt * ax) * D;
+  double v2 = (vi.y + P.dt * ay) * D;
+  Vn[s] = make_double2(v1, v2);
+  Xn[s] = make_double2(xi.x + P.dt * v1, xi.y + P.dt * v2);
+  if (tapeV) tapeV[s] = vi;
+}
+
+// Adjoint pass R: density recompute, _adj_integrate! and _adj_drag!
+// (src/adjoint.jl:193-228).  v' is the taped V_{n+1}, which is bit-identical
+// to the (V + dt a) D that the reference recomputes.
+__global__ void __launch_bounds__(kBlock)
+k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
+        const double *__restrict__ theta, const int *__restrict__ scell,
+        const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ PI,
+        double2 *__restrict__ gX, double2 *__restrict__ gV, double2 *__restrict__ abar,
+        double *__restrict__ galpha) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s];
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  double rho = density_at(X, xi, r, P);
+  double inv = 1.0 / rho;
+  PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
+  Interp ic = interp_cell(xi.x, xi.y, P);
+  double al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  double2 gx = gX[s], gv = gV[s], vp = Vnext[s];
+  double gvx = gv.x + P.dt * gx.x;
+  double gvy = gv.y + P.dt * gx.y;
+  double D = 1.0 / (1.0 + P.dt * al);
+  abar[s] = make_double2((P.dt * D) * gvx, (P.dt * D) * gvy);
+  galpha[s] = ((-P.dt) * D) * (vp.x * gvx + vp.y * gvy);
+  gV[s] = make_double2(D * gvx, D * gvy);
+}
+
+// Adjoint pass P1: adj_pass1_kernel! (src/adjoint.jl:30-122) followed by the
+// design term of adj_design_kernel! (src/adjoint.jl:166-185) and gV += gVs.
+__global__ void __launch_bounds__(kBlock)
+k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
+         const double2 *__restrict__ PI, const double2 *__restrict__ abar,
+         const double *__restrict__ galpha, const double *__restrict__ theta,
+         const int *__restrict__ scell, const int *__restrict__ cstart, Grid g, PhysF64 P,
+         double2 *__restrict__ gXs, double2 *__restrict__ gV, double *__restrict__ grho,
+         double *__restrict__ gtheta) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s], vi = V[s], ab = abar[s], pii = PI[s];
+  double pti = pii.x, iri = pii.y;
+  double gx = 0.0, gy = 0.0, gvx = 0.0, gvy = 0.0, grh = 0.0, sumS = 0.0;
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  for (int w = 0; w < 3; ++w) {
+    for (int j = r.b[w]; j < r.e[w]; ++j) {
+      double2 xj = X[j];
+      double dx = xi.x - xj.x;
+      double dy = xi.y - xj.y;
+      double r2 = dx * dx + dy * dy;
+      if (r2 < P.r2max && r2 > P.r2min) {
+        double rr = sqrt(r2);
+        double q = rr * P.invh;
+        double u = 1.0 - 0.5 * q;
+        double F = ((P.Fc * u) * u) * u;
+        double G = (P.Gc * u) * u;
+        double2 pij = PI[j];
+        double Pij = P.m * (pti + pij.x);
+        double Cbase = (P.twomumass * iri) * pij.y;
+        double C = Cbase * F;
+        double2 abj = abar[j];
+        double2 vj = V[j];
+        double dax = ab.x - abj.x;
+        double day = ab.y - abj.y;
+        double dvx = vi.x - vj.x;
+        double dvy = vi.y - vj.y;
+        double ad_d = -(dax * dx + day * dy);
+        double ad_v = dax * dvx + day * dvy;
+        gx -= (Pij * F) * dax;
+        gy -= (Pij * F) * day;
+        double Fbar = Pij * ad_d + Cbase * ad_v;
+        double wgt = ((Fbar * G) * P.invh) / rr;
+        gx += wgt * dx;
+        gy += wgt * dy;
+        gvx += C * dax;
+        gvy += C * day;
+        grh -= (C * ad_v) * iri;
+        sumS += F * ad_d;
+      }
+    }
+  }
+  grh += (((-2.0 * P.m) * pti) * iri) * sumS;
+  grh += (((P.c2 * P.m) * iri) * iri) * sumS;
+  gx += ab.x * wall_daccel(xi.x, P.Lx, P.kw);
+  gy += ab.y * wall_daccel(xi.y, P.Ly, P.kw);
+  // adj_design_kernel!: gXs += galpha * grad(alpha); gtheta += bilinear weights * galpha
+  double ga = galpha[s];
+  Interp ic = interp_cell(xi.x, xi.y, P);
+  double dadx, dady;
+  interp_alpha(theta, ic, P, &dadx, &dady);
+  gx += ga * dadx;
+  gy += ga * dady;
+  atomicAdd(&gtheta[ic.l00], (ga * (1.0 - ic.fx)) * (1.0 - ic.fy));
+  atomicAdd(&gtheta[ic.l00 + P.ngy], (ga * ic.fx) * (1.0 - ic.fy));
+  atomicAdd(&gtheta[ic.l00 + 1], (ga * (1.0 - ic.fx)) * ic.fy);
+  atomicAdd(&gtheta[ic.l00 + P.ngy + 1], (ga * ic.fx) * ic.fy);
+  gXs[s] = make_double2(gx, gy);
+  grho[s] = grh;
+  double2 gvo = gV[s];
+  gV[s] = make_double2(gvo.x + gvx, gvo.y + gvy);
+}
+
+// Adjoint pass P2: adj_pass2_kernel! (src/adjoint.jl:126-156) and gX += gXs.
+__global__ void __launch_bounds__(kBlock)
+k_adj_p2(const double2 *__restrict__ X, const double *__restrict__ grho,
+         const double2 *__restrict__ gXs, const int *__restrict__ scell,
+         const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ gX) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s];
+  double gri = grho[s];
+  double gx = 0.0, gy = 0.0;
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  for (int w = 0; w < 3; ++w) {
+    for (int j = r.b[w]; j < r.e[w]; ++j) {
+      double2 xj = X[j];
+      double dx = xi.x - xj.x;
+      double dy = xi.y - xj.y;
+      double r2 = dx * dx + dy * dy;
+      if (r2 < P.r2max && r2 > P.r2min) {
+        double q = sqrt(r2) * P.invh;
+        double u = 1.0 - 0.5 * q;
+        double wgt = (P.m * (((P.Fc * u) * u) * u)) * (gri + grho[j]);
+        gx += wgt * dx;
+        gy += wgt * dy;
+      }
+    }
+  }
+  double2 gs = gXs[s];
+  double sx = gs.x + gx;
+  double sy = gs.y + gy;
+  double2 go = gX[s];
+  gX[s] = make_double2(go.x + sx, go.y + sy);
+}
+
+// target_objective (src/driver.jl:219-234) in FP64: per-block partial sums
+// in a fixed order, then one block sums the partials, so J is bit-identical
+// from run to run.
+__global__ void k_objective_partial(const double2 *__restrict__ X, int N, double xc, double yc,
+                                    double s2, double *__restrict__ partials,
+                                    double2 *__restrict__ seed) {
+  __shared__ double red[kBlock];
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (s < N) {
+    double2 x = X[s];
+    double dx = x.x - xc;
+    double dy = x.y - yc;
+    double e = exp(-(dx * dx + dy * dy) / (2.0 * s2));
+    v = -e;
+    if (seed) seed[s] = make_double2((e * dx) / s2, (e * dy) / s2);
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void k_objective_final(const double *__restrict__ partials, int nb, double *J) {
+  __shared__ double red[kBlock];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kBlock) v += partials[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *J = red[0];
+}
+
+}  // namespace
+
+void launch_density_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
+                        Grid g, PhysF64 P, double2 *PI, double *rho, double2 *tapeX) {
+  if (g.N > 0) k_density<<<nblocks(g.N), kBlock, 0, s>>>(X, scell, cstart, g, P, PI, rho, tapeX);
+}
+void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
+                      const double *theta, const int *scell, const int *cstart, Grid g,
+                      PhysF64 P, double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
+                      double *alpha_out) {
+  if (g.N > 0)
+    k_force<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, theta, scell, cstart, g, P, Xn, Vn, tapeV,
+                                            a_out, alpha_out);
+}
+void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
+                      const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *PI,
+                      double2 *gX, double2 *gV, double2 *abar, double *galpha) {
+  if (g.N > 0)
+    k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, scell, cstart, g, P, PI, gX, gV,
+                                            abar, galpha);
+}
+void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
+                       const double2 *abar, const double *galpha, const double *theta,
+                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gXs,
+                       double2 *gV, double *grho, double *gtheta) {
+  if (g.N > 0)
+    k_adj_p1<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, abar, galpha, theta, scell, cstart, g, P,
+                                             gXs, gV, grho, gtheta);
+}
+void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
+                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gX) {
+  if (g.N > 0)
+    k_adj_p2<<<nblocks(g.N), kBlock, 0, s>>>(X, grho, gXs, scell, cstart, g, P, gX);
+}
+int objective_blocks(int N) { return N > 0 ? nblocks(N) : 1; }
+void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,
+                          double sigma, double *partials, int nb, double *J, double2 *seed) {
+  double s2 = sigma * sigma;
+  if (N > 0) k_objective_partial<<<nb, kBlock, 0, s>>>(X, N, xc, yc, s2, partials, seed);
+  else launch_fill_i32(s, (int *)partials, 2, 0);
+  k_objective_final<<<1, kBlock, 0, s>>>(partials, N > 0 ? nb : 1, J);
+}
+
+}  // namespace sph

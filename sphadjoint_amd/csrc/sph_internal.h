@@ -1,0 +1,134 @@
+// sph_internal.h -- shared definitions of libsphadj's kernels and host code.
+//
+// Device layout (HBM), per state:
+//   particles live in "slots": sorted by (cell, particle id) at the last
+//   neighbour rebuild, so each cell is a contiguous slot range and the three
+//   rows of a 3x3 cell sweep are three contiguous slot runs;
+//   X, V are double2 (interleaved x,y) per slot; ids[slot] = particle id;
+//   scell[slot] = the cell whose 3x3 block the slot sweeps;
+//   cstart[0..ncell] = first slot of each cell (exclusive prefix, cstart[ncell] = N).
+// Cells and ids are 0-based on the device; the ABI converts to the
+// reference's 1-based convention.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+constexpr int kBlock = 256;
+
+// Cell grid of a state (0-based cells c = cx + cy*nx).
+struct Grid {
+  int nx, ny, ncell, N;
+};
+
+// Constants of the FP64 kernels.  Every value is formed on the host with the
+// operation order the reference uses in T (see sph_api.cpp: make_phys).
+struct PhysF64 {
+  double m, rho0, c2, mu, gx, gy, dt, Lx, Ly, kw;
+  double A, invh, r2max, r2min, Fc, Gc, twomumass;
+  double hx, hy, gxmax, gymax, gxclamp, gyclamp;  // theta grid: spacing, ngx-1, ngx-1-1e-6
+  int ngx, ngy;
+};
+
+// Cell-index constants (src/neighbors.jl:38-42, 192-218): cs in T, 1/cs in T,
+// squared cutoffs in T.
+struct CellConstF64 {
+  double cs, invcs, rc2, r2min, dmax2;
+};
+
+// ---- integer helpers -------------------------------------------------------
+
+// unsafe_trunc(Int, floor(v)) + 2 clamped to [1, n], then made 0-based.
+// NaN / out-of-range inputs give the lowest cell, as x86-64's cvttsd2si
+// (typemin) followed by the clamp does in the reference.
+__device__ __forceinline__ int cell_coord_div(double v, int n) {
+  double f = floor(v);
+  long long t = (f > -1e15 && f < 1e15) ? (long long)f : (long long)(-(1LL << 60));
+  long long c = t + 2;
+  c = c < 1 ? 1 : (c > n ? n : c);
+  return (int)(c - 1);
+}
+// unsafe_trunc(Int32, floor(v)) + Int32(2) clamped to [1, n], 0-based.
+__device__ __forceinline__ int cell_coord_i32(double v, int n) {
+  double f = floor(v);
+  long long t = (f > -2147483648.0 && f < 2147483647.0) ? (long long)(int)f
+                                                          : (long long)INT32_MIN;
+  long long c = t + 2;
+  c = c < 1 ? 1 : (c > n ? n : c);
+  return (int)(c - 1);
+}
+
+// The three contiguous slot runs of a 3x3 sweep around cell `sc`, in the
+// reference's order (ddy outer, ddx inner; src/neighbors.jl:220-231).
+struct Runs {
+  int b[3], e[3];
+};
+__device__ __forceinline__ Runs sweep_runs(const int *__restrict__ cstart, int sc, int nx,
+                                           int ny) {
+  Runs r;
+  int cy = sc / nx;
+  int cx = sc - cy * nx;
+  int x0 = cx > 0 ? cx - 1 : 0;
+  int x1 = cx < nx - 1 ? cx + 1 : nx - 1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int jy = cy + k - 1;
+    if (jy >= 0 && jy < ny) {
+      r.b[k] = cstart[jy * nx + x0];
+      r.e[k] = cstart[jy * nx + x1 + 1];
+    } else {
+      r.b[k] = 0;
+      r.e[k] = 0;
+    }
+  }
+  return r;
+}
+
+// ---- launchers ----------------------------------------------------------------
+
+// cell list (sph_cells.hip)
+void launch_fill_i32(hipStream_t s, int *p, int n, int v);
+void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts);
+void launch_scan_excl(hipStream_t s, const int *in, int n, int *out, int *cursor, int *bsum);
+void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
+                 int *tmp_id, int *tmp_slot);
+void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
+                 const int *tmp_slot, int N, int *perm, int *new_ids);
+void launch_permute(hipStream_t s, const int *perm, const double2 *X, const double2 *V,
+                    const double2 *Xref, int check, Grid g, CellConstF64 k, double2 *Xn,
+                    double2 *Vn, double2 *Xrefn, int *scell, int *flags);
+void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, const int *scell,
+                        const int *cstart, Grid g, CellConstF64 k, int maxnb, long long sm,
+                        long long si, int *indices, int *nbcount, int *flags);
+void launch_scatter_i32(hipStream_t s, const int *src, const int *ids, int N, int *dst);
+void launch_inverse(hipStream_t s, const int *ids, int N, int *inv);
+void launch_gather_remap2(hipStream_t s, const double2 *src, const int *inv_src,
+                          const int *ids_dst, int N, double2 *dst);
+void launch_scatter2(hipStream_t s, const double2 *src, const int *ids, int N, double2 *dst);
+void launch_gather2(hipStream_t s, const double2 *src, const int *ids, int N, double2 *dst);
+void launch_scatter1(hipStream_t s, const double *src, const int *ids, int N, double *dst);
+void launch_copy2(hipStream_t s, const double2 *src, int N, double2 *dst);
+
+// FP64 physics (sph_f64.hip)
+void launch_density_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
+                        Grid g, PhysF64 P, double2 *PI, double *rho, double2 *tapeX);
+void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
+                      const double *theta, const int *scell, const int *cstart, Grid g,
+                      PhysF64 P, double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
+                      double *alpha_out);
+void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
+                      const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *PI,
+                      double2 *gX, double2 *gV, double2 *abar, double *galpha);
+void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
+                       const double2 *abar, const double *galpha, const double *theta,
+                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gXs,
+                       double2 *gV, double *grho, double *gtheta);
+void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
+                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gX);
+void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,
+                          double sigma, double *partials, int nblocks, double *J,
+                          double2 *seed);
+int objective_blocks(int N);
+
+}  // namespace sph
