@@ -39,8 +39,17 @@ ALG_BYTES_F64 = {
     "adjoint_p1": 88 + 40,              # read X, V, PI, abar, galpha, gV; write gXs, grho, gV
     "adjoint_p2": 56 + 16,              # read X, grho, gXs, gX; write gX
 }
-# the reference-level minimum (per particle-step), FP64 state
-ALG_STEP_BYTES_F64 = {"fwd": 96, "fwd_adj": 304}
+ALG_BYTES_F32C = {
+    "density": 8 + 4 + 8,               # read xr, cell; write (pterm, invrho)
+    "force_integrate": 44 + 56,         # read X64, xr, V, PI, cell; write X64', xr', V', tape X, V
+    "adjoint_r": 28 + 72,               # xr from tape X64; read X, xr, V', gX, gV, cell; write PI, abar, galpha, gV
+    "adjoint_p1": 64 + 20,              # read X, xr, V, PI, abar, galpha, gV, cell; write gXs, grho, gV
+    "adjoint_p2": 32 + 8,               # read xr, grho, gXs, gX, cell; write gX
+}
+ALG_BYTES = {"f64": ALG_BYTES_F64, "f32c": ALG_BYTES_F32C}
+# the reference-level minimum per particle-step (survey of the path):
+# FP64 state 96 / 304 B; FP64 positions with FP32 rest 72 / 200 B
+ALG_STEP_BYTES = {"f64": {"fwd": 96, "fwd_adj": 304}, "f32c": {"fwd": 72, "fwd_adj": 200}}
 
 
 def log(*a):
@@ -138,6 +147,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=32, help="steps of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="f64,f32c", help="comma list of f64,f32c")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
@@ -148,18 +158,21 @@ def main():
     N = sc.N
     log(f"workload {sc.name}: N={N}, K={args.steps}, W={args.warmup}")
 
-    f64 = run_gpu(sc, args.steps, args.warmup, "f64")
-    best, best_dtype = f64, "f64"
+    forms = {}
+    for form in args.forms.split(","):
+        forms[form] = run_gpu(sc, args.steps, args.warmup, form)
+    best_dtype = "f32c" if "f32c" in forms else "f64"
+    best = forms[best_dtype]
 
     kern = best["kernels"]
-    dom = max((k for k in kern if k in ALG_BYTES_F64), key=lambda k: kern[k]["total_ms"])
-    alg = ALG_BYTES_F64[dom] * N
+    table = ALG_BYTES[best_dtype]
+    dom = max((k for k in kern if k in table), key=lambda k: kern[k]["total_ms"])
+    alg = table[dom] * N
     achieved = alg / (kern[dom]["avg_us"] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": dom, "alg_bytes_per_launch": alg,
                 "avg_launch_us": round(kern[dom]["avg_us"], 3)}
-    step_gbs = ALG_STEP_BYTES_F64["fwd_adj"] * N * args.steps / best["fwdadj_s"] / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -172,7 +185,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(best["fwdadj_s"] * 1e3 / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": best_dtype, "data": "synthetic",
+        "dtype": {"f32c": "f32 (FP64 positions, J, gtheta)", "f64": "f64"}[best_dtype],
+        "data": "synthetic",
         "config": {"workload": f"block_{args.n_side}x{args.n_side} dam-break block, N={N}, "
                                f"h=1.2dx, tank 2.0x1.2, theta 512x256, {args.steps} steps "
                                f"forward+adjoint (full tape)",
@@ -180,9 +194,13 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {k: v for k, v in cpu.items() if k != "fwd_value"},
         "forms": {
-            "f64": {"fwd_adj": round(f64["value"], 1), "fwd": round(f64["fwd_value"], 1),
-                    "step_alg_GBs": round(step_gbs, 1)},
-        },
+            f: {"fwd_adj": round(r["value"], 1), "fwd": round(r["fwd_value"], 1),
+                "step_alg_GBs": round(ALG_STEP_BYTES[f]["fwd_adj"] * N * args.steps
+                                      / r["fwdadj_s"] / 1e9, 1),
+                "J": r["J"]}
+            for f, r in forms.items()},
+        "f32c_over_f64": (round(forms["f32c"]["value"] / forms["f64"]["value"], 3)
+                          if "f32c" in forms and "f64" in forms else None),
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in best["kernels"].items()},
     }
     print(json.dumps(line), flush=True)

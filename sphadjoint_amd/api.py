@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import ctypes as C
 import warnings
+import weakref
 
 import numpy as np
 
@@ -136,7 +137,7 @@ class CellList:
     """View of a state's cell list (src/neighbors.jl:9-19)."""
 
     def __init__(self, st):
-        self._st = st
+        self._ref = weakref.ref(st)
         info = st._info()
         self.nx, self.ny = int(info[0]), int(info[1])
         self.cs = st.rc
@@ -151,6 +152,7 @@ class CellList:
         L.call("sph_cell_list_get", st._h, _ptr(cellof), _ptr(counts), _ptr(starts), _ptr(order))
         return cellof, counts, starts, order
 
+    _st = property(lambda s: s._ref())
     cellof = property(lambda s: s.arrays()[0])
     counts = property(lambda s: s.arrays()[1])
     starts = property(lambda s: s.arrays()[2])
@@ -162,11 +164,13 @@ class NeighborList:
     is materialised from the positions of the last build on request."""
 
     def __init__(self, st):
-        self._st = st
+        self._ref = weakref.ref(st)
         info = st._info()
         self.maxnb, self.sm, self.si = int(info[3]), int(info[4]), int(info[5])
         self.rc = st.rc
         self.dmax = st.p.h * st.skin
+
+    _st = property(lambda s: s._ref())
 
     @property
     def interval(self):

@@ -17,6 +17,7 @@
 #include <utility>
 #include <vector>
 
+#include "sph_f32c.h"
 #include "sph_internal.h"
 
 using namespace sph;
@@ -109,6 +110,43 @@ PhysF64 make_phys(const sph_params &p) {
   return P;
 }
 
+// Constants of the f32c form: formed in FP64 and rounded once to FP32.
+PhysF32 make_phys32(const sph_params &p, double rc) {
+  PhysF32 P{};
+  const double h = p.h;
+  const double A = 7.0 / ((4.0 * M_PI) * (h * h));
+  P.m = (float)p.m;
+  P.rho0 = (float)p.rho0;
+  P.c2 = (float)(p.c * p.c);
+  P.mu = (float)p.mu;
+  P.gx = (float)p.gx;
+  P.gy = (float)p.gy;
+  P.dt = (float)p.dt;
+  P.Lx = (float)p.Lx;
+  P.Ly = (float)p.Ly;
+  P.kw = (float)p.kw;
+  P.mA = (float)(p.m * A);
+  P.invh = (float)(1.0 / h);
+  P.r2max = (float)(4.0 * h * h);
+  P.r2min = (float)(FLT_EPSILON * h * h);
+  P.Fc = (float)(-5.0 * A / (h * h));
+  P.Gc = (float)(7.5 * A / (h * h));
+  P.twomumass = (float)(2.0 * p.m * p.mu);
+  P.cs = (float)rc;
+  P.dt64 = p.dt;
+  P.hx = p.Lx / (double)(p.ngx - 1);
+  P.hy = p.Ly / (double)(p.ngy - 1);
+  P.inv_hx = (float)(1.0 / P.hx);
+  P.inv_hy = (float)(1.0 / P.hy);
+  P.gxmax = (double)(p.ngx - 1);
+  P.gymax = (double)(p.ngy - 1);
+  P.gxclamp = (double)(p.ngx - 1) - 1e-6;
+  P.gyclamp = (double)(p.ngy - 1) - 1e-6;
+  P.ngx = (int)p.ngx;
+  P.ngy = (int)p.ngy;
+  return P;
+}
+
 void check_params(const sph_params *p) {
   if (!p) fail(SPH_E_ARG, "params is NULL");
   if (!(p->h > 0) || !(p->dt > 0) || !(p->Lx > 0) || !(p->Ly > 0))
@@ -137,6 +175,9 @@ struct ProfRec {
 struct sph_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // objects created on this context plus the handle itself; the context is
+  // freed when the handle has been destroyed and the last object is gone
+  int refs = 1;
   // optional per-kernel timing with events on this stream
   bool prof = false;
   std::vector<ProfRec> pending;
@@ -146,6 +187,20 @@ struct sph_ctx {
 };
 
 namespace {
+
+void ctx_retain(sph_ctx *c) { c->refs += 1; }
+
+void ctx_release(sph_ctx *c) {
+  if (--c->refs > 0) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &r : c->pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->evpool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
 
 hipEvent_t prof_event(sph_ctx *c) {
   if (!c->evpool.empty()) {
@@ -201,14 +256,16 @@ struct sph_state {
   int64_t N = 0;
   Grid g{};
   PhysF64 P{};
+  PhysF32 P32{};
   CellConstF64 kc{};
   double skin = 0.2;
   int64_t interval = 4, maxnb = 64, sm = 0, si = 0;
   int layout = SPH_LAYOUT_SLOT;
   int64_t age = kAgeNever;
   uint64_t epoch = 0;  // serial number of the current slot layout
-  // current layout
+  // current layout; for the f32c form scell holds the binning cell of each slot
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
+  float2 *V32 = nullptr, *V32n = nullptr, *xr = nullptr, *xrn = nullptr, *PI32 = nullptr;
   double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
   double2 *Xref = nullptr, *Xref2 = nullptr;
   double2 *PI = nullptr;
@@ -244,12 +301,14 @@ struct sph_tape {
   int64_t N = 0, cap = 0, n = 0;
   int precision = SPH_F64;
   double2 *X = nullptr, *V = nullptr;  // (cap+1) entries of N; entry k (1-based) at (k-1)*N
+  float2 *V32 = nullptr;                // f32c form: FP32 velocities instead of V
   std::vector<int> entry_epoch;         // epoch record of each entry (cap+1)
   std::vector<TapeEpoch> epochs;        // records in use
   std::vector<TapeEpoch> pool;          // retained allocations
   bool has_geom = false;
   Grid g{};
   PhysF64 P{};
+  double rc = 0.0;  // cell width of the recorded epochs
   int ncell = 0;
 };
 
@@ -262,6 +321,10 @@ struct sph_workspace {
   double2 *Vtmp = nullptr, *t2 = nullptr, *gX_id = nullptr, *gV_id = nullptr;
   double *galpha = nullptr, *grho = nullptr, *gtheta = nullptr;
   int *inv = nullptr;
+  // f32c form
+  float2 *gX32 = nullptr, *gV32 = nullptr, *gXs32 = nullptr, *abar32 = nullptr, *PI32 = nullptr;
+  float2 *Vtmp32 = nullptr, *t2f = nullptr, *xr32 = nullptr;
+  float *galpha32 = nullptr, *grho32 = nullptr;
 };
 
 namespace {
@@ -274,6 +337,8 @@ void state_free(sph_state *st) {
   double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
                     &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
   for (auto pp : d2) dfree(*pp);
+  float2 **f2[] = {&st->V32, &st->V32n, &st->xr, &st->xrn, &st->PI32};
+  for (auto pp : f2) dfree(*pp);
   double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
                    &st->tmp1, &st->obj_part, &st->obj_J};
   for (auto pp : d1) dfree(*pp);
@@ -307,11 +372,19 @@ void materialise_aux(sph_state *st) {
   if (st->aux_state != 1) return;
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
-  launch_density_f64(s, st->Xn, st->scell, st->cstart, st->g, st->P, st->PI, st->rho_slot, nullptr);
-  launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->scell, st->cstart, st->g, st->P,
-                   nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
+  if (st->precision == SPH_F32C) {
+    launch_density_f32c(s, st->xrn, st->scell, st->cstart, st->g, st->P32, st->PI32, st->rho_slot);
+    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell,
+                      st->cstart, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      st->a_slot, st->alpha_slot);
+    launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
+  } else {
+    launch_density_f64(s, st->Xn, st->scell, st->cstart, st->g, st->P, st->PI, st->rho_slot, nullptr);
+    launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->scell, st->cstart, st->g, st->P,
+                     nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
+    launch_scatter2(s, st->PI, st->ids, N, st->pi_id);
+  }
   launch_scatter2(s, st->a_slot, st->ids, N, st->a_id);
-  launch_scatter2(s, st->PI, st->ids, N, st->pi_id);
   launch_scatter1(s, st->alpha_slot, st->ids, N, st->alpha_id);
   launch_scatter1(s, st->rho_slot, st->ids, N, st->rho_id);
   st->aux_state = 2;
@@ -323,12 +396,20 @@ void do_build_neighbors(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   do_build(st);
   int check = (st->age != kAgeNever) ? 1 : 0;
-  timed(st->ctx, K_PERMUTE, [&] {
-    launch_permute(s, st->perm, st->X, st->V, st->Xref, check, st->g, st->kc, st->Xn, st->Vn,
-                   st->Xref2, st->scell, st->flags);
-  });
+  if (st->precision == SPH_F32C) {
+    timed(st->ctx, K_PERMUTE, [&] {
+      launch_permute_f32c(s, st->perm, st->cellof, st->X, st->V32, st->Xref, check, st->g, st->P32,
+                          st->kc.dmax2, st->Xn, st->V32n, st->xr, st->Xref2, st->scell, st->flags);
+    });
+    std::swap(st->V32, st->V32n);
+  } else {
+    timed(st->ctx, K_PERMUTE, [&] {
+      launch_permute(s, st->perm, st->X, st->V, st->Xref, check, st->g, st->kc, st->Xn, st->Vn,
+                     st->Xref2, st->scell, st->flags);
+    });
+    std::swap(st->V, st->Vn);
+  }
   std::swap(st->X, st->Xn);
-  std::swap(st->V, st->Vn);
   std::swap(st->Xref, st->Xref2);
   std::swap(st->ids, st->new_ids);
   std::swap(st->cstart, st->cl_cstart);
@@ -347,6 +428,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
   if (!tp->has_geom) {
     tp->g = st->g;
     tp->P = st->P;
+    tp->rc = st->kc.cs;
     tp->ncell = st->g.ncell;
     tp->has_geom = true;
   } else if (tp->ncell != st->g.ncell || tp->g.nx != st->g.nx) {
@@ -378,12 +460,28 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   if (st->age >= st->interval) do_build_neighbors(st);   // maybe_rebuild!
   st->age += 1;
   double2 *tX = nullptr, *tV = nullptr;
+  float2 *tV32 = nullptr;
   if (tp) {
     tape_epoch_for(tp, st);
     tp->n += 1;
     tp->entry_epoch[tp->n] = (int)tp->epochs.size() - 1;
     tX = tp->X + (size_t)(tp->n - 1) * tp->N;
-    tV = tp->V + (size_t)(tp->n - 1) * tp->N;
+    if (st->precision == SPH_F32C) tV32 = tp->V32 + (size_t)(tp->n - 1) * tp->N;
+    else tV = tp->V + (size_t)(tp->n - 1) * tp->N;
+  }
+  if (st->precision == SPH_F32C) {
+    timed(st->ctx, K_DENSITY, [&] {
+      launch_density_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32, st->PI32, nullptr);
+    });
+    timed(st->ctx, K_FORCE, [&] {
+      launch_force_f32c(s, st->X, st->xr, st->V32, st->PI32, theta, st->scell, st->cstart, st->g,
+                        st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
+    });
+    std::swap(st->X, st->Xn);
+    std::swap(st->xr, st->xrn);
+    std::swap(st->V32, st->V32n);
+    st->aux_state = 1;
+    return;
   }
   timed(st->ctx, K_DENSITY, [&] {
     launch_density_f64(s, st->X, st->scell, st->cstart, st->g, st->P, st->PI, nullptr, tX);
@@ -397,8 +495,8 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   st->aux_state = 1;
 }
 
-void need_f64(int precision) {
-  if (precision != SPH_F64) fail(SPH_E_ARG, "precision %d is not available for this entry point", precision);
+void need_precision(int precision) {
+  if (precision != SPH_F64 && precision != SPH_F32C) fail(SPH_E_ARG, "unknown precision %d", precision);
 }
 
 void copy_theta(sph_state *st, const sph_field *th) {
@@ -437,14 +535,7 @@ int sph_ctx_create(int device, sph_ctx **out) {
 int sph_ctx_destroy(sph_ctx *ctx) {
   return guarded([&] {
     if (!ctx) return;
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto &r : ctx->pending) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
-    }
-    for (auto e : ctx->evpool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_release(ctx);
   });
 }
 
@@ -496,6 +587,7 @@ int sph_field_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, const double *host,
     f->d = dalloc<double>(ngy * ngx);
     if (host) HIPC(hipMemcpy(f->d, host, sizeof(double) * ngy * ngx, hipMemcpyHostToDevice));
     else HIPC(hipMemset(f->d, 0, sizeof(double) * ngy * ngx));
+    ctx_retain(ctx);
     *out = f;
   });
 }
@@ -521,7 +613,9 @@ int sph_field_destroy(sph_field *f) {
     if (!f) return;
     (void)hipStreamSynchronize(f->ctx->stream);
     dfree(f->d);
+    sph_ctx *c = f->ctx;
     delete f;
+    ctx_release(c);
   });
 }
 
@@ -531,7 +625,7 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
     check_params(p);
-    need_f64(precision);
+    need_precision(precision);
     if (N < 1 || N > (int64_t)1 << 30) fail(SPH_E_ARG, "N=%lld out of range", (long long)N);
     if (!X0 || !V0) fail(SPH_E_ARG, "X0 and V0 are required");
     if (!(skin >= 0)) fail(SPH_E_ARG, "skin must be >= 0");
@@ -553,6 +647,7 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       if (layout == SPH_LAYOUT_SLOT) { st->sm = N; st->si = 1; }
       else { st->sm = 1; st->si = maxnb; }
       st->P = make_phys(*p);
+      st->P32 = make_phys32(*p, (2.0 * p->h) * (1.0 + skin));
       // NeighborList ctor: rc = Float64(support(p))*(1+skin), dmax = Float64(p.h)*skin;
       // CellList with cs = rc (src/neighbors.jl:28-36,174-190; src/driver.jl:33-34)
       const double rc = (2.0 * p->h) * (1.0 + skin);
@@ -572,16 +667,26 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       st->kc.r2min = (DBL_EPSILON * p->h) * p->h;
       st->kc.dmax2 = dmax * dmax;
       const size_t n = (size_t)N, nc = (size_t)st->g.ncell;
+      st->tmp2 = dalloc<double2>(n);
       st->ids = dalloc<int>(n);
       st->scell = dalloc<int>(n);
       st->cstart = dalloc<int>(nc + 1);
+      const bool f32c = precision == SPH_F32C;
       st->X = dalloc<double2>(n);
-      st->V = dalloc<double2>(n);
       st->Xn = dalloc<double2>(n);
-      st->Vn = dalloc<double2>(n);
       st->Xref = dalloc<double2>(n);
       st->Xref2 = dalloc<double2>(n);
-      st->PI = dalloc<double2>(n);
+      if (f32c) {
+        st->V32 = dalloc<float2>(n);
+        st->V32n = dalloc<float2>(n);
+        st->xr = dalloc<float2>(n);
+        st->xrn = dalloc<float2>(n);
+        st->PI32 = dalloc<float2>(n);
+      } else {
+        st->V = dalloc<double2>(n);
+        st->Vn = dalloc<double2>(n);
+        st->PI = dalloc<double2>(n);
+      }
       st->cellof = dalloc<int>(n);
       st->counts = dalloc<int>(nc);
       st->cl_cstart = dalloc<int>(nc + 1);
@@ -600,7 +705,6 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       st->alpha_slot = dalloc<double>(n);
       st->rho_slot = dalloc<double>(n);
       st->theta_last = dalloc<double>((size_t)(p->ngx * p->ngy));
-      st->tmp2 = dalloc<double2>(n);
       st->seed_slot = dalloc<double2>(n);
       st->tmp1 = dalloc<double>(n);
       st->obj_nb = objective_blocks((int)N);
@@ -612,9 +716,15 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       for (size_t i = 0; i < n; ++i) iota[i] = (int)i;
       HIPC(hipMemcpy(st->ids, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
       HIPC(hipMemcpy(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice));
-      HIPC(hipMemcpy(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
+      if (f32c) {
+        HIPC(hipMemcpy(st->tmp2, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
+        launch_d2_to_f2(s, st->tmp2, (int)N, st->V32);
+        HIPC(hipMemset(st->PI32, 0, sizeof(float2) * n));
+      } else {
+        HIPC(hipMemcpy(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
+        HIPC(hipMemset(st->PI, 0, sizeof(double2) * n));
+      }
       HIPC(hipMemset(st->flags, 0, sizeof(int) * 2));
-      HIPC(hipMemset(st->PI, 0, sizeof(double2) * n));
       HIPC(hipMemset(st->a_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(st->pi_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(st->alpha_id, 0, sizeof(double) * n));
@@ -626,6 +736,7 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       delete st;
       throw;
     }
+    ctx_retain(ctx);
     *out = st;
   });
 }
@@ -635,7 +746,9 @@ int sph_state_destroy(sph_state *st) {
     if (!st) return;
     (void)hipStreamSynchronize(st->ctx->stream);
     state_free(st);
+    sph_ctx *c = st->ctx;
     delete st;
+    ctx_release(c);
   });
 }
 
@@ -657,7 +770,10 @@ int sph_state_get(sph_state *st, int field, double *host) {
     switch (field) {
       case SPH_X:
       case SPH_V:
-        launch_scatter2(s, field == SPH_X ? st->X : st->V, st->ids, N, st->tmp2);
+        if (field == SPH_V && st->precision == SPH_F32C)
+          launch_scatter2f(s, st->V32, st->ids, N, st->tmp2);
+        else
+          launch_scatter2(s, field == SPH_X ? st->X : st->V, st->ids, N, st->tmp2);
         HIPC(hipMemcpyAsync(host, st->tmp2, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
         break;
       case SPH_A:
@@ -692,7 +808,16 @@ int sph_state_set(sph_state *st, int field, const double *host) {
     hipStream_t s = st->ctx->stream;
     materialise_aux(st);  // the spare buffers may be reused below
     HIPC(hipMemcpyAsync(st->tmp2, host, sizeof(double2) * st->N, hipMemcpyHostToDevice, s));
-    launch_gather2(s, st->tmp2, st->ids, (int)st->N, field == SPH_X ? st->X : st->V);
+    if (st->precision == SPH_F32C) {
+      if (field == SPH_V) {
+        launch_gather2f(s, st->tmp2, st->ids, (int)st->N, st->V32);
+      } else {
+        launch_gather2(s, st->tmp2, st->ids, (int)st->N, st->X);
+        if (st->age != kAgeNever) launch_xrel_f32c(s, st->X, st->scell, st->g, st->P32, st->xr);
+      }
+    } else {
+      launch_gather2(s, st->tmp2, st->ids, (int)st->N, field == SPH_X ? st->X : st->V);
+    }
     HIPC(hipStreamSynchronize(s));
   });
 }
@@ -798,7 +923,12 @@ int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape
       hipStream_t s = st->ctx->stream;
       const size_t off = (size_t)tape->n * tape->N;
       HIPC(hipMemcpyAsync(tape->X + off, st->X, sizeof(double2) * tape->N, hipMemcpyDeviceToDevice, s));
-      HIPC(hipMemcpyAsync(tape->V + off, st->V, sizeof(double2) * tape->N, hipMemcpyDeviceToDevice, s));
+      if (st->precision == SPH_F32C)
+        HIPC(hipMemcpyAsync(tape->V32 + off, st->V32, sizeof(float2) * tape->N,
+                            hipMemcpyDeviceToDevice, s));
+      else
+        HIPC(hipMemcpyAsync(tape->V + off, st->V, sizeof(double2) * tape->N,
+                            hipMemcpyDeviceToDevice, s));
       tape->entry_epoch[tape->n + 1] = tape->entry_epoch[tape->n];
     }
     HIPC(hipStreamSynchronize(st->ctx->stream));
@@ -809,7 +939,7 @@ int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape
 int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
-    need_f64(precision);
+    need_precision(precision);
     if (N < 1 || capacity < 0) fail(SPH_E_ARG, "bad tape size");
     HIPC(hipSetDevice(ctx->device));
     auto *tp = new sph_tape;
@@ -820,14 +950,17 @@ int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sp
       tp->precision = precision;
       const size_t tot = (size_t)(capacity + 1) * (size_t)N;
       tp->X = dalloc<double2>(tot);
-      tp->V = dalloc<double2>(tot);
+      if (precision == SPH_F32C) tp->V32 = dalloc<float2>(tot);
+      else tp->V = dalloc<double2>(tot);
       tp->entry_epoch.assign((size_t)capacity + 2, -1);
     } catch (...) {
       dfree(tp->X);
       dfree(tp->V);
+      dfree(tp->V32);
       delete tp;
       throw;
     }
+    ctx_retain(ctx);
     *out = tp;
   });
 }
@@ -844,7 +977,10 @@ int sph_tape_destroy(sph_tape *tp) {
       }
     dfree(tp->X);
     dfree(tp->V);
+    dfree(tp->V32);
+    sph_ctx *c = tp->ctx;
     delete tp;
+    ctx_release(c);
   });
 }
 
@@ -887,7 +1023,8 @@ int sph_tape_get(sph_tape *tp, int64_t k, double *X, double *V) {
         HIPC(hipStreamSynchronize(s));
       }
       if (V) {
-        launch_scatter2(s, tp->V + off, e.ids, (int)tp->N, tmp);
+        if (tp->precision == SPH_F32C) launch_scatter2f(s, tp->V32 + off, e.ids, (int)tp->N, tmp);
+        else launch_scatter2(s, tp->V + off, e.ids, (int)tp->N, tmp);
         HIPC(hipMemcpyAsync(V, tmp, sizeof(double2) * tp->N, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
       }
@@ -899,12 +1036,23 @@ int sph_tape_get(sph_tape *tp, int64_t k, double *X, double *V) {
   });
 }
 
+void ws_free(sph_workspace *ws) {
+  double2 **d2[] = {&ws->gX, &ws->gV, &ws->gXs, &ws->abar, &ws->PI, &ws->Vtmp, &ws->t2,
+                    &ws->gX_id, &ws->gV_id};
+  for (auto pp : d2) dfree(*pp);
+  float2 **f2[] = {&ws->gX32, &ws->gV32, &ws->gXs32, &ws->abar32, &ws->PI32, &ws->Vtmp32,
+                   &ws->t2f, &ws->xr32};
+  for (auto pp : f2) dfree(*pp);
+  dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
+  dfree(ws->galpha32); dfree(ws->grho32);
+}
+
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
                          sph_workspace **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
     check_params(p);
-    need_f64(precision);
+    need_precision(precision);
     if (N < 1) fail(SPH_E_ARG, "bad N");
     HIPC(hipSetDevice(ctx->device));
     auto *ws = new sph_workspace;
@@ -914,30 +1062,41 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       ws->p = *p;
       ws->precision = precision;
       const size_t n = (size_t)N;
-      ws->gX = dalloc<double2>(n);
-      ws->gV = dalloc<double2>(n);
-      ws->gXs = dalloc<double2>(n);
-      ws->abar = dalloc<double2>(n);
-      ws->PI = dalloc<double2>(n);
-      ws->Vtmp = dalloc<double2>(n);
       ws->t2 = dalloc<double2>(n);
       ws->gX_id = dalloc<double2>(n);
       ws->gV_id = dalloc<double2>(n);
-      ws->galpha = dalloc<double>(n);
-      ws->grho = dalloc<double>(n);
       ws->gtheta = dalloc<double>((size_t)(p->ngx * p->ngy));
       ws->inv = dalloc<int>(n);
+      if (precision == SPH_F32C) {
+        ws->gX32 = dalloc<float2>(n);
+        ws->gV32 = dalloc<float2>(n);
+        ws->gXs32 = dalloc<float2>(n);
+        ws->abar32 = dalloc<float2>(n);
+        ws->PI32 = dalloc<float2>(n);
+        ws->Vtmp32 = dalloc<float2>(n);
+        ws->t2f = dalloc<float2>(n);
+        ws->xr32 = dalloc<float2>(n);
+        ws->galpha32 = dalloc<float>(n);
+        ws->grho32 = dalloc<float>(n);
+      } else {
+        ws->gX = dalloc<double2>(n);
+        ws->gV = dalloc<double2>(n);
+        ws->gXs = dalloc<double2>(n);
+        ws->abar = dalloc<double2>(n);
+        ws->PI = dalloc<double2>(n);
+        ws->Vtmp = dalloc<double2>(n);
+        ws->galpha = dalloc<double>(n);
+        ws->grho = dalloc<double>(n);
+      }
       HIPC(hipMemset(ws->gX_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(ws->gV_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(ws->gtheta, 0, sizeof(double) * p->ngx * p->ngy));
     } catch (...) {
-      double2 **d2[] = {&ws->gX, &ws->gV, &ws->gXs, &ws->abar, &ws->PI, &ws->Vtmp, &ws->t2,
-                        &ws->gX_id, &ws->gV_id};
-      for (auto pp : d2) dfree(*pp);
-      dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
+      ws_free(ws);
       delete ws;
       throw;
     }
+    ctx_retain(ctx);
     *out = ws;
   });
 }
@@ -946,11 +1105,10 @@ int sph_workspace_destroy(sph_workspace *ws) {
   return guarded([&] {
     if (!ws) return;
     (void)hipStreamSynchronize(ws->ctx->stream);
-    double2 **d2[] = {&ws->gX, &ws->gV, &ws->gXs, &ws->abar, &ws->PI, &ws->Vtmp, &ws->t2,
-                      &ws->gX_id, &ws->gV_id};
-    for (auto pp : d2) dfree(*pp);
-    dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
+    ws_free(ws);
+    sph_ctx *c = ws->ctx;
     delete ws;
+    ctx_release(c);
   });
 }
 
@@ -982,6 +1140,96 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
 // within skin*h (src/neighbors.jl:133-144; src/driver.jl:162-167), and reusing
 // the forward's keeps the recomputed densities bit-identical to the forward's.
 // skin, interval, maxnb and layout are validated for interface parity.
+namespace {
+
+void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur) {
+  hipStream_t s = ws->ctx->stream;
+  sph_ctx *c = ws->ctx;
+  const int N = (int)ws->N;
+  const size_t n = (size_t)N;
+  const PhysF64 P = make_phys(ws->p);
+  const Grid g = tape->g;
+  for (int64_t k = tape->n; k >= 1; --k) {
+    const int e = tape->entry_epoch[k];
+    const TapeEpoch &E = tape->epochs[e];
+    const double2 *Xk = tape->X + (size_t)(k - 1) * n;
+    const double2 *Vk = tape->V + (size_t)(k - 1) * n;
+    const double2 *Vnext = tape->V + (size_t)k * n;
+    if (e != cur) {
+      // entry k+1 was stored in the next epoch's slot order: move the
+      // adjoint state and V_{k+1} into this epoch's order
+      timed(c, K_REMAP, [&] {
+        launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
+        launch_gather_remap2(s, ws->gX, ws->inv, E.ids, N, ws->t2);
+        std::swap(ws->gX, ws->t2);
+        launch_gather_remap2(s, ws->gV, ws->inv, E.ids, N, ws->t2);
+        std::swap(ws->gV, ws->t2);
+        launch_gather_remap2(s, Vnext, ws->inv, E.ids, N, ws->Vtmp);
+      });
+      Vnext = ws->Vtmp;
+      cur = e;
+    }
+    timed(c, K_ADJ_R, [&] {
+      launch_adj_r_f64(s, Xk, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI, ws->gX, ws->gV,
+                       ws->abar, ws->galpha);
+    });
+    timed(c, K_ADJ_P1, [&] {
+      launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, E.scell, E.cstart, g, P,
+                        ws->gXs, ws->gV, ws->grho, ws->gtheta);
+    });
+    timed(c, K_ADJ_P2, [&] {
+      launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, E.scell, E.cstart, g, P, ws->gX);
+    });
+  }
+  launch_scatter2(s, ws->gX, tape->epochs[cur].ids, N, ws->gX_id);
+  launch_scatter2(s, ws->gV, tape->epochs[cur].ids, N, ws->gV_id);
+}
+
+void backward_f32c(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur,
+                   double rc) {
+  hipStream_t s = ws->ctx->stream;
+  sph_ctx *c = ws->ctx;
+  const int N = (int)ws->N;
+  const size_t n = (size_t)N;
+  const PhysF32 P = make_phys32(ws->p, rc);
+  const Grid g = tape->g;
+  for (int64_t k = tape->n; k >= 1; --k) {
+    const int e = tape->entry_epoch[k];
+    const TapeEpoch &E = tape->epochs[e];
+    const double2 *Xk = tape->X + (size_t)(k - 1) * n;
+    const float2 *Vk = tape->V32 + (size_t)(k - 1) * n;
+    const float2 *Vnext = tape->V32 + (size_t)k * n;
+    if (e != cur) {
+      timed(c, K_REMAP, [&] {
+        launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
+        launch_gather_remap2f(s, ws->gX32, ws->inv, E.ids, N, ws->t2f);
+        std::swap(ws->gX32, ws->t2f);
+        launch_gather_remap2f(s, ws->gV32, ws->inv, E.ids, N, ws->t2f);
+        std::swap(ws->gV32, ws->t2f);
+        launch_gather_remap2f(s, Vnext, ws->inv, E.ids, N, ws->Vtmp32);
+      });
+      Vnext = ws->Vtmp32;
+      cur = e;
+    }
+    timed(c, K_ADJ_R, [&] {
+      launch_xrel_f32c(s, Xk, E.scell, g, P, ws->xr32);
+      launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI32,
+                        ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
+    });
+    timed(c, K_ADJ_P1, [&] {
+      launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, ws->PI32, ws->abar32, ws->galpha32, theta->d,
+                         E.scell, E.cstart, g, P, ws->gXs32, ws->gV32, ws->grho32, ws->gtheta);
+    });
+    timed(c, K_ADJ_P2, [&] {
+      launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, E.cstart, g, P, ws->gX32);
+    });
+  }
+  launch_scatter2f(s, ws->gX32, tape->epochs[cur].ids, N, ws->gX_id);
+  launch_scatter2f(s, ws->gV32, tape->epochs[cur].ids, N, ws->gV_id);
+}
+
+}  // namespace
+
 int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
                  const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout) {
   return guarded([&] {
@@ -994,58 +1242,26 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, cons
     hipStream_t s = ws->ctx->stream;
     const int N = (int)ws->N;
     const size_t n = (size_t)N;
-    const PhysF64 P = make_phys(ws->p);
     HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
-    const int64_t L = tape->n;
-    if (L == 0) {
+    if (tape->n == 0) {
       HIPC(hipMemcpyAsync(ws->gX_id, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
       HIPC(hipMemcpyAsync(ws->gV_id, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
       HIPC(hipStreamSynchronize(s));
       return;
     }
-    const Grid g = tape->g;
-    int cur = tape->entry_epoch[L];
-    // seeds (id order) into the layout of the final entry
+    const int cur = tape->entry_epoch[tape->n];
+    const int *ids = tape->epochs[cur].ids;
+    // seeds (id order) into the slot order of the final entry
+    const bool f32c = ws->precision == SPH_F32C;
     HIPC(hipMemcpyAsync(ws->t2, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    launch_gather2(s, ws->t2, tape->epochs[cur].ids, N, ws->gX);
+    if (f32c) launch_gather2f(s, ws->t2, ids, N, ws->gX32);
+    else launch_gather2(s, ws->t2, ids, N, ws->gX);
     HIPC(hipStreamSynchronize(s));
     HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    launch_gather2(s, ws->t2, tape->epochs[cur].ids, N, ws->gV);
-    for (int64_t k = L; k >= 1; --k) {
-      const int e = tape->entry_epoch[k];
-      const TapeEpoch &E = tape->epochs[e];
-      const double2 *Xk = tape->X + (size_t)(k - 1) * n;
-      const double2 *Vk = tape->V + (size_t)(k - 1) * n;
-      const double2 *Vnext = tape->V + (size_t)k * n;
-      if (e != cur) {
-        // entry k+1 was stored in the next epoch's slot order: move the
-        // adjoint state and V_{k+1} into this epoch's order
-        timed(ws->ctx, K_REMAP, [&] {
-          launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
-          launch_gather_remap2(s, ws->gX, ws->inv, E.ids, N, ws->t2);
-          std::swap(ws->gX, ws->t2);
-          launch_gather_remap2(s, ws->gV, ws->inv, E.ids, N, ws->t2);
-          std::swap(ws->gV, ws->t2);
-          launch_gather_remap2(s, Vnext, ws->inv, E.ids, N, ws->Vtmp);
-        });
-        Vnext = ws->Vtmp;
-        cur = e;
-      }
-      sph_ctx *c = ws->ctx;
-      timed(c, K_ADJ_R, [&] {
-        launch_adj_r_f64(s, Xk, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI, ws->gX, ws->gV,
-                         ws->abar, ws->galpha);
-      });
-      timed(c, K_ADJ_P1, [&] {
-        launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, E.scell, E.cstart, g,
-                          P, ws->gXs, ws->gV, ws->grho, ws->gtheta);
-      });
-      timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, E.scell, E.cstart, g, P, ws->gX);
-      });
-    }
-    launch_scatter2(s, ws->gX, tape->epochs[cur].ids, N, ws->gX_id);
-    launch_scatter2(s, ws->gV, tape->epochs[cur].ids, N, ws->gV_id);
+    if (f32c) launch_gather2f(s, ws->t2, ids, N, ws->gV32);
+    else launch_gather2(s, ws->t2, ids, N, ws->gV);
+    if (f32c) backward_f32c(ws, tape, theta, cur, tape->rc);
+    else backward_f64(ws, tape, theta, cur);
     HIPC(hipStreamSynchronize(s));
     prof_collect(ws->ctx);
   });

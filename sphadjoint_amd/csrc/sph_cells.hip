@@ -324,3 +324,58 @@ void launch_copy2(hipStream_t s, const double2 *src, int N, double2 *dst) {
 }
 
 }  // namespace sph
+
+// ---- float2 variants for the f32c form --------------------------------------
+#include "sph_f32c.h"
+
+namespace sph {
+namespace {
+__global__ void k_f2_to_d2(const float2 *__restrict__ src, int N, double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = make_double2(src[s].x, src[s].y);
+}
+__global__ void k_d2_to_f2(const double2 *__restrict__ src, int N, float2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = make_float2((float)src[s].x, (float)src[s].y);
+}
+__global__ void k_gather_remap2f(const float2 *__restrict__ src, const int *__restrict__ inv_src,
+                                 const int *__restrict__ ids_dst, int N, float2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[s] = src[inv_src[ids_dst[s]]];
+}
+// dst[s] = (float2) src[ids[s]]: id-order FP64 host data into the slot order
+__global__ void k_gather2f(const double2 *__restrict__ src, const int *__restrict__ ids, int N,
+                           float2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) {
+    double2 v = src[ids[s]];
+    dst[s] = make_float2((float)v.x, (float)v.y);
+  }
+}
+__global__ void k_scatter2f(const float2 *__restrict__ src, const int *__restrict__ ids, int N,
+                            double2 *__restrict__ dst) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) dst[ids[s]] = make_double2(src[s].x, src[s].y);
+}
+}  // namespace
+
+void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst) {
+  if (N > 0) k_f2_to_d2<<<nblocks(N), kBlock, 0, s>>>(src, N, dst);
+}
+void launch_d2_to_f2(hipStream_t s, const double2 *src, int N, float2 *dst) {
+  if (N > 0) k_d2_to_f2<<<nblocks(N), kBlock, 0, s>>>(src, N, dst);
+}
+void launch_gather_remap2f(hipStream_t s, const float2 *src, const int *inv_src,
+                           const int *ids_dst, int N, float2 *dst) {
+  if (N > 0) k_gather_remap2f<<<nblocks(N), kBlock, 0, s>>>(src, inv_src, ids_dst, N, dst);
+}
+void launch_gather2f(hipStream_t s, const double2 *src, const int *ids, int N, float2 *dst) {
+  if (N > 0) k_gather2f<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_scatter2f(hipStream_t s, const float2 *src, const int *ids, int N, double2 *dst) {
+  if (N > 0) k_scatter2f<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_scatter_pi(hipStream_t s, const float2 *src, const int *ids, int N, double2 *dst) {
+  launch_scatter2f(s, src, ids, N, dst);
+}
+}  // namespace sph

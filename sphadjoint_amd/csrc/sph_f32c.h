@@ -1,0 +1,48 @@
+// sph_f32c.h -- constants and launchers of the FP32-compensated form.
+#pragma once
+#include "sph_internal.h"
+
+namespace sph {
+
+// Constants of the f32c kernels.  Theta-grid quantities stay in FP64 because
+// interpolation reads the FP64 position; pair constants are FP32.
+struct PhysF32 {
+  float m, rho0, c2, mu, gx, gy, dt, Lx, Ly, kw;
+  float mA, invh, r2max, r2min, Fc, Gc, twomumass, cs;
+  float inv_hx, inv_hy;
+  double dt64, hx, hy, gxmax, gymax, gxclamp, gyclamp;
+  int ngx, ngy;
+};
+
+void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
+                         Grid g, PhysF32 P, float2 *PI, double *rho);
+void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
+                       const float2 *PI, const double *theta, const int *bcell, const int *cstart,
+                       Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn, double2 *tapeX,
+                       float2 *tapeV, double2 *a_out, double *alpha_out);
+void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
+                      float2 *xr);
+void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
+                         const float2 *V, const double2 *Xref, int check, Grid g, PhysF32 P,
+                         double dmax2, double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn,
+                         int *bcell, int *flags);
+void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
+                       const double *theta, const int *bcell, const int *cstart, Grid g, PhysF32 P,
+                       float2 *PI, float2 *gX, float2 *gV, float2 *abar, float *galpha);
+void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
+                        const float2 *PI, const float2 *abar, const float *galpha,
+                        const double *theta, const int *bcell, const int *cstart, Grid g,
+                        PhysF32 P, float2 *gXs, float2 *gV, float *grho, double *gtheta);
+void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
+                        const int *bcell, const int *cstart, Grid g, PhysF32 P, float2 *gX);
+
+// float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
+void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);
+void launch_d2_to_f2(hipStream_t s, const double2 *src, int N, float2 *dst);
+void launch_gather_remap2f(hipStream_t s, const float2 *src, const int *inv_src,
+                           const int *ids_dst, int N, float2 *dst);
+void launch_gather2f(hipStream_t s, const double2 *src, const int *ids, int N, float2 *dst);
+void launch_scatter2f(hipStream_t s, const float2 *src, const int *ids, int N, double2 *dst);
+void launch_scatter_pi(hipStream_t s, const float2 *src, const int *ids, int N, double2 *dst);
+
+}  // namespace sph

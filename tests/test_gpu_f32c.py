@@ -1,0 +1,120 @@
+"""GPU checks of the FP32-compensated ("f32c") form against the FP64 oracle.
+
+The f32c form is a different arithmetic, so the bar is an error bound, not
+bit-equality: state and gradients within the tolerances written below on short
+horizons, a fixed summation order (J bit-identical over repeated runs), and the
+same interaction set as FP64 (positions stay within ~1e-9 of FP64 over tens of
+steps).  Measured errors are printed for the record.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import sphadjoint_amd as S
+from oracle import oracle as O
+from sphadjoint_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def be():
+    return S.MI355XBackend(0)
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("nsteps", [1, 25, 200])
+def test_f32c_forward_tracks_fp64(be, nsteps):
+    say("f32c forward dambreak", nsteps)
+    sc = scenes.dambreak_40x40(nsteps=nsteps)
+    pg = S.SPHParams(**sc.params)
+    po = O.SPHParams(**sc.params)
+    g = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+    o = O.State(sc.X0, sc.V0, po)
+    S.simulate_(g, sc.theta, pg, be, nsteps)
+    O.simulate_(o, sc.theta, po, nsteps)
+    ex = np.abs(g.X - o.X).max()
+    ev = relerr(g.V, o.V)
+    er = relerr(g.rho, o.rho)
+    ep = relerr(g.pterm, o.pterm)
+    print(f"   X abs {ex:.3e}  V rel {ev:.3e}  rho rel {er:.3e}  pterm rel {ep:.3e}")
+    assert ex < (1e-8 if nsteps <= 25 else 1e-7)
+    assert er < 1e-6
+    assert ep < 1e-4
+    assert ev < 1e-3
+
+
+def test_f32c_cell_list_matches(be):
+    say("f32c cell list")
+    sc = scenes.dambreak_40x40()
+    pg = S.SPHParams(**sc.params)
+    po = O.SPHParams(**sc.params)
+    g = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+    o = O.State(sc.X0, sc.V0, po, layout="slot")
+    S.build_neighbors_(g.nl)
+    O.build_neighbors_(o)
+    for a, b in zip(g.cl.arrays(), o.cell_list()):
+        np.testing.assert_array_equal(a, b)
+
+
+def _grads(be, sc, nsteps, interval, cX, cV, precision):
+    pg = S.SPHParams(**sc.params)
+    st = S.State(be, sc.X0, sc.V0, pg, interval=interval, precision=precision)
+    tape = S.Tape(be, sc.N, nsteps, pg, precision=precision)
+    S.simulate_(st, sc.theta, pg, be, nsteps, tape=tape)
+    ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
+    S.backward_(ws, tape, sc.theta, pg, be, seedX=cX, seedV=cV, interval=interval)
+    J = float((st.X * cX).sum() + (st.V * cV).sum())
+    return J, ws.gX, ws.gV, ws.gtheta
+
+
+def test_f32c_adjoint_gradcheck_scene(be):
+    say("f32c adjoint, gradcheck 5x5, 12 steps")
+    sc = scenes.gradcheck_5x5(nsteps=12)
+    cX, cV = sc.extra["cX"], sc.extra["cV"]
+    J, gX, gV, gth = _grads(be, sc, 12, 1, cX, cV, "f32c")
+    po = O.SPHParams(**sc.params)
+    o = O.State(sc.X0, sc.V0, po, interval=1)
+    to = O.Tape(o.N, 12, po)
+    O.simulate_(o, sc.theta, po, 12, to)
+    wo = O.AdjointWorkspace(o.N, po)
+    O.backward_(wo, to, sc.theta, po, seedX=cX, seedV=cV, interval=1)
+    e = [relerr(gX, wo.gX), relerr(gV, wo.gV), relerr(gth, wo.gtheta)]
+    print("   gX %.3e gV %.3e gtheta %.3e" % tuple(e))
+    assert max(e) < 1e-4
+
+
+def test_f32c_adjoint_brinkman_200(be):
+    say("f32c adjoint, brinkman 56x37, 200 steps, vs FP64 GPU")
+    sc = scenes.brinkman_56x37(nsteps=200)
+    rng = np.random.default_rng(5)
+    cX = rng.standard_normal((2, sc.N))
+    cV = rng.standard_normal((2, sc.N))
+    J64, gX64, gV64, gth64 = _grads(be, sc, 200, 4, cX, cV, "f64")
+    J32, gX32, gV32, gth32 = _grads(be, sc, 200, 4, cX, cV, "f32c")
+    eJ = abs(J32 - J64) / abs(J64)
+    e = [relerr(gX32, gX64), relerr(gV32, gV64), relerr(gth32, gth64)]
+    print("   J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
+    assert eJ < 1e-6
+    assert max(e) < 1e-3
+
+
+def test_f32c_repeat_runs_bit_identical(be):
+    say("f32c determinism")
+    sc = scenes.brinkman_56x37(nsteps=100)
+    pg = S.SPHParams(**sc.params)
+    Js = []
+    for _ in range(3):
+        st = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+        S.simulate_(st, sc.theta, pg, be, 100)
+        Js.append(S.target_objective(st, 0.3, 0.2, 0.1)[0])
+    assert Js[0] == Js[1] == Js[2]
