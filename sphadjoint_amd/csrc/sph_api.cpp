@@ -147,6 +147,20 @@ PhysF32 make_phys32(const sph_params &p, double rc) {
   return P;
 }
 
+ThetaGrid make_theta_grid(const sph_params &p, double rc) {
+  ThetaGrid T{};
+  T.ngx = (int)p.ngx;
+  T.ngy = (int)p.ngy;
+  T.hx = p.Lx / (double)(p.ngx - 1);
+  T.hy = p.Ly / (double)(p.ngy - 1);
+  T.inv_hx = 1.0 / T.hx;
+  T.inv_hy = 1.0 / T.hy;
+  T.gxclamp = (double)(p.ngx - 1) - 1e-6;
+  T.gyclamp = (double)(p.ngy - 1) - 1e-6;
+  T.cs = rc;
+  return T;
+}
+
 void check_params(const sph_params *p) {
   if (!p) fail(SPH_E_ARG, "params is NULL");
   if (!(p->h > 0) || !(p->dt > 0) || !(p->Lx > 0) || !(p->Ly > 0))
@@ -161,11 +175,11 @@ void check_params(const sph_params *p) {
 // Kernels timed by the profiler (sph_profile_*).
 enum KernelId {
   K_ASSIGN, K_SCAN, K_FILL, K_RANK, K_PERMUTE, K_DENSITY, K_FORCE, K_ADJ_R, K_ADJ_P1, K_ADJ_P2,
-  K_REMAP, K_COUNT
+  K_REMAP, K_THETA, K_NLBUILD, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
   "cell_assign", "cell_scan", "cell_fill", "cell_rank", "slot_permute", "density", "force_integrate",
-  "adjoint_r", "adjoint_p1", "adjoint_p2", "layout_remap"};
+  "adjoint_r", "adjoint_p1", "adjoint_p2", "layout_remap", "theta_gather", "nl_build"};
 
 struct ProfRec {
   int kid;
@@ -266,6 +280,8 @@ struct sph_state {
   // current layout; for the f32c form scell holds the binning cell of each slot
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
   float2 *V32 = nullptr, *V32n = nullptr, *xr = nullptr, *xrn = nullptr, *PI32 = nullptr;
+  // transposed neighbour list of the current layout (entry m of slot s at nl[m*N + s])
+  int *nl = nullptr, *nbcnt = nullptr;
   double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
   double2 *Xref = nullptr, *Xref2 = nullptr;
   double2 *PI = nullptr;
@@ -293,6 +309,7 @@ struct sph_state {
 struct TapeEpoch {
   uint64_t serial = 0;
   const sph_state *owner = nullptr;
+  int64_t first = 0;  // first tape entry of the epoch
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
 };
 
@@ -309,6 +326,7 @@ struct sph_tape {
   Grid g{};
   PhysF64 P{};
   double rc = 0.0;  // cell width of the recorded epochs
+  CellConstF64 kc{};
   int ncell = 0;
 };
 
@@ -321,6 +339,9 @@ struct sph_workspace {
   double2 *Vtmp = nullptr, *t2 = nullptr, *gX_id = nullptr, *gV_id = nullptr;
   double *galpha = nullptr, *grho = nullptr, *gtheta = nullptr;
   int *inv = nullptr;
+  double4 *tpart = nullptr;  // per theta cell partial sums of the gtheta gather
+  int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
+  int64_t nl_cap = 0;                                      // maxnb the list was sized for
   // f32c form
   float2 *gX32 = nullptr, *gV32 = nullptr, *gXs32 = nullptr, *abar32 = nullptr, *PI32 = nullptr;
   float2 *Vtmp32 = nullptr, *t2f = nullptr, *xr32 = nullptr;
@@ -332,7 +353,7 @@ namespace {
 void state_free(sph_state *st) {
   int **ints[] = {&st->ids, &st->scell, &st->cstart, &st->cellof, &st->counts, &st->cl_cstart,
                   &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
-                  &st->flags, &st->nl_idx, &st->nl_cnt};
+                  &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt};
   for (auto pp : ints) dfree(*pp);
   double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
                     &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
@@ -373,14 +394,14 @@ void materialise_aux(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
-    launch_density_f32c(s, st->xrn, st->scell, st->cstart, st->g, st->P32, st->PI32, st->rho_slot);
-    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell,
-                      st->cstart, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
+    launch_density_f32c(s, st->xrn, st->nl, st->nbcnt, st->g, st->P32, st->PI32, st->rho_slot);
+    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->nl,
+                      st->nbcnt, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
                       st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
-    launch_density_f64(s, st->Xn, st->scell, st->cstart, st->g, st->P, st->PI, st->rho_slot, nullptr);
-    launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->scell, st->cstart, st->g, st->P,
+    launch_density_f64(s, st->Xn, st->nl, st->nbcnt, st->g, st->P, st->PI, st->rho_slot, nullptr);
+    launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->nl, st->nbcnt, st->g, st->P,
                      nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
     launch_scatter2(s, st->PI, st->ids, N, st->pi_id);
   }
@@ -414,6 +435,15 @@ void do_build_neighbors(sph_state *st) {
   std::swap(st->ids, st->new_ids);
   std::swap(st->cstart, st->cl_cstart);
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
+  timed(st->ctx, K_NLBUILD, [&] {
+    if (st->precision == SPH_F32C)
+      launch_nl_build_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32,
+                           (float)(st->kc.cs * st->kc.cs), (int)st->maxnb, st->nl, st->nbcnt,
+                           st->flags);
+    else
+      launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
+                          st->nbcnt, st->flags);
+  });
   st->epoch += 1;
   st->age = 0;
 }
@@ -429,6 +459,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
     tp->g = st->g;
     tp->P = st->P;
     tp->rc = st->kc.cs;
+    tp->kc = st->kc;
     tp->ncell = st->g.ncell;
     tp->has_geom = true;
   } else if (tp->ncell != st->g.ncell || tp->g.nx != st->g.nx) {
@@ -445,6 +476,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
   }
   e.owner = st;
   e.serial = st->epoch;
+  e.first = tp->n + 1;
   hipStream_t s = st->ctx->stream;
   HIPC(hipMemcpyAsync(e.ids, st->ids, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
   HIPC(hipMemcpyAsync(e.scell, st->scell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
@@ -471,11 +503,11 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32, st->PI32, nullptr);
+      launch_density_f32c(s, st->xr, st->nl, st->nbcnt, st->g, st->P32, st->PI32, nullptr);
     });
     timed(st->ctx, K_FORCE, [&] {
-      launch_force_f32c(s, st->X, st->xr, st->V32, st->PI32, theta, st->scell, st->cstart, st->g,
-                        st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
+      launch_force_f32c(s, st->X, st->xr, st->V32, st->PI32, theta, st->scell, st->nl, st->nbcnt,
+                        st->g, st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
     std::swap(st->xr, st->xrn);
@@ -484,10 +516,10 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
     return;
   }
   timed(st->ctx, K_DENSITY, [&] {
-    launch_density_f64(s, st->X, st->scell, st->cstart, st->g, st->P, st->PI, nullptr, tX);
+    launch_density_f64(s, st->X, st->nl, st->nbcnt, st->g, st->P, st->PI, nullptr, tX);
   });
   timed(st->ctx, K_FORCE, [&] {
-    launch_force_f64(s, st->X, st->V, st->PI, theta, st->scell, st->cstart, st->g, st->P, st->Xn,
+    launch_force_f64(s, st->X, st->V, st->PI, theta, st->nl, st->nbcnt, st->g, st->P, st->Xn,
                      st->Vn, tV, nullptr, nullptr);
   });
   std::swap(st->X, st->Xn);
@@ -627,6 +659,8 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
     check_params(p);
     need_precision(precision);
     if (N < 1 || N > (int64_t)1 << 30) fail(SPH_E_ARG, "N=%lld out of range", (long long)N);
+    if (precision == SPH_F32C && N >= (int64_t)1 << 27)
+      fail(SPH_E_ARG, "the f32c form packs slot indices in 27 bits: N=%lld too large", (long long)N);
     if (!X0 || !V0) fail(SPH_E_ARG, "X0 and V0 are required");
     if (!(skin >= 0)) fail(SPH_E_ARG, "skin must be >= 0");
     if (interval < 1) fail(SPH_E_ARG, "interval must be >= 1");
@@ -687,6 +721,8 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
         st->Vn = dalloc<double2>(n);
         st->PI = dalloc<double2>(n);
       }
+      st->nl = dalloc<int>(n * (size_t)maxnb);
+      st->nbcnt = dalloc<int>(n);
       st->cellof = dalloc<int>(n);
       st->counts = dalloc<int>(nc);
       st->cl_cstart = dalloc<int>(nc + 1);
@@ -1044,7 +1080,8 @@ void ws_free(sph_workspace *ws) {
                    &ws->t2f, &ws->xr32};
   for (auto pp : f2) dfree(*pp);
   dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
-  dfree(ws->galpha32); dfree(ws->grho32);
+  dfree(ws->galpha32); dfree(ws->grho32); dfree(ws->tpart);
+  dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags);
 }
 
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
@@ -1067,6 +1104,10 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       ws->gV_id = dalloc<double2>(n);
       ws->gtheta = dalloc<double>((size_t)(p->ngx * p->ngy));
       ws->inv = dalloc<int>(n);
+      ws->tpart = dalloc<double4>((size_t)((p->ngx - 1) * (p->ngy - 1)));
+      ws->nbcnt = dalloc<int>(n);
+      ws->flags = dalloc<int>(2);
+      HIPC(hipMemset(ws->flags, 0, sizeof(int) * 2));
       if (precision == SPH_F32C) {
         ws->gX32 = dalloc<float2>(n);
         ws->gV32 = dalloc<float2>(n);
@@ -1142,16 +1183,26 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
 // skin, interval, maxnb and layout are validated for interface parity.
 namespace {
 
-void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur) {
+void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur, int maxnb) {
   hipStream_t s = ws->ctx->stream;
   sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
   const PhysF64 P = make_phys(ws->p);
   const Grid g = tape->g;
+  const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
+  int listed = -1;  // epoch whose neighbour list ws->nl holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
+    if (e != listed) {
+      // the epoch's list, rebuilt from the positions of its first entry
+      timed(c, K_NLBUILD, [&] {
+        launch_nl_build_f64(s, tape->X + (size_t)(E.first - 1) * n, E.scell, E.cstart, g, tape->kc,
+                            maxnb, ws->nl, ws->nbcnt, ws->flags);
+      });
+      listed = e;
+    }
     const double2 *Xk = tape->X + (size_t)(k - 1) * n;
     const double2 *Vk = tape->V + (size_t)(k - 1) * n;
     const double2 *Vnext = tape->V + (size_t)k * n;
@@ -1170,15 +1221,18 @@ void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int
       cur = e;
     }
     timed(c, K_ADJ_R, [&] {
-      launch_adj_r_f64(s, Xk, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI, ws->gX, ws->gV,
+      launch_adj_r_f64(s, Xk, Vnext, theta->d, ws->nl, ws->nbcnt, g, P, ws->PI, ws->gX, ws->gV,
                        ws->abar, ws->galpha);
     });
+    timed(c, K_THETA, [&] {
+      launch_theta_gather_f64(s, Xk, ws->galpha, E.cstart, g, T, ws->tpart, ws->gtheta);
+    });
     timed(c, K_ADJ_P1, [&] {
-      launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, E.scell, E.cstart, g, P,
-                        ws->gXs, ws->gV, ws->grho, ws->gtheta);
+      launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, ws->nl, ws->nbcnt, g, P,
+                        ws->gXs, ws->gV, ws->grho);
     });
     timed(c, K_ADJ_P2, [&] {
-      launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, E.scell, E.cstart, g, P, ws->gX);
+      launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, ws->nl, ws->nbcnt, g, P, ws->gX);
     });
   }
   launch_scatter2(s, ws->gX, tape->epochs[cur].ids, N, ws->gX_id);
@@ -1186,16 +1240,26 @@ void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int
 }
 
 void backward_f32c(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur,
-                   double rc) {
+                   double rc, int maxnb) {
   hipStream_t s = ws->ctx->stream;
   sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
   const PhysF32 P = make_phys32(ws->p, rc);
   const Grid g = tape->g;
+  const ThetaGrid T = make_theta_grid(ws->p, rc);
+  int listed = -1;  // epoch whose neighbour list ws->nl holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
+    if (e != listed) {
+      timed(c, K_NLBUILD, [&] {
+        launch_xrel_f32c(s, tape->X + (size_t)(E.first - 1) * n, E.scell, g, P, ws->xr32);
+        launch_nl_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P, (float)(rc * rc), maxnb, ws->nl,
+                             ws->nbcnt, ws->flags);
+      });
+      listed = e;
+    }
     const double2 *Xk = tape->X + (size_t)(k - 1) * n;
     const float2 *Vk = tape->V32 + (size_t)(k - 1) * n;
     const float2 *Vnext = tape->V32 + (size_t)k * n;
@@ -1213,15 +1277,18 @@ void backward_f32c(sph_workspace *ws, sph_tape *tape, const sph_field *theta, in
     }
     timed(c, K_ADJ_R, [&] {
       launch_xrel_f32c(s, Xk, E.scell, g, P, ws->xr32);
-      launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, E.cstart, g, P, ws->PI32,
+      launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, ws->nl, ws->nbcnt, g, P, ws->PI32,
                         ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
     });
+    timed(c, K_THETA, [&] {
+      launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
+    });
     timed(c, K_ADJ_P1, [&] {
-      launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, ws->PI32, ws->abar32, ws->galpha32, theta->d,
-                         E.scell, E.cstart, g, P, ws->gXs32, ws->gV32, ws->grho32, ws->gtheta);
+      launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, ws->PI32, ws->abar32, ws->galpha32, theta->d, ws->nl,
+                         ws->nbcnt, g, P, ws->gXs32, ws->gV32, ws->grho32);
     });
     timed(c, K_ADJ_P2, [&] {
-      launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, E.cstart, g, P, ws->gX32);
+      launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, ws->nl, ws->nbcnt, g, P, ws->gX32);
     });
   }
   launch_scatter2f(s, ws->gX32, tape->epochs[cur].ids, N, ws->gX_id);
@@ -1260,8 +1327,13 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, cons
     HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
     if (f32c) launch_gather2f(s, ws->t2, ids, N, ws->gV32);
     else launch_gather2(s, ws->t2, ids, N, ws->gV);
-    if (f32c) backward_f32c(ws, tape, theta, cur, tape->rc);
-    else backward_f64(ws, tape, theta, cur);
+    if (ws->nl_cap < maxnb) {
+      dfree(ws->nl);
+      ws->nl = dalloc<int>(n * (size_t)maxnb);
+      ws->nl_cap = maxnb;
+    }
+    if (f32c) backward_f32c(ws, tape, theta, cur, tape->rc, (int)maxnb);
+    else backward_f64(ws, tape, theta, cur, (int)maxnb);
     HIPC(hipStreamSynchronize(s));
     prof_collect(ws->ctx);
   });

@@ -8,13 +8,18 @@
 //   * velocities, pterm/invrho and the adjoint state are FP32;
 //   * density, acceleration and adjoint pair sums use TwoSum (Knuth) error-free
 //     accumulation; rho - rho0 is formed from the compensated pair;
-//   * gtheta accumulates in FP64 atomics, J in FP64;
+//   * gtheta is reduced in FP64 by the deterministic gather (sph_theta.hip),
+//     J in FP64;
 //   * square roots use v_sqrt_f32 / v_rsq_f32 (1 ulp) instead of the correctly
 //     rounded expansions, and 1/r is an rsq instead of a division.
-// The sweep is the 3x3 cell block around the particle's own cell, in the
-// reference's cell order (ddy outer, ddx inner, ascending id inside a cell),
-// so every sum has a fixed order and repeated runs are bit-identical.
-// Physics follows src/forward.jl:25-147 and src/adjoint.jl:30-228.
+// Each rebuild builds a transposed neighbour list (entry m of slot s at
+// nl[m*N + s]) of the 3x3 cell block around the particle's own cell, in the
+// reference's order (ddy outer, ddx inner, ascending id inside a cell); an
+// entry packs the neighbour slot with the code of its cell in the block,
+// (j << 4) | code, from which the exact FP32 shift between the two cell
+// frames follows.  Every sum has a fixed order: repeated runs are
+// bit-identical.  Physics follows src/forward.jl:25-147 and
+// src/adjoint.jl:30-228.
 #include "sph_f32c.h"
 
 namespace sph {
@@ -97,51 +102,90 @@ __device__ __forceinline__ float2 rel_coord(double2 x, int cell, const Grid &g, 
   return make_float2((float)(x.x - ox), (float)(x.y - oy));
 }
 
-// Visits the nine cells of the 3x3 block around cell `bc` in the reference's
-// order; for each cell calls body(j0, j1, sx, sy) where [j0, j1) is the cell's
-// slot run and (sx, sy) is the shift that moves an xr of that cell into the
-// frame of cell bc (the shifts are exact multiples of cs in FP32).
-template <class Body>
-__device__ __forceinline__ void sweep9(const int *__restrict__ cstart, int bc, const Grid &g,
-                                       float cs, Body &&body) {
-  int by = bc / g.nx;
-  int bx = bc - by * g.nx;
-  int x0 = bx > 0 ? bx - 1 : 0;
-  int x1 = bx < g.nx - 1 ? bx + 1 : g.nx - 1;
+// shift that moves an xr of the cell with code (dy+1)*3 + (dx+1), where
+// (dx, dy) is its offset from the sweeping particle's cell, into the frame of
+// that particle (exact multiples of cs in FP32)
+__device__ __forceinline__ float2 shift_of(int code, float cs) {
+  const int row = (code * 11) >> 5;            // code / 3 for code < 9
+  const int col = code - row * 3;
+  return make_float2((float)(col - 1) * cs, (float)(row - 1) * cs);
+}
+
+struct Pair32 {
+  float dx, dy, r2;
+  int j;
+};
+// pair of the sweeping particle (own frame xi) with list entry e
+__device__ __forceinline__ Pair32 pair32(float2 xi, const float2 *__restrict__ xr, int e,
+                                         float cs) {
+  const int j = e >> 4;
+  const float2 sh = shift_of(e & 15, cs);
+  const float2 xj = xr[j];
+  Pair32 p;
+  p.dx = (xi.x - sh.x) - xj.x;
+  p.dy = (xi.y - sh.y) - xj.y;
+  p.r2 = p.dx * p.dx + p.dy * p.dy;
+  p.j = j;
+  return p;
+}
+
+// neighbour list of the 3x3 block around the binning cell, cutoff rc; N < 2^27
+__global__ void __launch_bounds__(kBlock)
+k_nl_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
+             const int *__restrict__ cstart, Grid g, PhysF32 P, float rc2, int maxnb,
+             int *__restrict__ nl, int *__restrict__ nbcnt, int *__restrict__ flags) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  const float2 xi = xr[s];
+  const int bc = bcell[s];
+  const int by = bc / g.nx;
+  const int bx = bc - by * g.nx;
+  int m = 0;
+  bool over = false;
+#pragma unroll
   for (int dyi = -1; dyi <= 1; ++dyi) {
-    int jy = by + dyi;
+    const int jy = by + dyi;
     if (jy < 0 || jy >= g.ny) continue;
-    const int row = jy * g.nx;
-    float sy = (float)dyi * cs;
-    int b = cstart[row + x0];
-    for (int jx = x0; jx <= x1; ++jx) {
-      int e = cstart[row + jx + 1];
-      float sx = (float)(jx - bx) * cs;
-      body(b, e, sx, sy);
-      b = e;
+    const int base = jy * g.nx;
+#pragma unroll
+    for (int dxi = -1; dxi <= 1; ++dxi) {
+      const int jx = bx + dxi;
+      if (jx < 0 || jx >= g.nx) continue;
+      const int code = (dyi + 1) * 3 + (dxi + 1);
+      const float2 sh = shift_of(code, P.cs);
+      const float xs = xi.x - sh.x, ys = xi.y - sh.y;
+      const int e = cstart[base + jx + 1];
+      for (int j = cstart[base + jx]; j < e; ++j) {
+        const float2 xj = xr[j];
+        const float dx = xs - xj.x, dy = ys - xj.y;
+        const float r2 = dx * dx + dy * dy;
+        if (r2 < rc2 && r2 > P.r2min) {
+          if (m < maxnb) {
+            nl[(size_t)m * g.N + s] = (j << 4) | code;
+            ++m;
+          } else {
+            over = true;
+          }
+        }
+      }
     }
   }
+  nbcnt[s] = m;
+  if (over) flags[0] = 1;
 }
 
 __device__ __forceinline__ float2 density_pair(float2 xi, const float2 *__restrict__ xr,
-                                               const int *__restrict__ cstart, int bc,
-                                               const Grid &g, const PhysF32 &P) {
+                                               const NbRow &row, const PhysF32 &P) {
   CSum acc;
-  sweep9(cstart, bc, g, P.cs, [&](int j0, int j1, float sx, float sy) {
-    const float xs = xi.x - sx, ys = xi.y - sy;
-    for (int j = j0; j < j1; ++j) {
-      float2 xj = xr[j];
-      float dx = xs - xj.x;
-      float dy = ys - xj.y;
-      float r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-        float u = 1.f - 0.5f * q;
-        float u2 = u * u;
-        acc.add(P.mA * (u2 * u2 * (2.f * q + 1.f)));
-      }
+  for (int m = 0; m < row.n; ++m) {
+    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
+    if (pr.r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+      float u = 1.f - 0.5f * q;
+      float u2 = u * u;
+      acc.add(P.mA * (u2 * u2 * (2.f * q + 1.f)));
     }
-  });
+  }
   acc.add(P.mA);                  // self term
   // rho - rho0 from the compensated pair (exact subtraction, then the tail)
   float drho = (acc.s - P.rho0) + acc.c;
@@ -154,12 +198,12 @@ __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-            const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ PI,
+k_density32(const float2 *__restrict__ xr, const int *__restrict__ nl,
+            const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ PI,
             double *__restrict__ rho_out) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
-  float2 rd = density_pair(xr[s], xr, cstart, bcell[s], g, P);
+  float2 rd = density_pair(xr[s], xr, nb_row(nl, nbcnt, s, g.N), P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -168,35 +212,30 @@ __global__ void __launch_bounds__(kBlock)
 k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ V, const float2 *__restrict__ PI,
           const double *__restrict__ theta, const int *__restrict__ bcell,
-          const int *__restrict__ cstart, Grid g, PhysF32 P, double2 *__restrict__ Xn,
-          float2 *__restrict__ xrn, float2 *__restrict__ Vn, double2 *__restrict__ tapeX,
-          float2 *__restrict__ tapeV, double2 *__restrict__ a_out, double *__restrict__ alpha_out) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+          const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF32 P,
+          double2 *__restrict__ Xn, float2 *__restrict__ xrn, float2 *__restrict__ Vn,
+          double2 *__restrict__ tapeX, float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
+          double *__restrict__ alpha_out) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
-  const int bc = bcell[s];
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
   CSum ax, ay;
-  sweep9(cstart, bc, g, P.cs, [&](int j0, int j1, float sx, float sy) {
-    const float xs = xi.x - sx, ys = xi.y - sy;
-    for (int j = j0; j < j1; ++j) {
-      float2 xj = xr[j];
-      float dx = xs - xj.x;
-      float dy = ys - xj.y;
-      float r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-        float u = 1.f - 0.5f * q;
-        float F = P.Fc * u * u * u;
-        float2 pij = PI[j];
-        float PF = P.m * (pii.x + pij.x) * F;
-        float C = P.twomumass * pii.y * pij.y * F;
-        float2 vj = V[j];
-        ax.add(C * (vi.x - vj.x) - PF * dx);
-        ay.add(C * (vi.y - vj.y) - PF * dy);
-      }
+  for (int m = 0; m < row.n; ++m) {
+    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
+    if (pr.r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+      float u = 1.f - 0.5f * q;
+      float F = P.Fc * u * u * u;
+      float2 pij = PI[pr.j];
+      float PF = P.m * (pii.x + pij.x) * F;
+      float C = P.twomumass * pii.y * pij.y * F;
+      float2 vj = V[pr.j];
+      ax.add(C * (vi.x - vj.x) - PF * pr.dx);
+      ay.add(C * (vi.y - vj.y) - PF * pr.dy);
     }
-  });
+  }
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
   float axv = ax.value() + (P.gx + wall_accel((float)x64.x, P.Lx, P.kw));
@@ -212,7 +251,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   Vn[s] = make_float2(v1, v2);
   double2 xn = make_double2(x64.x + P.dt64 * (double)v1, x64.y + P.dt64 * (double)v2);
   Xn[s] = xn;
-  xrn[s] = rel_coord(xn, bc, g, P);
+  xrn[s] = rel_coord(xn, bcell[s], g, P);
   if (tapeX) {
     tapeX[s] = x64;
     tapeV[s] = vi;
@@ -252,13 +291,13 @@ __global__ void k_permute32(const int *__restrict__ perm, const int *__restrict_
 __global__ void __launch_bounds__(kBlock)
 k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ Vnext, const double *__restrict__ theta,
-          const int *__restrict__ bcell, const int *__restrict__ cstart, Grid g, PhysF32 P,
+          const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF32 P,
           float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
           float2 *__restrict__ abar, float *__restrict__ galpha) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
-  float2 rd = density_pair(xr[s], xr, cstart, bcell[s], g, P);
+  float2 rd = density_pair(xr[s], xr, nb_row(nl, nbcnt, s, g.N), P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
@@ -277,52 +316,49 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const float2 *__restrict__ V, const float2 *__restrict__ PI,
            const float2 *__restrict__ abar, const float *__restrict__ galpha,
-           const double *__restrict__ theta, const int *__restrict__ bcell,
-           const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ gXs,
-           float2 *__restrict__ gV, float *__restrict__ grho, double *__restrict__ gtheta) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+           const double *__restrict__ theta, const int *__restrict__ nl,
+           const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ gXs,
+           float2 *__restrict__ gV, float *__restrict__ grho) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const float pti = pii.x, iri = pii.y;
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
   CSum gx, gy, gvx, gvy, grh, sumS;
-  sweep9(cstart, bcell[s], g, P.cs, [&](int j0, int j1, float sx, float sy) {
-    const float xs = xi.x - sx, ys = xi.y - sy;
-    for (int j = j0; j < j1; ++j) {
-      float2 xj = xr[j];
-      float dx = xs - xj.x;
-      float dy = ys - xj.y;
-      float r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        float rinv = __builtin_amdgcn_rsqf(r2);
-        float q = (r2 * rinv) * P.invh;
-        float u = 1.f - 0.5f * q;
-        float F = P.Fc * u * u * u;
-        float G = P.Gc * u * u;
-        float2 pij = PI[j];
-        float Pij = P.m * (pti + pij.x);
-        float Cbase = P.twomumass * iri * pij.y;
-        float C = Cbase * F;
-        float2 abj = abar[j];
-        float2 vj = V[j];
-        float dax = ab.x - abj.x;
-        float day = ab.y - abj.y;
-        float dvx = vi.x - vj.x;
-        float dvy = vi.y - vj.y;
-        float ad_d = -(dax * dx + day * dy);
-        float ad_v = dax * dvx + day * dvy;
-        float Fbar = Pij * ad_d + Cbase * ad_v;
-        float wgt = Fbar * G * P.invh * rinv;
-        float PF = Pij * F;
-        gx.add(wgt * dx - PF * dax);
-        gy.add(wgt * dy - PF * day);
-        gvx.add(C * dax);
-        gvy.add(C * day);
-        grh.add(-(C * ad_v * iri));
-        sumS.add(F * ad_d);
-      }
+  for (int m = 0; m < row.n; ++m) {
+    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
+    if (pr.r2 < P.r2max) {
+      const int j = pr.j;
+      const float dx = pr.dx, dy = pr.dy;
+      float rinv = __builtin_amdgcn_rsqf(pr.r2);
+      float q = (pr.r2 * rinv) * P.invh;
+      float u = 1.f - 0.5f * q;
+      float F = P.Fc * u * u * u;
+      float G = P.Gc * u * u;
+      float2 pij = PI[j];
+      float Pij = P.m * (pti + pij.x);
+      float Cbase = P.twomumass * iri * pij.y;
+      float C = Cbase * F;
+      float2 abj = abar[j];
+      float2 vj = V[j];
+      float dax = ab.x - abj.x;
+      float day = ab.y - abj.y;
+      float dvx = vi.x - vj.x;
+      float dvy = vi.y - vj.y;
+      float ad_d = -(dax * dx + day * dy);
+      float ad_v = dax * dvx + day * dvy;
+      float Fbar = Pij * ad_d + Cbase * ad_v;
+      float wgt = Fbar * G * P.invh * rinv;
+      float PF = Pij * F;
+      gx.add(wgt * dx - PF * dax);
+      gy.add(wgt * dy - PF * day);
+      gvx.add(C * dax);
+      gvy.add(C * day);
+      grh.add(-(C * ad_v * iri));
+      sumS.add(F * ad_d);
     }
-  });
+  }
   float S = sumS.value();
   float grv = grh.value() + (-2.f * P.m * pti * iri) * S + (P.c2 * P.m * iri * iri) * S;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
@@ -333,12 +369,6 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   interp_alpha(theta, ic, P, &dadx, &dady);
   gxv += ga * dadx;
   gyv += ga * dady;
-  double gad = (double)ga;
-  double fx = ic.fx, fy = ic.fy;
-  atomicAdd(&gtheta[ic.l00], gad * (1.0 - fx) * (1.0 - fy));
-  atomicAdd(&gtheta[ic.l00 + P.ngy], gad * fx * (1.0 - fy));
-  atomicAdd(&gtheta[ic.l00 + 1], gad * (1.0 - fx) * fy);
-  atomicAdd(&gtheta[ic.l00 + P.ngy + 1], gad * fx * fy);
   gXs[s] = make_float2(gxv, gyv);
   grho[s] = grv;
   float2 gvo = gV[s];
@@ -348,29 +378,24 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 // adjoint P2: adj_pass2_kernel! and gX += gXs (src/adjoint.jl:126-156,187-191)
 __global__ void __launch_bounds__(kBlock)
 k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
-           const float2 *__restrict__ gXs, const int *__restrict__ bcell,
-           const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ gX) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+           const float2 *__restrict__ gXs, const int *__restrict__ nl,
+           const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ gX) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const float2 xi = xr[s];
   const float gri = grho[s];
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
   CSum gx, gy;
-  sweep9(cstart, bcell[s], g, P.cs, [&](int j0, int j1, float sx, float sy) {
-    const float xs = xi.x - sx, ys = xi.y - sy;
-    for (int j = j0; j < j1; ++j) {
-      float2 xj = xr[j];
-      float dx = xs - xj.x;
-      float dy = ys - xj.y;
-      float r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-        float u = 1.f - 0.5f * q;
-        float wgt = P.m * (P.Fc * u * u * u) * (gri + grho[j]);
-        gx.add(wgt * dx);
-        gy.add(wgt * dy);
-      }
+  for (int m = 0; m < row.n; ++m) {
+    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
+    if (pr.r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+      float u = 1.f - 0.5f * q;
+      float wgt = P.m * (P.Fc * u * u * u) * (gri + grho[pr.j]);
+      gx.add(wgt * pr.dx);
+      gy.add(wgt * pr.dy);
     }
-  });
+  }
   float2 gs = gXs[s];
   float2 go = gX[s];
   gX[s] = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
@@ -378,16 +403,23 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
 
 }  // namespace
 
-void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                         Grid g, PhysF32 P, float2 *PI, double *rho) {
-  if (g.N > 0) k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, PI, rho);
+void launch_nl_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
+                          Grid g, PhysF32 P, float rc2, int maxnb, int *nl, int *nbcnt,
+                          int *flags) {
+  if (g.N > 0)
+    k_nl_build32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, rc2, maxnb, nl, nbcnt,
+                                                 flags);
+}
+void launch_density_f32c(hipStream_t s, const float2 *xr, const int *nl, const int *nbcnt, Grid g,
+                         PhysF32 P, float2 *PI, double *rho) {
+  if (g.N > 0) k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, nl, nbcnt, g, P, PI, rho);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
-                       const float2 *PI, const double *theta, const int *bcell, const int *cstart,
-                       Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn, double2 *tapeX,
-                       float2 *tapeV, double2 *a_out, double *alpha_out) {
+                       const float2 *PI, const double *theta, const int *bcell, const int *nl,
+                       const int *nbcnt, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
+                       double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out) {
   if (g.N > 0)
-    k_force32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, cstart, g, P, Xn, xrn,
+    k_force32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nl, nbcnt, g, P, Xn, xrn,
                                               Vn, tapeX, tapeV, a_out, alpha_out);
 }
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
@@ -403,23 +435,23 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                                                 Vn, xr, Xrefn, bcell, flags);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
-                       const double *theta, const int *bcell, const int *cstart, Grid g, PhysF32 P,
+                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
                        float2 *PI, float2 *gX, float2 *gV, float2 *abar, float *galpha) {
   if (g.N > 0)
-    k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, cstart, g, P, PI, gX, gV,
+    k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV,
                                               abar, galpha);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
-                        const double *theta, const int *bcell, const int *cstart, Grid g,
-                        PhysF32 P, float2 *gXs, float2 *gV, float *grho, double *gtheta) {
+                        const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
+                        float2 *gXs, float2 *gV, float *grho) {
   if (g.N > 0)
-    k_adj_p132<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, cstart, g,
-                                               P, gXs, gV, grho, gtheta);
+    k_adj_p132<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, nl, nbcnt, g, P,
+                                               gXs, gV, grho);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *bcell, const int *cstart, Grid g, PhysF32 P, float2 *gX) {
-  if (g.N > 0) k_adj_p232<<<nblocks(g.N), kBlock, 0, s>>>(xr, grho, gXs, bcell, cstart, g, P, gX);
+                        const int *nl, const int *nbcnt, Grid g, PhysF32 P, float2 *gX) {
+  if (g.N > 0) k_adj_p232<<<nblocks(g.N), kBlock, 0, s>>>(xr, grho, gXs, nl, nbcnt, g, P, gX);
 }
 
 }  // namespace sph

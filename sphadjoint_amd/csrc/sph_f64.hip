@@ -1,20 +1,21 @@
 // sph_f64.hip -- plain FP64 physics kernels for gfx950.
 //
 // Forward (src/forward.jl) and gather-form adjoint (src/adjoint.jl) over the
-// slot layout: each thread owns one slot and sweeps the three contiguous slot
-// runs of its 3x3 cell block, in the reference's neighbour order (ddy outer,
-// ddx inner, ascending particle id inside a cell).  That is exactly the order
-// of the reference's neighbour list, so every sum is accumulated in the same
-// order with the same operations.  This file is compiled with
-// -ffp-contract=off: a*b+c stays two rounded operations as in Julia, and the
-// FP64 results equal the reference's CPU Float64 arithmetic.
+// slot layout.  As in the reference, each rebuild materialises a transposed
+// fixed-row neighbour list (src/neighbors.jl:116-245): entry m of slot s is at
+// nl[m*N + s], so at a given m all lanes of a wavefront read consecutive
+// words.  The list holds every slot within 2h(1+skin) at build time, in the
+// reference's order (ddy outer, ddx inner, ascending particle id inside a
+// cell); the pair kernels keep the pairs with r^2 < (2h)^2 at the current
+// positions.  Every sum is therefore the reference's, term for term and in the
+// same order.  This file is compiled with -ffp-contract=off: a*b+c stays two
+// rounded operations as in Julia, so the FP64 results equal the reference's
+// CPU Float64 arithmetic.
 //
-// Kernels per step:
-//   forward:  density (pterm, invrho)         -> force + implicit-drag integrate
-//   adjoint:  R  (density recompute, integrate and drag adjoints)
-//             P1 (pair adjoint, design term and theta-bar atomics)
-//             P2 (density adjoint, accumulate gX)
-// All are HBM/L2-bound gathers with ~10 flop/B; there is no dense contraction.
+// Kernels: list build (per rebuild); forward density (pterm, invrho) and
+// force + implicit-drag integrate; adjoint R (density recompute, integrate
+// and drag adjoints), P1 (pair adjoint and design term), P2 (density adjoint,
+// gX += gXs).  gtheta is the deterministic gather of sph_theta.hip.
 #pragma clang fp contract(off)
 #include "sph_internal.h"
 
@@ -78,34 +79,65 @@ __device__ __forceinline__ double interp_alpha(const double *__restrict__ th, co
 
 // density sum of slot s (density_kernel!, src/forward.jl:25-65)
 __device__ __forceinline__ double density_at(const double2 *__restrict__ X, double2 xi,
-                                             const Runs &r, const PhysF64 &P) {
+                                             const NbRow &row, const PhysF64 &P) {
   double acc = 0.0;
+  for (int m = 0; m < row.n; ++m) {
+    const int j = row.at(m);
+    double2 xj = X[j];
+    double dx = xi.x - xj.x;
+    double dy = xi.y - xj.y;
+    double r2 = dx * dx + dy * dy;
+    if (r2 < P.r2max) {
+      double q = sqrt(r2) * P.invh;
+      double u = 1.0 - 0.5 * q;
+      double u2 = u * u;
+      acc += P.m * (((P.A * u2) * u2) * (2.0 * q + 1.0));
+    }
+  }
+  return acc + P.m * P.A;
+}
+
+// _nl_build! (src/neighbors.jl:192-245) into slot space: the candidates of
+// the 3x3 block around the sweep cell, kept when r2min < r2 < rc2.
+__global__ void __launch_bounds__(kBlock)
+k_nl_build(const double2 *__restrict__ X, const int *__restrict__ scell,
+           const int *__restrict__ cstart, Grid g, CellConstF64 k, int maxnb,
+           int *__restrict__ nl, int *__restrict__ nbcnt, int *__restrict__ flags) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  const double2 xi = X[s];
+  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  int m = 0;
+  bool over = false;
+#pragma unroll
   for (int w = 0; w < 3; ++w) {
     for (int j = r.b[w]; j < r.e[w]; ++j) {
       double2 xj = X[j];
       double dx = xi.x - xj.x;
       double dy = xi.y - xj.y;
       double r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        double q = sqrt(r2) * P.invh;
-        double u = 1.0 - 0.5 * q;
-        double u2 = u * u;
-        acc += P.m * (((P.A * u2) * u2) * (2.0 * q + 1.0));
+      if (k.r2min < r2 && r2 < k.rc2) {
+        if (m < maxnb) {
+          nl[(size_t)m * g.N + s] = j;
+          ++m;
+        } else {
+          over = true;
+        }
       }
     }
   }
-  return acc + P.m * P.A;
+  nbcnt[s] = m;
+  if (over) flags[0] = 1;
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_density(const double2 *__restrict__ X, const int *__restrict__ scell,
-          const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ PI,
+k_density(const double2 *__restrict__ X, const int *__restrict__ nl,
+          const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ PI,
           double *__restrict__ rho_out, double2 *__restrict__ tapeX) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s];
-  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
-  double rho = density_at(X, xi, r, P);
+  double rho = density_at(X, xi, nb_row(nl, nbcnt, s, g.N), P);
   // eos_kernel! (src/forward.jl:68-75)
   double inv = 1.0 / rho;
   PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
@@ -118,34 +150,33 @@ k_density(const double2 *__restrict__ X, const int *__restrict__ scell,
 __global__ void __launch_bounds__(kBlock)
 k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
         const double2 *__restrict__ PI, const double *__restrict__ theta,
-        const int *__restrict__ scell, const int *__restrict__ cstart, Grid g, PhysF64 P,
+        const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
         double2 *__restrict__ Xn, double2 *__restrict__ Vn, double2 *__restrict__ tapeV,
         double2 *__restrict__ a_out, double *__restrict__ alpha_out) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s], vi = V[s], pii = PI[s];
   double pti = pii.x, iri = pii.y;
-  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
   double ax = 0.0, ay = 0.0;
-  for (int w = 0; w < 3; ++w) {
-    for (int j = r.b[w]; j < r.e[w]; ++j) {
-      double2 xj = X[j];
-      double dx = xi.x - xj.x;
-      double dy = xi.y - xj.y;
-      double r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        double q = sqrt(r2) * P.invh;
-        double u = 1.0 - 0.5 * q;
-        double F = ((P.Fc * u) * u) * u;
-        double2 pij = PI[j];
-        double Pij = P.m * (pti + pij.x);
-        ax -= (Pij * F) * dx;
-        ay -= (Pij * F) * dy;
-        double C = ((P.twomumass * iri) * pij.y) * F;
-        double2 vj = V[j];
-        ax += C * (vi.x - vj.x);
-        ay += C * (vi.y - vj.y);
-      }
+  for (int m = 0; m < row.n; ++m) {
+    const int j = row.at(m);
+    double2 xj = X[j];
+    double dx = xi.x - xj.x;
+    double dy = xi.y - xj.y;
+    double r2 = dx * dx + dy * dy;
+    if (r2 < P.r2max) {
+      double q = sqrt(r2) * P.invh;
+      double u = 1.0 - 0.5 * q;
+      double F = ((P.Fc * u) * u) * u;
+      double2 pij = PI[j];
+      double Pij = P.m * (pti + pij.x);
+      ax -= (Pij * F) * dx;
+      ay -= (Pij * F) * dy;
+      double C = ((P.twomumass * iri) * pij.y) * F;
+      double2 vj = V[j];
+      ax += C * (vi.x - vj.x);
+      ay += C * (vi.y - vj.y);
     }
   }
   Interp ic = interp_cell(xi.x, xi.y, P);
@@ -170,15 +201,14 @@ k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
 // to the (V + dt a) D that the reference recomputes.
 __global__ void __launch_bounds__(kBlock)
 k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
-        const double *__restrict__ theta, const int *__restrict__ scell,
-        const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ PI,
+        const double *__restrict__ theta, const int *__restrict__ nl,
+        const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ PI,
         double2 *__restrict__ gX, double2 *__restrict__ gV, double2 *__restrict__ abar,
         double *__restrict__ galpha) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s];
-  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
-  double rho = density_at(X, xi, r, P);
+  double rho = density_at(X, xi, nb_row(nl, nbcnt, s, g.N), P);
   double inv = 1.0 / rho;
   PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
   Interp ic = interp_cell(xi.x, xi.y, P);
@@ -198,67 +228,62 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
          const double2 *__restrict__ PI, const double2 *__restrict__ abar,
          const double *__restrict__ galpha, const double *__restrict__ theta,
-         const int *__restrict__ scell, const int *__restrict__ cstart, Grid g, PhysF64 P,
-         double2 *__restrict__ gXs, double2 *__restrict__ gV, double *__restrict__ grho,
-         double *__restrict__ gtheta) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+         const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
+         double2 *__restrict__ gXs, double2 *__restrict__ gV, double *__restrict__ grho) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s], vi = V[s], ab = abar[s], pii = PI[s];
   double pti = pii.x, iri = pii.y;
   double gx = 0.0, gy = 0.0, gvx = 0.0, gvy = 0.0, grh = 0.0, sumS = 0.0;
-  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
-  for (int w = 0; w < 3; ++w) {
-    for (int j = r.b[w]; j < r.e[w]; ++j) {
-      double2 xj = X[j];
-      double dx = xi.x - xj.x;
-      double dy = xi.y - xj.y;
-      double r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        double rr = sqrt(r2);
-        double q = rr * P.invh;
-        double u = 1.0 - 0.5 * q;
-        double F = ((P.Fc * u) * u) * u;
-        double G = (P.Gc * u) * u;
-        double2 pij = PI[j];
-        double Pij = P.m * (pti + pij.x);
-        double Cbase = (P.twomumass * iri) * pij.y;
-        double C = Cbase * F;
-        double2 abj = abar[j];
-        double2 vj = V[j];
-        double dax = ab.x - abj.x;
-        double day = ab.y - abj.y;
-        double dvx = vi.x - vj.x;
-        double dvy = vi.y - vj.y;
-        double ad_d = -(dax * dx + day * dy);
-        double ad_v = dax * dvx + day * dvy;
-        gx -= (Pij * F) * dax;
-        gy -= (Pij * F) * day;
-        double Fbar = Pij * ad_d + Cbase * ad_v;
-        double wgt = ((Fbar * G) * P.invh) / rr;
-        gx += wgt * dx;
-        gy += wgt * dy;
-        gvx += C * dax;
-        gvy += C * day;
-        grh -= (C * ad_v) * iri;
-        sumS += F * ad_d;
-      }
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  for (int m = 0; m < row.n; ++m) {
+    const int j = row.at(m);
+    double2 xj = X[j];
+    double dx = xi.x - xj.x;
+    double dy = xi.y - xj.y;
+    double r2 = dx * dx + dy * dy;
+    if (r2 < P.r2max) {
+      double rr = sqrt(r2);
+      double q = rr * P.invh;
+      double u = 1.0 - 0.5 * q;
+      double F = ((P.Fc * u) * u) * u;
+      double G = (P.Gc * u) * u;
+      double2 pij = PI[j];
+      double Pij = P.m * (pti + pij.x);
+      double Cbase = (P.twomumass * iri) * pij.y;
+      double C = Cbase * F;
+      double2 abj = abar[j];
+      double2 vj = V[j];
+      double dax = ab.x - abj.x;
+      double day = ab.y - abj.y;
+      double dvx = vi.x - vj.x;
+      double dvy = vi.y - vj.y;
+      double ad_d = -(dax * dx + day * dy);
+      double ad_v = dax * dvx + day * dvy;
+      gx -= (Pij * F) * dax;
+      gy -= (Pij * F) * day;
+      double Fbar = Pij * ad_d + Cbase * ad_v;
+      double wgt = ((Fbar * G) * P.invh) / rr;
+      gx += wgt * dx;
+      gy += wgt * dy;
+      gvx += C * dax;
+      gvy += C * day;
+      grh -= (C * ad_v) * iri;
+      sumS += F * ad_d;
     }
   }
   grh += (((-2.0 * P.m) * pti) * iri) * sumS;
   grh += (((P.c2 * P.m) * iri) * iri) * sumS;
   gx += ab.x * wall_daccel(xi.x, P.Lx, P.kw);
   gy += ab.y * wall_daccel(xi.y, P.Ly, P.kw);
-  // adj_design_kernel!: gXs += galpha * grad(alpha); gtheta += bilinear weights * galpha
+  // adj_design_kernel!: gXs += galpha * grad(alpha); its gtheta scatter is the
+  // deterministic gather of sph_theta.hip
   double ga = galpha[s];
   Interp ic = interp_cell(xi.x, xi.y, P);
   double dadx, dady;
   interp_alpha(theta, ic, P, &dadx, &dady);
   gx += ga * dadx;
   gy += ga * dady;
-  atomicAdd(&gtheta[ic.l00], (ga * (1.0 - ic.fx)) * (1.0 - ic.fy));
-  atomicAdd(&gtheta[ic.l00 + P.ngy], (ga * ic.fx) * (1.0 - ic.fy));
-  atomicAdd(&gtheta[ic.l00 + 1], (ga * (1.0 - ic.fx)) * ic.fy);
-  atomicAdd(&gtheta[ic.l00 + P.ngy + 1], (ga * ic.fx) * ic.fy);
   gXs[s] = make_double2(gx, gy);
   grho[s] = grh;
   double2 gvo = gV[s];
@@ -268,27 +293,26 @@ k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
 // Adjoint pass P2: adj_pass2_kernel! (src/adjoint.jl:126-156) and gX += gXs.
 __global__ void __launch_bounds__(kBlock)
 k_adj_p2(const double2 *__restrict__ X, const double *__restrict__ grho,
-         const double2 *__restrict__ gXs, const int *__restrict__ scell,
-         const int *__restrict__ cstart, Grid g, PhysF64 P, double2 *__restrict__ gX) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+         const double2 *__restrict__ gXs, const int *__restrict__ nl,
+         const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ gX) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s];
   double gri = grho[s];
   double gx = 0.0, gy = 0.0;
-  Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
-  for (int w = 0; w < 3; ++w) {
-    for (int j = r.b[w]; j < r.e[w]; ++j) {
-      double2 xj = X[j];
-      double dx = xi.x - xj.x;
-      double dy = xi.y - xj.y;
-      double r2 = dx * dx + dy * dy;
-      if (r2 < P.r2max && r2 > P.r2min) {
-        double q = sqrt(r2) * P.invh;
-        double u = 1.0 - 0.5 * q;
-        double wgt = (P.m * (((P.Fc * u) * u) * u)) * (gri + grho[j]);
-        gx += wgt * dx;
-        gy += wgt * dy;
-      }
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  for (int m = 0; m < row.n; ++m) {
+    const int j = row.at(m);
+    double2 xj = X[j];
+    double dx = xi.x - xj.x;
+    double dy = xi.y - xj.y;
+    double r2 = dx * dx + dy * dy;
+    if (r2 < P.r2max) {
+      double q = sqrt(r2) * P.invh;
+      double u = 1.0 - 0.5 * q;
+      double wgt = (P.m * (((P.Fc * u) * u) * u)) * (gri + grho[j]);
+      gx += wgt * dx;
+      gy += wgt * dy;
     }
   }
   double2 gs = gXs[s];
@@ -339,37 +363,41 @@ __global__ void k_objective_final(const double *__restrict__ partials, int nb, d
 
 }  // namespace
 
-void launch_density_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
-                        Grid g, PhysF64 P, double2 *PI, double *rho, double2 *tapeX) {
-  if (g.N > 0) k_density<<<nblocks(g.N), kBlock, 0, s>>>(X, scell, cstart, g, P, PI, rho, tapeX);
+void launch_nl_build_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
+                         Grid g, CellConstF64 k, int maxnb, int *nl, int *nbcnt, int *flags) {
+  if (g.N > 0)
+    k_nl_build<<<nblocks(g.N), kBlock, 0, s>>>(X, scell, cstart, g, k, maxnb, nl, nbcnt, flags);
+}
+void launch_density_f64(hipStream_t s, const double2 *X, const int *nl, const int *nbcnt, Grid g,
+                        PhysF64 P, double2 *PI, double *rho, double2 *tapeX) {
+  if (g.N > 0) k_density<<<nblocks(g.N), kBlock, 0, s>>>(X, nl, nbcnt, g, P, PI, rho, tapeX);
 }
 void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
-                      const double *theta, const int *scell, const int *cstart, Grid g,
-                      PhysF64 P, double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
+                      const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF64 P,
+                      double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
                       double *alpha_out) {
   if (g.N > 0)
-    k_force<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, theta, scell, cstart, g, P, Xn, Vn, tapeV,
-                                            a_out, alpha_out);
+    k_force<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, theta, nl, nbcnt, g, P, Xn, Vn, tapeV, a_out,
+                                            alpha_out);
 }
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
-                      const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *PI,
+                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
                       double2 *gX, double2 *gV, double2 *abar, double *galpha) {
   if (g.N > 0)
-    k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, scell, cstart, g, P, PI, gX, gV,
-                                            abar, galpha);
+    k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV, abar,
+                                            galpha);
 }
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
-                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gXs,
-                       double2 *gV, double *grho, double *gtheta) {
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
+                       double2 *gV, double *grho) {
   if (g.N > 0)
-    k_adj_p1<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, abar, galpha, theta, scell, cstart, g, P,
-                                             gXs, gV, grho, gtheta);
+    k_adj_p1<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, abar, galpha, theta, nl, nbcnt, g, P, gXs,
+                                             gV, grho);
 }
 void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
-                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gX) {
-  if (g.N > 0)
-    k_adj_p2<<<nblocks(g.N), kBlock, 0, s>>>(X, grho, gXs, scell, cstart, g, P, gX);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gX) {
+  if (g.N > 0) k_adj_p2<<<nblocks(g.N), kBlock, 0, s>>>(X, grho, gXs, nl, nbcnt, g, P, gX);
 }
 int objective_blocks(int N) { return N > 0 ? nblocks(N) : 1; }
 void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,

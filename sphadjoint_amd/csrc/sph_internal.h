@@ -37,6 +37,22 @@ struct CellConstF64 {
   double cs, invcs, rc2, r2min, dmax2;
 };
 
+// Design-field grid for the gtheta gather (src/params.jl:30-32,53-68).
+struct ThetaGrid {
+  int ngx, ngy;
+  double hx, hy, inv_hx, inv_hy, gxclamp, gyclamp, cs;
+};
+
+// Slot block of this workgroup.  Workgroups are dealt round-robin to the 8
+// XCDs (each with its own L2); renumbering them so that every XCD walks one
+// contiguous range of slots keeps the three cell rows a sweep reads in that
+// XCD's L2 instead of fetching them once per XCD.
+__device__ __forceinline__ int xcd_block() {
+  const int w = blockIdx.x, nb = gridDim.x;
+  const int x = w & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (w >> 3);
+}
+
 // ---- integer helpers -------------------------------------------------------
 
 // unsafe_trunc(Int, floor(v)) + 2 clamped to [1, n], then made 0-based.
@@ -85,6 +101,23 @@ __device__ __forceinline__ Runs sweep_runs(const int *__restrict__ cstart, int s
   return r;
 }
 
+// Row of the transposed neighbour list (the reference's slot-major layout,
+// src/neighbors.jl:124-131): entry m of slot s at nl[m*N + s], so the lanes
+// of a wavefront read consecutive words for each m.
+struct NbRow {
+  const int *__restrict__ p;
+  int n, N;
+  __device__ __forceinline__ int at(int m) const { return p[(size_t)m * N]; }
+};
+__device__ __forceinline__ NbRow nb_row(const int *__restrict__ nl,
+                                        const int *__restrict__ nbcnt, int s, int N) {
+  NbRow r;
+  r.p = nl + s;
+  r.n = nbcnt[s];
+  r.N = N;
+  return r;
+}
+
 // ---- launchers ----------------------------------------------------------------
 
 // cell list (sph_cells.hip)
@@ -111,24 +144,34 @@ void launch_scatter1(hipStream_t s, const double *src, const int *ids, int N, do
 void launch_copy2(hipStream_t s, const double2 *src, int N, double2 *dst);
 
 // FP64 physics (sph_f64.hip)
-void launch_density_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
-                        Grid g, PhysF64 P, double2 *PI, double *rho, double2 *tapeX);
+void launch_nl_build_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
+                         Grid g, CellConstF64 k, int maxnb, int *nl, int *nbcnt, int *flags);
+void launch_density_f64(hipStream_t s, const double2 *X, const int *nl, const int *nbcnt, Grid g,
+                        PhysF64 P, double2 *PI, double *rho, double2 *tapeX);
 void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
-                      const double *theta, const int *scell, const int *cstart, Grid g,
-                      PhysF64 P, double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
+                      const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF64 P,
+                      double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
                       double *alpha_out);
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
-                      const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *PI,
+                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
                       double2 *gX, double2 *gV, double2 *abar, double *galpha);
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
-                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gXs,
-                       double2 *gV, double *grho, double *gtheta);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
+                       double2 *gV, double *grho);
 void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
-                       const int *scell, const int *cstart, Grid g, PhysF64 P, double2 *gX);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gX);
 void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,
                           double sigma, double *partials, int nblocks, double *J,
                           double2 *seed);
 int objective_blocks(int N);
+
+// deterministic gtheta gather (sph_theta.hip)
+void launch_theta_gather_f64(hipStream_t s, const double2 *X, const double *galpha,
+                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
+                             double *gtheta);
+void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galpha,
+                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
+                             double *gtheta);
 
 }  // namespace sph

@@ -303,3 +303,25 @@ def test_single_particle_and_boundaries(be):
         O.simulate_(o, sc.theta, po, 10)
         np.testing.assert_array_equal(g.X, o.X)
         np.testing.assert_array_equal(g.V, o.V)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_gtheta_bit_identical_over_runs(be, precision):
+    """gtheta is reduced by a fixed-order gather (no float atomics), so the
+    whole gradient, not only J, repeats bit for bit."""
+    say("gtheta determinism", precision)
+    sc = scenes.brinkman_56x37(nsteps=60)
+    pg = S.SPHParams(**sc.params)
+    rng = np.random.default_rng(2)
+    cX = rng.standard_normal((2, sc.N))
+    cV = rng.standard_normal((2, sc.N))
+    outs = []
+    for _ in range(2):
+        st = S.State(be, sc.X0, sc.V0, pg, precision=precision)
+        tape = S.Tape(be, sc.N, 60, pg, precision=precision)
+        S.simulate_(st, sc.theta, pg, be, 60, tape=tape)
+        ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
+        S.backward_(ws, tape, sc.theta, pg, be, seedX=cX, seedV=cV)
+        outs.append((ws.gtheta, ws.gX, ws.gV))
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
