@@ -129,6 +129,30 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
 int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
                          double *seed);
 
+/* --- binomial checkpointing of the time loop.  No counterpart in the
+ * reference, whose Tape keeps every step (src/driver.jl:38-65) and whose README
+ * recommends recompute checkpointing for long horizons.  Snapshots are taken at
+ * neighbour-list epoch starts (every `interval` steps) at the positions of the
+ * binomial (revolve) schedule; each epoch is recomputed right before its
+ * reverse sweep, so gradients equal the full-tape ones bit for bit. --- */
+typedef struct sph_ckpt sph_ckpt;
+int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, int64_t nsteps,
+                    int64_t nsnap, double skin, int64_t interval, int64_t maxnb, sph_ckpt **out);
+int sph_ckpt_destroy(sph_ckpt *ck);
+/* simulate! over nsteps from X0, V0 ((2,N), id order); the final state is sph_ckpt_state's */
+int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta);
+/* the checkpointer's state (borrowed; its X/V are the final state after sph_ckpt_forward) */
+int sph_ckpt_state(sph_ckpt *ck, sph_state **st);
+/* backward! from seeds at the final state; fills ws like sph_backward */
+int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
+                      const double *seedX, const double *seedV);
+/* {epochs, snapshots, forward epochs of the last forward, forward epochs of the
+ *  last backward (recomputation included), epochs taped in the last backward} */
+int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5);
+/* the schedule alone, on the host: {forward epochs in total, taped epochs,
+ * most snapshots alive at once} for `epochs` epochs and `nsnap` snapshots */
+int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3);
+
 /* --- measurement: per-kernel GPU time with events on the context's stream.
  * No counterpart in the reference (its stage timing lives in
  * scripts/06_metal_profile.jl:158-290); used by bench.py for the roofline. */

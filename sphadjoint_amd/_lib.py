@@ -65,6 +65,13 @@ SIGNATURES = {
     "sph_workspace_get": (C.c_int, [_vp, C.c_int, _vp]),
     "sph_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _i64, _i64, C.c_int]),
     "sph_target_objective": (C.c_int, [_vp, _d, _d, _d, C.POINTER(_d), _vp]),
+    "sph_ckpt_create": (C.c_int, [_vp, _PP, C.c_int, _i64, _i64, _i64, _d, _i64, _i64, _pp]),
+    "sph_ckpt_destroy": (C.c_int, [_vp]),
+    "sph_ckpt_forward": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "sph_ckpt_state": (C.c_int, [_vp, _pp]),
+    "sph_ckpt_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "sph_ckpt_stats": (C.c_int, [_vp, _vp]),
+    "sph_ckpt_plan": (C.c_int, [_i64, _i64, _vp]),
     "sph_profile_enable": (C.c_int, [_vp, C.c_int]),
     "sph_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(_i64), C.POINTER(_d)]),
     "sph_profile_name": (C.c_char_p, [C.c_int]),

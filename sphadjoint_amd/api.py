@@ -228,11 +228,23 @@ class State:
         self.cl = CellList(self)
         self.nl = NeighborList(self)
 
+    @classmethod
+    def _borrow(cls, h, owner, backend, p, N, skin, precision):
+        # a state owned by another object (a Checkpointer): never destroyed here
+        st = cls.__new__(cls)
+        st.backend, st.p, st.skin, st.N = backend, p, float(skin), int(N)
+        st.rc = (2 * p.h) * (1 + st.skin)
+        st.precision = precision
+        st._h, st._owner = h, owner
+        st.cl = CellList(st)
+        st.nl = NeighborList(st)
+        return st
+
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owner", None) is None:
             L.load().sph_state_destroy(h)
-            self._h = None
+        self._h = None
 
     def _info(self):
         o = np.zeros(8, np.int64)
@@ -388,6 +400,78 @@ def backward_(ws: AdjointWorkspace, tape: Tape, theta, p: SPHParams, backend: MI
     L.call("sph_backward", ws._h, tape._h, f._h, _ptr(sx), _ptr(sv), float(skin), int(interval),
            int(maxnb), L.SPH_LAYOUT_SLOT if layout == "slot" else L.SPH_LAYOUT_PARTICLE)
     return ws
+
+
+class Checkpointer:
+    """Binomial (revolve) checkpointing of simulate!/backward! over ``nsteps``.
+
+    The reference's Tape keeps the state before every step (src/driver.jl:38-83);
+    this keeps at most ``snapshots`` states, taken at neighbour-list epoch starts
+    (every ``interval`` steps), at the positions of Griewank & Walther's binomial
+    schedule, and recomputes each epoch into a one-epoch tape right before its
+    reverse sweep.  Gradients are bit-identical to a full-tape ``backward_``;
+    recomputation costs about ``t`` extra forward passes for ``E`` epochs when
+    C(snapshots + t, t) >= E.
+
+    ``forward(X0, V0, theta)`` runs the whole horizon and returns the final
+    state (owned by the checkpointer; ``backward`` reuses its buffers, so read
+    the final state before calling ``backward``).
+    """
+
+    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, *, snapshots=64, skin=0.2,
+                 interval=4, maxnb=64, precision="f64"):
+        self.backend, self.p, self.N = backend, p, int(N)
+        self.nsteps, self.skin, self.interval = int(nsteps), float(skin), int(interval)
+        self.precision = _PREC[precision]
+        cp = p.c_struct()
+        h = C.c_void_p()
+        L.call("sph_ckpt_create", backend._h, C.byref(cp), self.precision, self.N, self.nsteps,
+               int(snapshots), self.skin, self.interval, int(maxnb), C.byref(h))
+        self._h = h
+        sh = C.c_void_p()
+        L.call("sph_ckpt_state", self._h, C.byref(sh))
+        self.state = State._borrow(sh, self, backend, p, self.N, self.skin, self.precision)
+        self._theta = None
+
+    def forward(self, X0, V0, theta):
+        x0 = _as_2n(X0, self.N)
+        v0 = _as_2n(V0, self.N)
+        self._theta = _field(theta, self.p, self.backend)
+        L.call("sph_ckpt_forward", self._h, _ptr(x0), _ptr(v0), self._theta._h)
+        check_neighbor_flags(self.state.nl)
+        return self.state
+
+    def backward(self, ws: AdjointWorkspace, *, seedX, seedV, theta=None):
+        f = self._theta if theta is None else _field(theta, self.p, self.backend)
+        if f is None:
+            raise RuntimeError("Checkpointer.backward needs a preceding forward")
+        sx = _as_2n(seedX, ws.N)
+        sv = _as_2n(seedV, ws.N)
+        L.call("sph_ckpt_backward", self._h, ws._h, f._h, _ptr(sx), _ptr(sv))
+        return ws
+
+    def stats(self):
+        o = np.zeros(5, np.int64)
+        L.call("sph_ckpt_stats", self._h, _ptr(o))
+        return dict(epochs=int(o[0]), snapshots=int(o[1]), forward_epochs=int(o[2]),
+                    backward_forward_epochs=int(o[3]), backward_taped_epochs=int(o[4]))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            st = getattr(self, "state", None)
+            if st is not None:
+                st._h = None
+            L.load().sph_ckpt_destroy(h)
+            self._h = None
+
+
+def ckpt_plan(epochs, snapshots):
+    """The binomial schedule on the host: forward epochs run in total (the
+    taped ones included), epochs taped, most snapshots alive at once."""
+    o = np.zeros(3, np.int64)
+    L.call("sph_ckpt_plan", int(epochs), int(snapshots), _ptr(o))
+    return dict(forward_epochs=int(o[0]), taped=int(o[1]), peak=int(o[2]))
 
 
 def target_objective(st: State, xc, yc, sigma):

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -319,6 +320,11 @@ struct sph_tape {
   int precision = SPH_F64;
   double2 *X = nullptr, *V = nullptr;  // (cap+1) entries of N; entry k (1-based) at (k-1)*N
   float2 *V32 = nullptr;                // f32c form: FP32 velocities instead of V
+  // optional (pterm, invrho) of every step, so that the reverse sweep skips
+  // the density recompute (used by the checkpointed driver's epoch tapes)
+  bool keep_pi = false;
+  double2 *PI = nullptr;
+  float2 *PI32 = nullptr;
   std::vector<int> entry_epoch;         // epoch record of each entry (cap+1)
   std::vector<TapeEpoch> epochs;        // records in use
   std::vector<TapeEpoch> pool;          // retained allocations
@@ -342,6 +348,8 @@ struct sph_workspace {
   double4 *tpart = nullptr;  // per theta cell partial sums of the gtheta gather
   int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
+  int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
+  int *iota = nullptr;     // particle-id order as a slot order
   // f32c form
   float2 *gX32 = nullptr, *gV32 = nullptr, *gXs32 = nullptr, *abar32 = nullptr, *PI32 = nullptr;
   float2 *Vtmp32 = nullptr, *t2f = nullptr, *xr32 = nullptr;
@@ -493,20 +501,27 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   st->age += 1;
   double2 *tX = nullptr, *tV = nullptr;
   float2 *tV32 = nullptr;
+  double2 *PI = st->PI;
+  float2 *PI32 = st->PI32;
   if (tp) {
     tape_epoch_for(tp, st);
     tp->n += 1;
     tp->entry_epoch[tp->n] = (int)tp->epochs.size() - 1;
-    tX = tp->X + (size_t)(tp->n - 1) * tp->N;
-    if (st->precision == SPH_F32C) tV32 = tp->V32 + (size_t)(tp->n - 1) * tp->N;
-    else tV = tp->V + (size_t)(tp->n - 1) * tp->N;
+    const size_t off = (size_t)(tp->n - 1) * tp->N;
+    tX = tp->X + off;
+    if (st->precision == SPH_F32C) tV32 = tp->V32 + off;
+    else tV = tp->V + off;
+    if (tp->keep_pi) {
+      if (st->precision == SPH_F32C) PI32 = tp->PI32 + off;
+      else PI = tp->PI + off;
+    }
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->nl, st->nbcnt, st->g, st->P32, st->PI32, nullptr);
+      launch_density_f32c(s, st->xr, st->nl, st->nbcnt, st->g, st->P32, PI32, nullptr);
     });
     timed(st->ctx, K_FORCE, [&] {
-      launch_force_f32c(s, st->X, st->xr, st->V32, st->PI32, theta, st->scell, st->nl, st->nbcnt,
+      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->nl, st->nbcnt,
                         st->g, st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
@@ -516,10 +531,10 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
     return;
   }
   timed(st->ctx, K_DENSITY, [&] {
-    launch_density_f64(s, st->X, st->nl, st->nbcnt, st->g, st->P, st->PI, nullptr, tX);
+    launch_density_f64(s, st->X, st->nl, st->nbcnt, st->g, st->P, PI, nullptr, tX);
   });
   timed(st->ctx, K_FORCE, [&] {
-    launch_force_f64(s, st->X, st->V, st->PI, theta, st->nl, st->nbcnt, st->g, st->P, st->Xn,
+    launch_force_f64(s, st->X, st->V, PI, theta, st->nl, st->nbcnt, st->g, st->P, st->Xn,
                      st->Vn, tV, nullptr, nullptr);
   });
   std::swap(st->X, st->Xn);
@@ -619,6 +634,8 @@ int sph_field_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, const double *host,
     f->d = dalloc<double>(ngy * ngx);
     if (host) HIPC(hipMemcpy(f->d, host, sizeof(double) * ngy * ngx, hipMemcpyHostToDevice));
     else HIPC(hipMemset(f->d, 0, sizeof(double) * ngy * ngx));
+    // the context's stream does not order itself after the null stream
+    HIPC(hipDeviceSynchronize());
     ctx_retain(ctx);
     *out = f;
   });
@@ -750,22 +767,22 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       // State ctor: X, V copied in; a, rho, pterm, invrho, alpha start at zero
       std::vector<int> iota(n);
       for (size_t i = 0; i < n; ++i) iota[i] = (int)i;
-      HIPC(hipMemcpy(st->ids, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-      HIPC(hipMemcpy(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice));
+      HIPC(hipMemcpyAsync(st->ids, iota.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
       if (f32c) {
-        HIPC(hipMemcpy(st->tmp2, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpyAsync(st->tmp2, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
         launch_d2_to_f2(s, st->tmp2, (int)N, st->V32);
-        HIPC(hipMemset(st->PI32, 0, sizeof(float2) * n));
+        HIPC(hipMemsetAsync(st->PI32, 0, sizeof(float2) * n, s));
       } else {
-        HIPC(hipMemcpy(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice));
-        HIPC(hipMemset(st->PI, 0, sizeof(double2) * n));
+        HIPC(hipMemcpyAsync(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+        HIPC(hipMemsetAsync(st->PI, 0, sizeof(double2) * n, s));
       }
-      HIPC(hipMemset(st->flags, 0, sizeof(int) * 2));
-      HIPC(hipMemset(st->a_id, 0, sizeof(double2) * n));
-      HIPC(hipMemset(st->pi_id, 0, sizeof(double2) * n));
-      HIPC(hipMemset(st->alpha_id, 0, sizeof(double) * n));
-      HIPC(hipMemset(st->rho_id, 0, sizeof(double) * n));
-      HIPC(hipMemset(st->counts, 0, sizeof(int) * nc));
+      HIPC(hipMemsetAsync(st->flags, 0, sizeof(int) * 2, s));
+      HIPC(hipMemsetAsync(st->a_id, 0, sizeof(double2) * n, s));
+      HIPC(hipMemsetAsync(st->pi_id, 0, sizeof(double2) * n, s));
+      HIPC(hipMemsetAsync(st->alpha_id, 0, sizeof(double) * n, s));
+      HIPC(hipMemsetAsync(st->rho_id, 0, sizeof(double) * n, s));
+      HIPC(hipMemsetAsync(st->counts, 0, sizeof(int) * nc, s));
       HIPC(hipStreamSynchronize(s));
     } catch (...) {
       state_free(st);
@@ -1014,6 +1031,8 @@ int sph_tape_destroy(sph_tape *tp) {
     dfree(tp->X);
     dfree(tp->V);
     dfree(tp->V32);
+    dfree(tp->PI);
+    dfree(tp->PI32);
     sph_ctx *c = tp->ctx;
     delete tp;
     ctx_release(c);
@@ -1081,7 +1100,7 @@ void ws_free(sph_workspace *ws) {
   for (auto pp : f2) dfree(*pp);
   dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
   dfree(ws->galpha32); dfree(ws->grho32); dfree(ws->tpart);
-  dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags);
+  dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags); dfree(ws->ids_cur); dfree(ws->iota);
 }
 
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
@@ -1108,6 +1127,13 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       ws->nbcnt = dalloc<int>(n);
       ws->flags = dalloc<int>(2);
       HIPC(hipMemset(ws->flags, 0, sizeof(int) * 2));
+      ws->ids_cur = dalloc<int>(n);
+      ws->iota = dalloc<int>(n);
+      {
+        std::vector<int> io(n);
+        for (size_t i = 0; i < n; ++i) io[i] = (int)i;
+        HIPC(hipMemcpy(ws->iota, io.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+      }
       if (precision == SPH_F32C) {
         ws->gX32 = dalloc<float2>(n);
         ws->gV32 = dalloc<float2>(n);
@@ -1132,6 +1158,7 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       HIPC(hipMemset(ws->gX_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(ws->gV_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(ws->gtheta, 0, sizeof(double) * p->ngx * p->ngy));
+      HIPC(hipDeviceSynchronize());
     } catch (...) {
       ws_free(ws);
       delete ws;
@@ -1180,119 +1207,189 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
 // rebuild schedule gives the same interaction set while displacements stay
 // within skin*h (src/neighbors.jl:133-144; src/driver.jl:162-167), and reusing
 // the forward's keeps the recomputed densities bit-identical to the forward's.
-// skin, interval, maxnb and layout are validated for interface parity.
+// skin, interval and layout are validated for interface parity; maxnb sizes
+// the rebuilt lists as in the reference.
 namespace {
 
-void backward_f64(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur, int maxnb) {
+void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
+  if (ws->nl_cap < maxnb) {
+    dfree(ws->nl);
+    ws->nl = dalloc<int>((size_t)ws->N * (size_t)maxnb);
+    ws->nl_cap = maxnb;
+  }
+}
+
+// Reverse sweep over every entry of `tape`.  The adjoint state (gX, gV) enters
+// in the slot order whose ids are `ids_in` and leaves in the slot order of the
+// tape's first entry, whose ids are copied to ws->ids_cur.  When `lnl` is
+// given it is the already built neighbour list of the tape's single epoch.
+void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int *ids_in,
+            int maxnb, const int *lnl, const int *lcnt) {
   hipStream_t s = ws->ctx->stream;
   sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
-  const PhysF64 P = make_phys(ws->p);
+  const bool f32c = ws->precision == SPH_F32C;
   const Grid g = tape->g;
+  const PhysF64 P = make_phys(ws->p);
+  const PhysF32 P32 = make_phys32(ws->p, tape->rc);
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
-  int listed = -1;  // epoch whose neighbour list ws->nl holds
+  int cur = tape->entry_epoch[tape->n];
+  // the adjoint state into the slot order of the final entry
+  timed(c, K_REMAP, [&] {
+    launch_inverse(s, ids_in, N, ws->inv);
+    if (f32c) {
+      launch_gather_remap2f(s, ws->gX32, ws->inv, tape->epochs[cur].ids, N, ws->t2f);
+      std::swap(ws->gX32, ws->t2f);
+      launch_gather_remap2f(s, ws->gV32, ws->inv, tape->epochs[cur].ids, N, ws->t2f);
+      std::swap(ws->gV32, ws->t2f);
+    } else {
+      launch_gather_remap2(s, ws->gX, ws->inv, tape->epochs[cur].ids, N, ws->t2);
+      std::swap(ws->gX, ws->t2);
+      launch_gather_remap2(s, ws->gV, ws->inv, tape->epochs[cur].ids, N, ws->t2);
+      std::swap(ws->gV, ws->t2);
+    }
+  });
+  const bool use_given = lnl != nullptr && tape->epochs.size() == 1;
+  const int *nl = use_given ? lnl : ws->nl;
+  const int *nbcnt = use_given ? lcnt : ws->nbcnt;
+  int listed = use_given ? 0 : -1;  // epoch whose neighbour list `nl` holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
+    const size_t off = (size_t)(k - 1) * n;
+    const double2 *Xk = tape->X + off;
     if (e != listed) {
       // the epoch's list, rebuilt from the positions of its first entry
       timed(c, K_NLBUILD, [&] {
-        launch_nl_build_f64(s, tape->X + (size_t)(E.first - 1) * n, E.scell, E.cstart, g, tape->kc,
-                            maxnb, ws->nl, ws->nbcnt, ws->flags);
+        const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
+        if (f32c) {
+          launch_xrel_f32c(s, X0, E.scell, g, P32, ws->xr32);
+          launch_nl_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P32,
+                               (float)(tape->rc * tape->rc), maxnb, ws->nl, ws->nbcnt, ws->flags);
+        } else {
+          launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
+                              ws->flags);
+        }
       });
       listed = e;
     }
-    const double2 *Xk = tape->X + (size_t)(k - 1) * n;
-    const double2 *Vk = tape->V + (size_t)(k - 1) * n;
-    const double2 *Vnext = tape->V + (size_t)k * n;
-    if (e != cur) {
-      // entry k+1 was stored in the next epoch's slot order: move the
-      // adjoint state and V_{k+1} into this epoch's order
-      timed(c, K_REMAP, [&] {
-        launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
-        launch_gather_remap2(s, ws->gX, ws->inv, E.ids, N, ws->t2);
-        std::swap(ws->gX, ws->t2);
-        launch_gather_remap2(s, ws->gV, ws->inv, E.ids, N, ws->t2);
-        std::swap(ws->gV, ws->t2);
-        launch_gather_remap2(s, Vnext, ws->inv, E.ids, N, ws->Vtmp);
+    if (f32c) {
+      const float2 *Vk = tape->V32 + off;
+      const float2 *Vnext = tape->V32 + (size_t)k * n;
+      if (e != cur) {
+        // entry k+1 was stored in the next epoch's slot order: move the
+        // adjoint state and V_{k+1} into this epoch's order
+        timed(c, K_REMAP, [&] {
+          launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
+          launch_gather_remap2f(s, ws->gX32, ws->inv, E.ids, N, ws->t2f);
+          std::swap(ws->gX32, ws->t2f);
+          launch_gather_remap2f(s, ws->gV32, ws->inv, E.ids, N, ws->t2f);
+          std::swap(ws->gV32, ws->t2f);
+          launch_gather_remap2f(s, Vnext, ws->inv, E.ids, N, ws->Vtmp32);
+        });
+        Vnext = ws->Vtmp32;
+        cur = e;
+      }
+      const float2 *PIk = ws->PI32;
+      timed(c, K_ADJ_R, [&] {
+        if (tape->keep_pi) {
+          PIk = tape->PI32 + off;
+          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, g, P32, ws->xr32, ws->gX32,
+                              ws->gV32, ws->abar32, ws->galpha32);
+        } else {
+          launch_xrel_f32c(s, Xk, E.scell, g, P32, ws->xr32);
+          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, nl, nbcnt, g, P32, ws->PI32,
+                            ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
+        }
       });
-      Vnext = ws->Vtmp;
-      cur = e;
+      timed(c, K_THETA, [&] {
+        launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
+      });
+      timed(c, K_ADJ_P1, [&] {
+        launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, nl, nbcnt,
+                           g, P32, ws->gXs32, ws->gV32, ws->grho32);
+      });
+      timed(c, K_ADJ_P2, [&] {
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, nl, nbcnt, g, P32, ws->gX32);
+      });
+    } else {
+      const double2 *Vk = tape->V + off;
+      const double2 *Vnext = tape->V + (size_t)k * n;
+      if (e != cur) {
+        timed(c, K_REMAP, [&] {
+          launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
+          launch_gather_remap2(s, ws->gX, ws->inv, E.ids, N, ws->t2);
+          std::swap(ws->gX, ws->t2);
+          launch_gather_remap2(s, ws->gV, ws->inv, E.ids, N, ws->t2);
+          std::swap(ws->gV, ws->t2);
+          launch_gather_remap2(s, Vnext, ws->inv, E.ids, N, ws->Vtmp);
+        });
+        Vnext = ws->Vtmp;
+        cur = e;
+      }
+      const double2 *PIk = ws->PI;
+      timed(c, K_ADJ_R, [&] {
+        if (tape->keep_pi) {
+          PIk = tape->PI + off;
+          launch_adj_rpw_f64(s, Xk, Vnext, theta->d, g, P, ws->gX, ws->gV, ws->abar, ws->galpha);
+        } else {
+          launch_adj_r_f64(s, Xk, Vnext, theta->d, nl, nbcnt, g, P, ws->PI, ws->gX, ws->gV,
+                           ws->abar, ws->galpha);
+        }
+      });
+      timed(c, K_THETA, [&] {
+        launch_theta_gather_f64(s, Xk, ws->galpha, E.cstart, g, T, ws->tpart, ws->gtheta);
+      });
+      timed(c, K_ADJ_P1, [&] {
+        launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, ws->gXs,
+                          ws->gV, ws->grho);
+      });
+      timed(c, K_ADJ_P2, [&] {
+        launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, nl, nbcnt, g, P, ws->gX);
+      });
     }
-    timed(c, K_ADJ_R, [&] {
-      launch_adj_r_f64(s, Xk, Vnext, theta->d, ws->nl, ws->nbcnt, g, P, ws->PI, ws->gX, ws->gV,
-                       ws->abar, ws->galpha);
-    });
-    timed(c, K_THETA, [&] {
-      launch_theta_gather_f64(s, Xk, ws->galpha, E.cstart, g, T, ws->tpart, ws->gtheta);
-    });
-    timed(c, K_ADJ_P1, [&] {
-      launch_adj_p1_f64(s, Xk, Vk, ws->PI, ws->abar, ws->galpha, theta->d, ws->nl, ws->nbcnt, g, P,
-                        ws->gXs, ws->gV, ws->grho);
-    });
-    timed(c, K_ADJ_P2, [&] {
-      launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, ws->nl, ws->nbcnt, g, P, ws->gX);
-    });
   }
-  launch_scatter2(s, ws->gX, tape->epochs[cur].ids, N, ws->gX_id);
-  launch_scatter2(s, ws->gV, tape->epochs[cur].ids, N, ws->gV_id);
+  HIPC(hipMemcpyAsync(ws->ids_cur, tape->epochs[cur].ids, sizeof(int) * n,
+                      hipMemcpyDeviceToDevice, s));
 }
 
-void backward_f32c(sph_workspace *ws, sph_tape *tape, const sph_field *theta, int cur,
-                   double rc, int maxnb) {
+// seeds (host, particle-id order) into the adjoint state, in id order
+void bw_seed(sph_workspace *ws, const double *seedX, const double *seedV) {
   hipStream_t s = ws->ctx->stream;
-  sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
-  const PhysF32 P = make_phys32(ws->p, rc);
-  const Grid g = tape->g;
-  const ThetaGrid T = make_theta_grid(ws->p, rc);
-  int listed = -1;  // epoch whose neighbour list ws->nl holds
-  for (int64_t k = tape->n; k >= 1; --k) {
-    const int e = tape->entry_epoch[k];
-    const TapeEpoch &E = tape->epochs[e];
-    if (e != listed) {
-      timed(c, K_NLBUILD, [&] {
-        launch_xrel_f32c(s, tape->X + (size_t)(E.first - 1) * n, E.scell, g, P, ws->xr32);
-        launch_nl_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P, (float)(rc * rc), maxnb, ws->nl,
-                             ws->nbcnt, ws->flags);
-      });
-      listed = e;
-    }
-    const double2 *Xk = tape->X + (size_t)(k - 1) * n;
-    const float2 *Vk = tape->V32 + (size_t)(k - 1) * n;
-    const float2 *Vnext = tape->V32 + (size_t)k * n;
-    if (e != cur) {
-      timed(c, K_REMAP, [&] {
-        launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
-        launch_gather_remap2f(s, ws->gX32, ws->inv, E.ids, N, ws->t2f);
-        std::swap(ws->gX32, ws->t2f);
-        launch_gather_remap2f(s, ws->gV32, ws->inv, E.ids, N, ws->t2f);
-        std::swap(ws->gV32, ws->t2f);
-        launch_gather_remap2f(s, Vnext, ws->inv, E.ids, N, ws->Vtmp32);
-      });
-      Vnext = ws->Vtmp32;
-      cur = e;
-    }
-    timed(c, K_ADJ_R, [&] {
-      launch_xrel_f32c(s, Xk, E.scell, g, P, ws->xr32);
-      launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, ws->nl, ws->nbcnt, g, P, ws->PI32,
-                        ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
-    });
-    timed(c, K_THETA, [&] {
-      launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
-    });
-    timed(c, K_ADJ_P1, [&] {
-      launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, ws->PI32, ws->abar32, ws->galpha32, theta->d, ws->nl,
-                         ws->nbcnt, g, P, ws->gXs32, ws->gV32, ws->grho32);
-    });
-    timed(c, K_ADJ_P2, [&] {
-      launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, ws->nl, ws->nbcnt, g, P, ws->gX32);
-    });
+  HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
+  if (ws->precision == SPH_F32C) {
+    HIPC(hipMemcpyAsync(ws->t2, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    launch_d2_to_f2(s, ws->t2, N, ws->gX32);
+    HIPC(hipStreamSynchronize(s));
+    HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    launch_d2_to_f2(s, ws->t2, N, ws->gV32);
+  } else {
+    HIPC(hipMemcpyAsync(ws->gX, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(ws->gV, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
   }
-  launch_scatter2f(s, ws->gX32, tape->epochs[cur].ids, N, ws->gX_id);
-  launch_scatter2f(s, ws->gV32, tape->epochs[cur].ids, N, ws->gV_id);
+  HIPC(hipMemcpyAsync(ws->ids_cur, ws->iota, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+}
+
+// the adjoint state, whose slot order is ws->ids_cur, into id order (gX_id, gV_id)
+void bw_finish(sph_workspace *ws) {
+  hipStream_t s = ws->ctx->stream;
+  const int N = (int)ws->N;
+  if (ws->precision == SPH_F32C) {
+    launch_scatter2f(s, ws->gX32, ws->ids_cur, N, ws->gX_id);
+    launch_scatter2f(s, ws->gV32, ws->ids_cur, N, ws->gV_id);
+  } else {
+    launch_scatter2(s, ws->gX, ws->ids_cur, N, ws->gX_id);
+    launch_scatter2(s, ws->gV, ws->ids_cur, N, ws->gV_id);
+  }
+}
+
+void check_bw_args(sph_workspace *ws, const sph_field *theta, const double *seedX,
+                   const double *seedV) {
+  if (!ws || !theta || !seedX || !seedV) fail(SPH_E_ARG, "NULL argument");
+  if (theta->ngy != ws->p.ngy || theta->ngx != ws->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
 }
 
 }  // namespace
@@ -1300,40 +1397,17 @@ void backward_f32c(sph_workspace *ws, sph_tape *tape, const sph_field *theta, in
 int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
                  const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout) {
   return guarded([&] {
-    if (!ws || !tape || !theta || !seedX || !seedV) fail(SPH_E_ARG, "NULL argument");
+    check_bw_args(ws, theta, seedX, seedV);
+    if (!tape) fail(SPH_E_ARG, "NULL argument");
     if (tape->N != ws->N) fail(SPH_E_ARG, "tape and workspace N differ");
     if (tape->precision != ws->precision) fail(SPH_E_ARG, "tape and workspace precision differ");
-    if (theta->ngy != ws->p.ngy || theta->ngx != ws->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
     if (!(skin >= 0) || interval < 1 || maxnb < 1) fail(SPH_E_ARG, "bad neighbour-list settings");
     if (layout != SPH_LAYOUT_SLOT && layout != SPH_LAYOUT_PARTICLE) fail(SPH_E_ARG, "bad layout");
     hipStream_t s = ws->ctx->stream;
-    const int N = (int)ws->N;
-    const size_t n = (size_t)N;
-    HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
-    if (tape->n == 0) {
-      HIPC(hipMemcpyAsync(ws->gX_id, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-      HIPC(hipMemcpyAsync(ws->gV_id, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-      HIPC(hipStreamSynchronize(s));
-      return;
-    }
-    const int cur = tape->entry_epoch[tape->n];
-    const int *ids = tape->epochs[cur].ids;
-    // seeds (id order) into the slot order of the final entry
-    const bool f32c = ws->precision == SPH_F32C;
-    HIPC(hipMemcpyAsync(ws->t2, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    if (f32c) launch_gather2f(s, ws->t2, ids, N, ws->gX32);
-    else launch_gather2(s, ws->t2, ids, N, ws->gX);
-    HIPC(hipStreamSynchronize(s));
-    HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    if (f32c) launch_gather2f(s, ws->t2, ids, N, ws->gV32);
-    else launch_gather2(s, ws->t2, ids, N, ws->gV);
-    if (ws->nl_cap < maxnb) {
-      dfree(ws->nl);
-      ws->nl = dalloc<int>(n * (size_t)maxnb);
-      ws->nl_cap = maxnb;
-    }
-    if (f32c) backward_f32c(ws, tape, theta, cur, tape->rc, (int)maxnb);
-    else backward_f64(ws, tape, theta, cur, (int)maxnb);
+    ensure_ws_list(ws, maxnb);
+    bw_seed(ws, seedX, seedV);
+    if (tape->n > 0) bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr);
+    bw_finish(ws);
     HIPC(hipStreamSynchronize(s));
     prof_collect(ws->ctx);
   });
@@ -1353,6 +1427,451 @@ int sph_target_objective(sph_state *st, double xc, double yc, double sigma, doub
       HIPC(hipMemcpyAsync(seed, st->tmp2, sizeof(double2) * N, hipMemcpyDeviceToHost, s));
     }
     HIPC(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Binomial checkpointing of the time loop.  The reference keeps every step on
+// its Tape (src/driver.jl:38-65); here only snapshots at neighbour-list epoch
+// starts are kept, at positions chosen by the binomial (revolve) schedule of
+// Griewank & Walther, and each epoch is recomputed into a one-epoch tape right
+// before its reverse sweep.  Snapshots sit at rebuild steps, where the rebuild
+// is a pure function of the positions and ids, so a restored run repeats the
+// original forward bit for bit and the gradients equal the full-tape ones.
+
+namespace {
+
+// C(c + t, t), saturating
+int64_t beta(int64_t c, int64_t t) {
+  long double v = 1.0L;
+  for (int64_t k = 1; k <= t; ++k) {
+    v = v * (long double)(c + k) / (long double)k;
+    if (v > 4.0e18L) return (int64_t)4e18;
+  }
+  return (int64_t)(v + 0.5L);
+}
+
+// Fewest epoch advances that reverse l epochs whose first state is held, with
+// c more snapshots free (s = c + 1 in Griewank's count, which includes the
+// held one): with t the least repetition number such that beta(s, t) >= l,
+// the minimum is t*l - C(s+t, t-1).
+int64_t revolve_cost(int64_t l, int64_t c) {
+  if (l <= 1) return 0;
+  const int64_t s = c + 1;
+  int64_t t = 0;
+  while (beta(s, t) < l) ++t;
+  // C(s+t, t-1) = beta(s+1, t-1)
+  return t * l - beta(s + 1, t - 1);
+}
+
+// First advance j of an optimal schedule: the largest j with
+// j + cost(l-j, c-1) + cost(j, c) equal to cost(l, c).
+int64_t revolve_split(int64_t l, int64_t c) {
+  const int64_t best = revolve_cost(l, c);
+  for (int64_t j = l - 1; j >= 1; --j)
+    if (j + revolve_cost(l - j, c - 1) + revolve_cost(j, c) == best) return j;
+  return 1;
+}
+
+// The schedule as a sequence of operations on a runner.  With `dry` set only
+// the counters move (used by sph_ckpt_plan and the CPU tests).
+struct Schedule {
+  int64_t E = 0, nsnap = 0;
+  bool dry = true;
+  int64_t advanced = 0, taped = 0, live = 0, peak = 0;
+  std::vector<std::pair<int64_t, int64_t>> chain;  // (a, free slots) of the forward's levels
+  // hooks (unused when dry)
+  std::function<void(int64_t)> restore, store, drop, tape_epoch, adjoint;
+  std::function<void(int64_t, int64_t)> advance;
+
+  void op_restore(int64_t a) { if (!dry) restore(a); }
+  void op_advance(int64_t a, int64_t b) {
+    advanced += b - a;
+    if (!dry && b > a) advance(a, b);
+  }
+  void op_store(int64_t a) {
+    live += 1;
+    if (live > peak) peak = live;
+    if (!dry) store(a);
+  }
+  void op_drop(int64_t a) {
+    live -= 1;
+    if (!dry) drop(a);
+  }
+  void op_taped(int64_t e) {
+    taped += 1;
+    if (!dry) tape_epoch(e);
+  }
+  void op_adjoint(int64_t e) { if (!dry) adjoint(e); }
+
+  // forward: the leftmost descent of the recursion, ending with the last
+  // epoch taped (its tape serves the first reverse step)
+  void forward() {
+    chain.clear();
+    int64_t a = 0, c = nsnap;
+    for (;;) {
+      chain.emplace_back(a, c);
+      const int64_t l = E - a;
+      if (l == 1) {
+        op_taped(a);
+        return;
+      }
+      if (c == 0) {
+        op_advance(a, E - 1);
+        op_taped(E - 1);
+        return;
+      }
+      const int64_t j = revolve_split(l, c);
+      op_advance(a, a + j);
+      op_store(a + j);
+      a += j;
+      c -= 1;
+    }
+  }
+
+  void reverse(int64_t a, int64_t b, int64_t c) {
+    const int64_t l = b - a;
+    if (l == 1) {
+      op_restore(a);
+      op_taped(a);
+      op_adjoint(a);
+      return;
+    }
+    if (c == 0) {
+      for (int64_t e = b - 1; e >= a; --e) {
+        op_restore(a);
+        op_advance(a, e);
+        op_taped(e);
+        op_adjoint(e);
+      }
+      return;
+    }
+    const int64_t j = revolve_split(l, c);
+    op_restore(a);
+    op_advance(a, a + j);
+    op_store(a + j);
+    reverse(a + j, b, c - 1);
+    op_drop(a + j);
+    reverse(a, a + j, c);
+  }
+
+  void backward() {
+    for (int64_t k = (int64_t)chain.size() - 1; k >= 0; --k) {
+      const int64_t a = chain[k].first, c = chain[k].second;
+      if (k == (int64_t)chain.size() - 1) {
+        // the deepest level: its last epoch is on the tape already
+        op_adjoint(E - 1);
+        for (int64_t e = E - 2; e >= a; --e) {
+          op_restore(a);
+          op_advance(a, e);
+          op_taped(e);
+          op_adjoint(e);
+        }
+      } else {
+        op_drop(chain[k + 1].first);
+        reverse(a, chain[k + 1].first, c);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+struct sph_ckpt {
+  sph_ctx *ctx = nullptr;
+  sph_params p{};
+  int precision = SPH_F64;
+  int64_t N = 0, nsteps = 0, nsnap = 0, interval = 4, maxnb = 64, E = 0;
+  double skin = 0.2;
+  sph_state *work = nullptr;
+  sph_tape *mini = nullptr;
+  struct Snap {
+    double2 *X = nullptr, *V = nullptr;
+    float2 *V32 = nullptr;
+    int *ids = nullptr;
+  };
+  std::vector<Snap> snaps;              // slot 0: the initial state; 1..nsnap: schedule
+  std::vector<int> slot_of;             // epoch -> slot (-1: none)
+  std::vector<int> free_slots;
+  int64_t at = -1;                      // the work state sits at the start of this epoch
+  int64_t on_tape = -1;                 // epoch held by the one-epoch tape
+  bool forward_done = false;
+  Schedule sched;
+  const sph_field *theta = nullptr;
+  sph_workspace *ws = nullptr;
+  int64_t fwd_advanced = 0, fwd_taped = 0;
+};
+
+namespace {
+
+int64_t ck_steps(const sph_ckpt *ck, int64_t e) {
+  int64_t rest = ck->nsteps - e * ck->interval;
+  return rest < ck->interval ? rest : ck->interval;
+}
+
+void ck_copy(sph_ckpt *ck, bool to_snap, int slot) {
+  hipStream_t s = ck->ctx->stream;
+  sph_state *st = ck->work;
+  sph_ckpt::Snap &q = ck->snaps[slot];
+  const size_t n = (size_t)ck->N;
+  auto cp = [&](void *dst, const void *src, size_t bytes) {
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+  };
+  if (to_snap) {
+    cp(q.X, st->X, sizeof(double2) * n);
+    if (ck->precision == SPH_F32C) cp(q.V32, st->V32, sizeof(float2) * n);
+    else cp(q.V, st->V, sizeof(double2) * n);
+    cp(q.ids, st->ids, sizeof(int) * n);
+  } else {
+    cp(st->X, q.X, sizeof(double2) * n);
+    if (ck->precision == SPH_F32C) cp(st->V32, q.V32, sizeof(float2) * n);
+    else cp(st->V, q.V, sizeof(double2) * n);
+    cp(st->ids, q.ids, sizeof(int) * n);
+    // the next step rebuilds the cell list and neighbour list from these
+    // positions, exactly as the original run did at this epoch start
+    st->age = kAgeNever;
+    st->aux_state = 0;
+    st->epoch += 1;
+  }
+}
+
+void ck_bind(sph_ckpt *ck) {
+  Schedule &S = ck->sched;
+  S.dry = false;
+  S.restore = [ck](int64_t a) {
+    if (ck->at == a) return;
+    int slot = ck->slot_of[a];
+    if (slot < 0) fail(SPH_E_STATE, "checkpoint schedule lost the snapshot of epoch %lld", (long long)a);
+    ck_copy(ck, false, slot);
+    ck->at = a;
+  };
+  S.store = [ck](int64_t a) {
+    if (ck->at != a) fail(SPH_E_STATE, "snapshot of epoch %lld requested away from it", (long long)a);
+    if (ck->free_slots.empty()) fail(SPH_E_STATE, "checkpoint schedule ran out of snapshots");
+    int slot = ck->free_slots.back();
+    ck->free_slots.pop_back();
+    ck_copy(ck, true, slot);
+    ck->slot_of[a] = slot;
+  };
+  S.drop = [ck](int64_t a) {
+    int slot = ck->slot_of[a];
+    if (slot > 0) ck->free_slots.push_back(slot);
+    ck->slot_of[a] = -1;
+  };
+  S.advance = [ck](int64_t a, int64_t b) {
+    if (ck->at != a) fail(SPH_E_STATE, "advance from epoch %lld, state at %lld", (long long)a,
+                          (long long)ck->at);
+    for (int64_t e = a; e < b; ++e)
+      for (int64_t k = 0; k < ck_steps(ck, e); ++k) do_step(ck->work, ck->theta->d, nullptr);
+    ck->at = b;
+  };
+  S.tape_epoch = [ck](int64_t e) {
+    if (ck->at != e) fail(SPH_E_STATE, "taped epoch %lld, state at %lld", (long long)e,
+                          (long long)ck->at);
+    sph_tape *tp = ck->mini;
+    for (auto &r : tp->epochs) tp->pool.push_back(r);
+    tp->epochs.clear();
+    tp->n = 0;
+    const int64_t ns = ck_steps(ck, e);
+    for (int64_t k = 0; k < ns; ++k) do_step(ck->work, ck->theta->d, tp);
+    hipStream_t s = ck->ctx->stream;
+    const size_t off = (size_t)tp->n * tp->N;
+    HIPC(hipMemcpyAsync(tp->X + off, ck->work->X, sizeof(double2) * tp->N,
+                        hipMemcpyDeviceToDevice, s));
+    if (ck->precision == SPH_F32C)
+      HIPC(hipMemcpyAsync(tp->V32 + off, ck->work->V32, sizeof(float2) * tp->N,
+                          hipMemcpyDeviceToDevice, s));
+    else
+      HIPC(hipMemcpyAsync(tp->V + off, ck->work->V, sizeof(double2) * tp->N,
+                          hipMemcpyDeviceToDevice, s));
+    tp->entry_epoch[tp->n + 1] = tp->entry_epoch[tp->n];
+    ck->at = e + 1;
+    ck->on_tape = e;
+  };
+  S.adjoint = [ck](int64_t e) {
+    if (ck->on_tape != e) fail(SPH_E_STATE, "adjoint of epoch %lld without its tape", (long long)e);
+    // the work state has not stepped since the epoch was taped: its list is
+    // the epoch's list
+    bw_run(ck->ws, ck->mini, ck->theta, ck->ws->ids_cur, (int)ck->maxnb, ck->work->nl,
+           ck->work->nbcnt);
+  };
+}
+
+void ck_free(sph_ckpt *ck) {
+  for (auto &q : ck->snaps) {
+    dfree(q.X);
+    dfree(q.V);
+    dfree(q.V32);
+    dfree(q.ids);
+  }
+  if (ck->work) sph_state_destroy(ck->work);
+  if (ck->mini) sph_tape_destroy(ck->mini);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3) {
+  return guarded([&] {
+    if (epochs < 1 || nsnap < 0 || !out3) fail(SPH_E_ARG, "bad plan arguments");
+    Schedule S;
+    S.E = epochs;
+    S.nsnap = nsnap;
+    S.forward();
+    S.backward();
+    out3[0] = S.advanced + S.taped;   // forward epochs run in total
+    out3[1] = S.taped;                // epochs run with a tape
+    out3[2] = S.peak;                 // most snapshots alive at once (excluding the initial)
+  });
+}
+
+int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, int64_t nsteps,
+                    int64_t nsnap, double skin, int64_t interval, int64_t maxnb, sph_ckpt **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    check_params(p);
+    need_precision(precision);
+    if (nsteps < 1) fail(SPH_E_ARG, "nsteps must be >= 1");
+    if (nsnap < 0) fail(SPH_E_ARG, "nsnap must be >= 0");
+    if (interval < 1) fail(SPH_E_ARG, "interval must be >= 1");
+    auto *ck = new sph_ckpt;
+    try {
+      ck->ctx = ctx;
+      ck->p = *p;
+      ck->precision = precision;
+      ck->N = N;
+      ck->nsteps = nsteps;
+      ck->interval = interval;
+      ck->maxnb = maxnb;
+      ck->skin = skin;
+      ck->E = (nsteps + interval - 1) / interval;
+      ck->nsnap = nsnap < ck->E ? nsnap : ck->E;
+      std::vector<double> zeros((size_t)N * 2, 0.0);
+      int rc = sph_state_create(ctx, p, precision, N, zeros.data(), zeros.data(), skin, interval,
+                                maxnb, SPH_LAYOUT_SLOT, &ck->work);
+      if (rc != SPH_OK) throw Error(rc, g_err);
+      rc = sph_tape_create(ctx, N, interval, precision, &ck->mini);
+      if (rc != SPH_OK) throw Error(rc, g_err);
+      ck->mini->keep_pi = true;
+      const size_t tot = (size_t)(interval + 1) * (size_t)N;
+      if (precision == SPH_F32C) ck->mini->PI32 = dalloc<float2>(tot);
+      else ck->mini->PI = dalloc<double2>(tot);
+      ck->snaps.resize((size_t)ck->nsnap + 1);
+      for (auto &q : ck->snaps) {
+        q.X = dalloc<double2>((size_t)N);
+        if (precision == SPH_F32C) q.V32 = dalloc<float2>((size_t)N);
+        else q.V = dalloc<double2>((size_t)N);
+        q.ids = dalloc<int>((size_t)N);
+      }
+      ck->sched.E = ck->E;
+      ck->sched.nsnap = ck->nsnap;
+      ck_bind(ck);
+    } catch (...) {
+      ck_free(ck);
+      delete ck;
+      throw;
+    }
+    ctx_retain(ctx);
+    *out = ck;
+  });
+}
+
+int sph_ckpt_destroy(sph_ckpt *ck) {
+  return guarded([&] {
+    if (!ck) return;
+    (void)hipStreamSynchronize(ck->ctx->stream);
+    ck_free(ck);
+    sph_ctx *c = ck->ctx;
+    delete ck;
+    ctx_release(c);
+  });
+}
+
+// forward over nsteps from (X0, V0) (host, id order); snapshots per the schedule
+int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta) {
+  return guarded([&] {
+    if (!ck || !X0 || !V0 || !theta) fail(SPH_E_ARG, "NULL argument");
+    if (theta->ngy != ck->p.ngy || theta->ngx != ck->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
+    sph_state *st = ck->work;
+    hipStream_t s = ck->ctx->stream;
+    const size_t n = (size_t)ck->N;
+    // reset the work state to (X0, V0) in particle-id order
+    std::vector<int> io(n);
+    for (size_t i = 0; i < n; ++i) io[i] = (int)i;
+    HIPC(hipMemcpyAsync(st->ids, io.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    if (ck->precision == SPH_F32C) {
+      HIPC(hipMemcpyAsync(st->tmp2, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+      launch_d2_to_f2(s, st->tmp2, (int)n, st->V32);
+    } else {
+      HIPC(hipMemcpyAsync(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+    st->age = kAgeNever;
+    st->aux_state = 0;
+    st->epoch += 1;
+    copy_theta(st, theta);
+    ck->theta = theta;
+    ck->slot_of.assign((size_t)ck->E + 1, -1);
+    ck->free_slots.clear();
+    for (int k = (int)ck->nsnap; k >= 1; --k) ck->free_slots.push_back(k);
+    ck->at = 0;
+    ck->on_tape = -1;
+    ck_copy(ck, true, 0);
+    ck->slot_of[0] = 0;
+    ck->sched.advanced = ck->sched.taped = ck->sched.live = ck->sched.peak = 0;
+    ck->sched.forward();
+    ck->fwd_advanced = ck->sched.advanced;
+    ck->fwd_taped = ck->sched.taped;
+    ck->forward_done = true;
+    HIPC(hipStreamSynchronize(s));
+    prof_collect(ck->ctx);
+  });
+}
+
+int sph_ckpt_state(sph_ckpt *ck, sph_state **st) {
+  return guarded([&] {
+    if (!ck || !st) fail(SPH_E_ARG, "NULL argument");
+    *st = ck->work;
+  });
+}
+
+int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta, const double *seedX,
+                      const double *seedV) {
+  return guarded([&] {
+    if (!ck) fail(SPH_E_ARG, "NULL argument");
+    check_bw_args(ws, theta, seedX, seedV);
+    if (!ck->forward_done) fail(SPH_E_STATE, "sph_ckpt_backward needs a preceding sph_ckpt_forward");
+    if (ws->N != ck->N || ws->precision != ck->precision)
+      fail(SPH_E_ARG, "workspace does not match the checkpointer");
+    if (theta != ck->theta) copy_theta(ck->work, theta);
+    ck->theta = theta;
+    ck->ws = ws;
+    ensure_ws_list(ws, ck->maxnb);
+    bw_seed(ws, seedX, seedV);
+    ck->sched.advanced = ck->sched.taped = 0;
+    ck->sched.live = (int64_t)ck->sched.chain.size() - 1;
+    ck->sched.backward();
+    bw_finish(ws);
+    ck->forward_done = false;
+    HIPC(hipStreamSynchronize(ws->ctx->stream));
+    prof_collect(ws->ctx);
+  });
+}
+
+// {epochs, snapshots, forward epochs of the last forward, forward epochs of the
+//  last backward (recomputation), epochs taped in the last backward}
+int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5) {
+  return guarded([&] {
+    if (!ck || !out5) fail(SPH_E_ARG, "NULL argument");
+    out5[0] = ck->E;
+    out5[1] = ck->nsnap;
+    out5[2] = ck->fwd_advanced + ck->fwd_taped;
+    out5[3] = ck->sched.advanced + ck->sched.taped;
+    out5[4] = ck->sched.taped;
   });
 }
 

@@ -32,6 +32,9 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
                        float2 *PI, float2 *gX, float2 *gV, float2 *abar, float *galpha);
+void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
+                         const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
+                         float2 *gV, float2 *abar, float *galpha);
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
                         const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,

@@ -311,6 +311,29 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   gV[s] = make_float2(D * gvx, D * gvy);
 }
 
+// adjoint R without the density recompute (tapes that kept each step's
+// pterm/invrho); also forms this step's relative coordinates xr
+__global__ void __launch_bounds__(kBlock)
+k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
+            const float2 *__restrict__ Vnext, const double *__restrict__ theta, Grid g, PhysF32 P,
+            float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
+            float2 *__restrict__ abar, float *__restrict__ galpha) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  const double2 x64 = X[s];
+  xr[s] = rel_coord(x64, bcell[s], g, P);
+  InterpF ic = interp_cell(x64.x, x64.y, P);
+  float al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
+  float gvx = gv.x + P.dt * gx.x;
+  float gvy = gv.y + P.dt * gx.y;
+  float D = 1.f / (1.f + P.dt * al);
+  float dtD = P.dt * D;
+  abar[s] = make_float2(dtD * gvx, dtD * gvy);
+  galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
+  gV[s] = make_float2(D * gvx, D * gvy);
+}
+
 // adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185)
 __global__ void __launch_bounds__(kBlock)
 k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
@@ -440,6 +463,13 @@ void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
   if (g.N > 0)
     k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV,
                                               abar, galpha);
+}
+void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
+                         const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
+                         float2 *gV, float2 *abar, float *galpha) {
+  if (g.N > 0)
+    k_adj_rpw32<<<nblocks(g.N), kBlock, 0, s>>>(X, bcell, Vnext, theta, g, P, xr, gX, gV, abar,
+                                                galpha);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,

@@ -222,6 +222,26 @@ k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
   gV[s] = make_double2(D * gvx, D * gvy);
 }
 
+// Adjoint pass R without the density recompute, for tapes that kept each
+// step's (pterm, invrho): the same _adj_integrate! and _adj_drag! arithmetic.
+__global__ void __launch_bounds__(kBlock)
+k_adj_rpw(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
+          const double *__restrict__ theta, Grid g, PhysF64 P, double2 *__restrict__ gX,
+          double2 *__restrict__ gV, double2 *__restrict__ abar, double *__restrict__ galpha) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  double2 xi = X[s];
+  Interp ic = interp_cell(xi.x, xi.y, P);
+  double al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  double2 gx = gX[s], gv = gV[s], vp = Vnext[s];
+  double gvx = gv.x + P.dt * gx.x;
+  double gvy = gv.y + P.dt * gx.y;
+  double D = 1.0 / (1.0 + P.dt * al);
+  abar[s] = make_double2((P.dt * D) * gvx, (P.dt * D) * gvy);
+  galpha[s] = ((-P.dt) * D) * (vp.x * gvx + vp.y * gvy);
+  gV[s] = make_double2(D * gvx, D * gvy);
+}
+
 // Adjoint pass P1: adj_pass1_kernel! (src/adjoint.jl:30-122) followed by the
 // design term of adj_design_kernel! (src/adjoint.jl:166-185) and gV += gVs.
 __global__ void __launch_bounds__(kBlock)
@@ -386,6 +406,12 @@ void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, con
   if (g.N > 0)
     k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV, abar,
                                             galpha);
+}
+void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
+                        const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
+                        double2 *abar, double *galpha) {
+  if (g.N > 0)
+    k_adj_rpw<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, g, P, gX, gV, abar, galpha);
 }
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,

@@ -155,6 +155,9 @@ void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const d
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
                       double2 *gX, double2 *gV, double2 *abar, double *galpha);
+void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
+                        const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
+                        double2 *abar, double *galpha);
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
                        const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
