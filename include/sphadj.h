@@ -75,6 +75,16 @@ int sph_field_set(sph_field *f, const double *host);
 int sph_field_get(const sph_field *f, double *host);
 int sph_field_destroy(sph_field *f);
 
+/* Device memory owned by the library, for the *_dev entry points' inputs and
+ * outputs (the counterpart of a KernelAbstractions.allocate'd array).  No
+ * counterpart entry point in the reference.  host == NULL zero-fills. */
+typedef struct sph_buffer sph_buffer;
+int sph_buffer_create(sph_ctx *ctx, int64_t nbytes, const void *host, sph_buffer **out);
+int sph_buffer_destroy(sph_buffer *b);
+int sph_buffer_ptr(const sph_buffer *b, void **ptr);   /* the device address */
+int sph_buffer_read(const sph_buffer *b, void *host, int64_t nbytes);
+int sph_buffer_write(sph_buffer *b, const void *host, int64_t nbytes);
+
 /* --- State (src/driver.jl:5-36) with its CellList and NeighborList (src/neighbors.jl:28-36,174-190) --- */
 int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
                      const double *X0, const double *V0, double skin, int64_t interval,
@@ -129,28 +139,49 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
 int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
                          double *seed);
 
+/* Device-memory variants of the calls above: seedX, seedV and seed are device
+ * pointers ((2,N) interleaved doubles, id order) on the context's device, e.g.
+ * from sph_buffer_ptr.  The reference's backward! accepts device seeds the same
+ * way (copyto!, src/driver.jl:144-146); callers order their own writes of
+ * these buffers before the call.  J stays a host value. */
+int sph_backward_dev(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
+                     const double *seedX, const double *seedV, double skin, int64_t interval,
+                     int64_t maxnb, int layout);
+int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
+                             double *seed);
+
 /* --- binomial checkpointing of the time loop.  No counterpart in the
  * reference, whose Tape keeps every step (src/driver.jl:38-65) and whose README
- * recommends recompute checkpointing for long horizons.  Snapshots are taken at
- * neighbour-list epoch starts (every `interval` steps) at the positions of the
- * binomial (revolve) schedule; each epoch is recomputed right before its
- * reverse sweep, so gradients equal the full-tape ones bit for bit. --- */
+ * recommends recompute checkpointing for long horizons.  The horizon is cut
+ * into segments of `seg_epochs` neighbour-list epochs (seg_epochs*interval
+ * steps); at most `nsnap` segment-start states are kept, at the positions of
+ * the binomial (revolve) schedule, and each segment is recomputed onto a
+ * one-segment tape right before its reverse sweep.  Segment starts are
+ * neighbour-list rebuilds, so gradients equal the full-tape ones bit for bit.
+ * Device memory: nsnap+1 states plus a tape of seg_epochs*interval steps. --- */
 typedef struct sph_ckpt sph_ckpt;
 int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, int64_t nsteps,
-                    int64_t nsnap, double skin, int64_t interval, int64_t maxnb, sph_ckpt **out);
+                    int64_t nsnap, int64_t seg_epochs, double skin, int64_t interval,
+                    int64_t maxnb, sph_ckpt **out);
 int sph_ckpt_destroy(sph_ckpt *ck);
 /* simulate! over nsteps from X0, V0 ((2,N), id order); the final state is sph_ckpt_state's */
 int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta);
+/* the same with X0, V0 in device memory */
+int sph_ckpt_forward_dev(sph_ckpt *ck, const double *X0, const double *V0,
+                         const sph_field *theta);
 /* the checkpointer's state (borrowed; its X/V are the final state after sph_ckpt_forward) */
 int sph_ckpt_state(sph_ckpt *ck, sph_state **st);
 /* backward! from seeds at the final state; fills ws like sph_backward */
 int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
                       const double *seedX, const double *seedV);
-/* {epochs, snapshots, forward epochs of the last forward, forward epochs of the
- *  last backward (recomputation included), epochs taped in the last backward} */
+/* the same with seeds in device memory */
+int sph_ckpt_backward_dev(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
+                          const double *seedX, const double *seedV);
+/* {segments, snapshots, segments run by the last forward, segments run by the
+ *  last backward (recomputation included), segments taped in the last backward} */
 int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5);
-/* the schedule alone, on the host: {forward epochs in total, taped epochs,
- * most snapshots alive at once} for `epochs` epochs and `nsnap` snapshots */
+/* the schedule alone, on the host: {segments run in total, segments taped,
+ * most snapshots alive at once} for `epochs` segments and `nsnap` snapshots */
 int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3);
 
 /* --- measurement: per-kernel GPU time with events on the context's stream.

@@ -65,7 +65,17 @@ SIGNATURES = {
     "sph_workspace_get": (C.c_int, [_vp, C.c_int, _vp]),
     "sph_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _i64, _i64, C.c_int]),
     "sph_target_objective": (C.c_int, [_vp, _d, _d, _d, C.POINTER(_d), _vp]),
-    "sph_ckpt_create": (C.c_int, [_vp, _PP, C.c_int, _i64, _i64, _i64, _d, _i64, _i64, _pp]),
+    "sph_buffer_create": (C.c_int, [_vp, _i64, _vp, _pp]),
+    "sph_buffer_destroy": (C.c_int, [_vp]),
+    "sph_buffer_ptr": (C.c_int, [_vp, _pp]),
+    "sph_buffer_read": (C.c_int, [_vp, _vp, _i64]),
+    "sph_buffer_write": (C.c_int, [_vp, _vp, _i64]),
+    "sph_backward_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _i64, _i64, C.c_int]),
+    "sph_target_objective_dev": (C.c_int, [_vp, _d, _d, _d, C.POINTER(_d), _vp]),
+    "sph_ckpt_forward_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "sph_ckpt_backward_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "sph_ckpt_create": (C.c_int, [_vp, _PP, C.c_int, _i64, _i64, _i64, _i64, _d, _i64, _i64,
+                                  _pp]),
     "sph_ckpt_destroy": (C.c_int, [_vp]),
     "sph_ckpt_forward": (C.c_int, [_vp, _vp, _vp, _vp]),
     "sph_ckpt_state": (C.c_int, [_vp, _pp]),

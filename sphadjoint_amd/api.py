@@ -124,6 +124,49 @@ def _field(theta, p: SPHParams, backend: MI355XBackend) -> Field:
     return f
 
 
+class DeviceArray:
+    """A (2, N) float64 array in GPU memory owned by the library (sph_buffer),
+    stored interleaved (x, y per particle) as the ABI expects.  Inputs and
+    outputs of the device-memory calls: initial states, adjoint seeds."""
+
+    def __init__(self, backend: MI355XBackend, A=None, N=None):
+        if A is not None:
+            a = _as_2n(A, N)
+            self.N = a.shape[0]
+            src = _ptr(a)
+        else:
+            self.N = int(N)
+            src = None
+        self.backend = backend
+        h = C.c_void_p()
+        L.call("sph_buffer_create", backend._h, 16 * self.N, src, C.byref(h))
+        self._h = h
+        p = C.c_void_p()
+        L.call("sph_buffer_ptr", self._h, C.byref(p))
+        self.ptr = p
+
+    def numpy(self):
+        out = np.empty((self.N, 2))
+        L.call("sph_buffer_read", self._h, _ptr(out), 16 * self.N)
+        return np.ascontiguousarray(out.T)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.load().sph_buffer_destroy(h)
+            self._h = None
+
+
+def _is_device(A):
+    return isinstance(A, DeviceArray)
+
+
+def _dev_2n(A, N):
+    if A.N != N:
+        raise ValueError(f"expected a device array of N={N}, got {A.N}")
+    return A, A.ptr
+
+
 def _as_2n(A, N=None):
     A = np.asarray(A, dtype=np.float64)
     if A.ndim != 2 or A.shape[0] != 2:
@@ -395,10 +438,17 @@ def backward_(ws: AdjointWorkspace, tape: Tape, theta, p: SPHParams, backend: MI
     if layout == "auto":
         layout = "slot"
     f = _field(theta, ws.p, ws.backend)
+    lay = L.SPH_LAYOUT_SLOT if layout == "slot" else L.SPH_LAYOUT_PARTICLE
+    if _is_device(seedX) and _is_device(seedV):
+        bx, px = _dev_2n(seedX, ws.N)
+        bv, pv = _dev_2n(seedV, ws.N)
+        L.call("sph_backward_dev", ws._h, tape._h, f._h, px, pv, float(skin), int(interval),
+               int(maxnb), lay)
+        return ws
     sx = _as_2n(seedX, ws.N)
     sv = _as_2n(seedV, ws.N)
     L.call("sph_backward", ws._h, tape._h, f._h, _ptr(sx), _ptr(sv), float(skin), int(interval),
-           int(maxnb), L.SPH_LAYOUT_SLOT if layout == "slot" else L.SPH_LAYOUT_PARTICLE)
+           int(maxnb), lay)
     return ws
 
 
@@ -406,45 +456,61 @@ class Checkpointer:
     """Binomial (revolve) checkpointing of simulate!/backward! over ``nsteps``.
 
     The reference's Tape keeps the state before every step (src/driver.jl:38-83);
-    this keeps at most ``snapshots`` states, taken at neighbour-list epoch starts
-    (every ``interval`` steps), at the positions of Griewank & Walther's binomial
-    schedule, and recomputes each epoch into a one-epoch tape right before its
-    reverse sweep.  Gradients are bit-identical to a full-tape ``backward_``;
-    recomputation costs about ``t`` extra forward passes for ``E`` epochs when
-    C(snapshots + t, t) >= E.
+    this cuts the horizon into segments of ``seg_epochs`` neighbour-list epochs
+    (``seg_epochs * interval`` steps), keeps at most ``snapshots`` segment-start
+    states at the positions of Griewank & Walther's binomial schedule, and
+    recomputes each segment onto a one-segment tape right before its reverse
+    sweep.  Gradients are bit-identical to a full-tape ``backward_``.  For E
+    segments, recomputation costs about ``t`` extra forward passes where
+    C(snapshots + 1 + t, t) >= E; with snapshots >= E - 1 every segment runs
+    twice in all (the minimum for any scheme that does not keep every step).
 
     ``forward(X0, V0, theta)`` runs the whole horizon and returns the final
     state (owned by the checkpointer; ``backward`` reuses its buffers, so read
     the final state before calling ``backward``).
     """
 
-    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, *, snapshots=64, skin=0.2,
-                 interval=4, maxnb=64, precision="f64"):
+    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, *, snapshots=64,
+                 seg_epochs=1, skin=0.2, interval=4, maxnb=64, precision="f64"):
         self.backend, self.p, self.N = backend, p, int(N)
         self.nsteps, self.skin, self.interval = int(nsteps), float(skin), int(interval)
+        self.seg_epochs = int(seg_epochs)
         self.precision = _PREC[precision]
         cp = p.c_struct()
         h = C.c_void_p()
         L.call("sph_ckpt_create", backend._h, C.byref(cp), self.precision, self.N, self.nsteps,
-               int(snapshots), self.skin, self.interval, int(maxnb), C.byref(h))
+               int(snapshots), self.seg_epochs, self.skin, self.interval, int(maxnb), C.byref(h))
         self._h = h
         sh = C.c_void_p()
         L.call("sph_ckpt_state", self._h, C.byref(sh))
         self.state = State._borrow(sh, self, backend, p, self.N, self.skin, self.precision)
         self._theta = None
 
-    def forward(self, X0, V0, theta):
-        x0 = _as_2n(X0, self.N)
-        v0 = _as_2n(V0, self.N)
+    def forward(self, X0, V0, theta, check_flags=True):
+        """X0, V0: (2, N) numpy arrays, or DeviceArrays."""
         self._theta = _field(theta, self.p, self.backend)
-        L.call("sph_ckpt_forward", self._h, _ptr(x0), _ptr(v0), self._theta._h)
-        check_neighbor_flags(self.state.nl)
+        if _is_device(X0) and _is_device(V0):
+            bx, px = _dev_2n(X0, self.N)
+            bv, pv = _dev_2n(V0, self.N)
+            L.call("sph_ckpt_forward_dev", self._h, px, pv, self._theta._h)
+        else:
+            x0 = _as_2n(X0, self.N)
+            v0 = _as_2n(V0, self.N)
+            L.call("sph_ckpt_forward", self._h, _ptr(x0), _ptr(v0), self._theta._h)
+        if check_flags:
+            check_neighbor_flags(self.state.nl)
         return self.state
 
     def backward(self, ws: AdjointWorkspace, *, seedX, seedV, theta=None):
+        """seedX, seedV: (2, N) numpy arrays, or DeviceArrays."""
         f = self._theta if theta is None else _field(theta, self.p, self.backend)
         if f is None:
             raise RuntimeError("Checkpointer.backward needs a preceding forward")
+        if _is_device(seedX) and _is_device(seedV):
+            bx, px = _dev_2n(seedX, ws.N)
+            bv, pv = _dev_2n(seedV, ws.N)
+            L.call("sph_ckpt_backward_dev", self._h, ws._h, f._h, px, pv)
+            return ws
         sx = _as_2n(seedX, ws.N)
         sv = _as_2n(seedV, ws.N)
         L.call("sph_ckpt_backward", self._h, ws._h, f._h, _ptr(sx), _ptr(sv))
@@ -474,11 +540,18 @@ def ckpt_plan(epochs, snapshots):
     return dict(forward_epochs=int(o[0]), taped=int(o[1]), peak=int(o[2]))
 
 
-def target_objective(st: State, xc, yc, sigma):
-    """target_objective(st.X, xc, yc, sigma) -> (J, dJ/dX_T of shape (2, N)), on the device."""
+def target_objective(st: State, xc, yc, sigma, device=False):
+    """target_objective(st.X, xc, yc, sigma) -> (J, dJ/dX_T of shape (2, N)), evaluated
+    on the device.  With ``device=True`` (or a DeviceArray to fill) the seed stays
+    in GPU memory as a :class:`DeviceArray`."""
     if not isinstance(st, State):
         raise TypeError("target_objective takes the State whose X is evaluated on the device")
     J = C.c_double()
+    if device:
+        buf = device if isinstance(device, DeviceArray) else DeviceArray(st.backend, N=st.N)
+        L.call("sph_target_objective_dev", st._h, float(xc), float(yc), float(sigma), C.byref(J),
+               buf.ptr)
+        return J.value, buf
     seed = np.empty((st.N, 2))
     L.call("sph_target_objective", st._h, float(xc), float(yc), float(sigma), C.byref(J), _ptr(seed))
     return J.value, np.ascontiguousarray(seed.T)

@@ -12,7 +12,9 @@ Tape, as the reference does; with an integer it runs the binomial
 checkpointing schedule of :class:`sphadjoint_amd.Checkpointer` under that
 snapshot budget.  Both give the same gradients bit for bit.
 
-Tensors cross the boundary through host memory: the C ABI takes host arrays.
+Tensors cross the boundary through host memory and must live on the CPU:
+libsphadj drives the GPU through the system HIP runtime, while PyTorch wheels
+carry their own copy of it, and the two cannot share the device in one process.
 """
 from __future__ import annotations
 
@@ -26,7 +28,7 @@ class Simulation:
     """Device buffers for repeated forward/adjoint passes over one scene size."""
 
     def __init__(self, backend: A.MI355XBackend, p: A.SPHParams, N, nsteps, *, snapshots=None,
-                 skin=0.2, interval=4, maxnb=64, precision="f64"):
+                 seg_epochs=1, skin=0.2, interval=4, maxnb=64, precision="f64"):
         self.backend, self.p, self.N, self.nsteps = backend, p, int(N), int(nsteps)
         self.skin, self.interval, self.maxnb, self.precision = skin, interval, maxnb, precision
         self.snapshots = snapshots
@@ -36,8 +38,8 @@ class Simulation:
             self.tape = A.Tape(backend, self.N, self.nsteps, p, precision=precision)
         else:
             self.ck = A.Checkpointer(backend, self.N, self.nsteps, p, snapshots=snapshots,
-                                     skin=skin, interval=interval, maxnb=maxnb,
-                                     precision=precision)
+                                     seg_epochs=seg_epochs, skin=skin, interval=interval,
+                                     maxnb=maxnb, precision=precision)
             self.tape = None
         self._serial = 0       # forward passes run
         self._ready = -1       # serial of the forward the buffers hold
@@ -78,8 +80,7 @@ class _SimulateFn(torch.autograd.Function):
                              V0.detach().cpu().numpy())
         ctx.sim = sim
         ctx.serial = sim._serial
-        ctx.devices = (theta.device, X0.device, V0.device)
-        return (torch.from_numpy(XT).to(theta.device), torch.from_numpy(VT).to(theta.device))
+        return torch.from_numpy(XT), torch.from_numpy(VT)
 
     @staticmethod
     def backward(ctx, gXT, gVT):
@@ -87,9 +88,7 @@ class _SimulateFn(torch.autograd.Function):
         gx = np.zeros((2, sim.N)) if gXT is None else gXT.detach().cpu().double().numpy()
         gv = np.zeros((2, sim.N)) if gVT is None else gVT.detach().cpu().double().numpy()
         gth, gX0, gV0 = sim.adjoint(gx, gv, serial=ctx.serial)
-        dt, dx, dv = ctx.devices
-        return (torch.from_numpy(gth).to(dt), torch.from_numpy(gX0).to(dx),
-                torch.from_numpy(gV0).to(dv), None)
+        return torch.from_numpy(gth), torch.from_numpy(gX0), torch.from_numpy(gV0), None
 
 
 def simulate(theta: torch.Tensor, X0: torch.Tensor, V0: torch.Tensor, sim: Simulation):
@@ -99,6 +98,9 @@ def simulate(theta: torch.Tensor, X0: torch.Tensor, V0: torch.Tensor, sim: Simul
     for t, name in ((theta, "theta"), (X0, "X0"), (V0, "V0")):
         if t.dtype != torch.float64:
             raise TypeError(f"{name} must be float64")
+        if t.device.type != "cpu":
+            raise TypeError(f"{name} must be a CPU tensor (the simulation's own device memory "
+                            "holds the state)")
     return _SimulateFn.apply(theta, X0, V0, sim)
 
 

@@ -264,6 +264,12 @@ struct sph_field {
   double *d = nullptr;
 };
 
+struct sph_buffer {
+  sph_ctx *ctx = nullptr;
+  int64_t nbytes = 0;
+  unsigned char *d = nullptr;
+};
+
 struct sph_state {
   sph_ctx *ctx = nullptr;
   sph_params p{};
@@ -665,6 +671,70 @@ int sph_field_destroy(sph_field *f) {
     sph_ctx *c = f->ctx;
     delete f;
     ctx_release(c);
+  });
+}
+
+int sph_buffer_create(sph_ctx *ctx, int64_t nbytes, const void *host, sph_buffer **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    if (nbytes < 1) fail(SPH_E_ARG, "buffer size must be >= 1 byte");
+    HIPC(hipSetDevice(ctx->device));
+    auto *b = new sph_buffer;
+    b->ctx = ctx;
+    b->nbytes = nbytes;
+    try {
+      b->d = dalloc<unsigned char>((size_t)nbytes);
+      hipStream_t s = ctx->stream;
+      if (host) HIPC(hipMemcpyAsync(b->d, host, (size_t)nbytes, hipMemcpyHostToDevice, s));
+      else HIPC(hipMemsetAsync(b->d, 0, (size_t)nbytes, s));
+      HIPC(hipStreamSynchronize(s));
+    } catch (...) {
+      dfree(b->d);
+      delete b;
+      throw;
+    }
+    ctx_retain(ctx);
+    *out = b;
+  });
+}
+
+int sph_buffer_destroy(sph_buffer *b) {
+  return guarded([&] {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    dfree(b->d);
+    sph_ctx *c = b->ctx;
+    delete b;
+    ctx_release(c);
+  });
+}
+
+int sph_buffer_ptr(const sph_buffer *b, void **ptr) {
+  return guarded([&] {
+    if (!b || !ptr) fail(SPH_E_ARG, "NULL argument");
+    *ptr = b->d;
+  });
+}
+
+int sph_buffer_read(const sph_buffer *b, void *host, int64_t nbytes) {
+  return guarded([&] {
+    if (!b || !host) fail(SPH_E_ARG, "NULL argument");
+    if (nbytes < 0 || nbytes > b->nbytes) fail(SPH_E_ARG, "read of %lld bytes from a %lld-byte buffer",
+                                                 (long long)nbytes, (long long)b->nbytes);
+    hipStream_t s = b->ctx->stream;
+    HIPC(hipMemcpyAsync(host, b->d, (size_t)nbytes, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+int sph_buffer_write(sph_buffer *b, const void *host, int64_t nbytes) {
+  return guarded([&] {
+    if (!b || !host) fail(SPH_E_ARG, "NULL argument");
+    if (nbytes < 0 || nbytes > b->nbytes) fail(SPH_E_ARG, "write of %lld bytes to a %lld-byte buffer",
+                                                 (long long)nbytes, (long long)b->nbytes);
+    hipStream_t s = b->ctx->stream;
+    HIPC(hipMemcpyAsync(b->d, host, (size_t)nbytes, hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
   });
 }
 
@@ -1222,7 +1292,7 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
 // Reverse sweep over every entry of `tape`.  The adjoint state (gX, gV) enters
 // in the slot order whose ids are `ids_in` and leaves in the slot order of the
 // tape's first entry, whose ids are copied to ws->ids_cur.  When `lnl` is
-// given it is the already built neighbour list of the tape's single epoch.
+// given it is the already built neighbour list of the tape's last epoch.
 void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int *ids_in,
             int maxnb, const int *lnl, const int *lcnt) {
   hipStream_t s = ws->ctx->stream;
@@ -1250,10 +1320,11 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       std::swap(ws->gV, ws->t2);
     }
   });
-  const bool use_given = lnl != nullptr && tape->epochs.size() == 1;
+  // the given list belongs to the tape's last epoch
+  const bool use_given = lnl != nullptr;
   const int *nl = use_given ? lnl : ws->nl;
   const int *nbcnt = use_given ? lcnt : ws->nbcnt;
-  int listed = use_given ? 0 : -1;  // epoch whose neighbour list `nl` holds
+  int listed = use_given ? (int)tape->epochs.size() - 1 : -1;  // epoch whose list `nl` holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
@@ -1273,6 +1344,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         }
       });
       listed = e;
+      nl = ws->nl;
+      nbcnt = ws->nbcnt;
     }
     if (f32c) {
       const float2 *Vk = tape->V32 + off;
@@ -1354,21 +1427,26 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
                       hipMemcpyDeviceToDevice, s));
 }
 
-// seeds (host, particle-id order) into the adjoint state, in id order
-void bw_seed(sph_workspace *ws, const double *seedX, const double *seedV) {
+// seeds (particle-id order; host memory, or device memory when `dev`) into
+// the adjoint state, in id order
+void bw_seed(sph_workspace *ws, const double *seedX, const double *seedV, bool dev) {
   hipStream_t s = ws->ctx->stream;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
   HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
-  if (ws->precision == SPH_F32C) {
+  if (ws->precision == SPH_F32C && dev) {
+    launch_d2_to_f2(s, reinterpret_cast<const double2 *>(seedX), N, ws->gX32);
+    launch_d2_to_f2(s, reinterpret_cast<const double2 *>(seedV), N, ws->gV32);
+  } else if (ws->precision == SPH_F32C) {
     HIPC(hipMemcpyAsync(ws->t2, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
     launch_d2_to_f2(s, ws->t2, N, ws->gX32);
     HIPC(hipStreamSynchronize(s));
     HIPC(hipMemcpyAsync(ws->t2, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
     launch_d2_to_f2(s, ws->t2, N, ws->gV32);
   } else {
-    HIPC(hipMemcpyAsync(ws->gX, seedX, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    HIPC(hipMemcpyAsync(ws->gV, seedV, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPC(hipMemcpyAsync(ws->gX, seedX, sizeof(double2) * n, kind, s));
+    HIPC(hipMemcpyAsync(ws->gV, seedV, sizeof(double2) * n, kind, s));
   }
   HIPC(hipMemcpyAsync(ws->ids_cur, ws->iota, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
 }
@@ -1394,8 +1472,13 @@ void check_bw_args(sph_workspace *ws, const sph_field *theta, const double *seed
 
 }  // namespace
 
-int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
-                 const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout) {
+}  // extern "C"
+
+namespace {
+
+int backward_impl(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
+                  const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout,
+                  bool dev) {
   return guarded([&] {
     check_bw_args(ws, theta, seedX, seedV);
     if (!tape) fail(SPH_E_ARG, "NULL argument");
@@ -1405,7 +1488,7 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, cons
     if (layout != SPH_LAYOUT_SLOT && layout != SPH_LAYOUT_PARTICLE) fail(SPH_E_ARG, "bad layout");
     hipStream_t s = ws->ctx->stream;
     ensure_ws_list(ws, maxnb);
-    bw_seed(ws, seedX, seedV);
+    bw_seed(ws, seedX, seedV, dev);
     if (tape->n > 0) bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr);
     bw_finish(ws);
     HIPC(hipStreamSynchronize(s));
@@ -1413,8 +1496,8 @@ int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, cons
   });
 }
 
-int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
-                         double *seed) {
+int objective_impl(sph_state *st, double xc, double yc, double sigma, double *J, double *seed,
+                   bool dev) {
   return guarded([&] {
     if (!st || !J) fail(SPH_E_ARG, "NULL argument");
     hipStream_t s = st->ctx->stream;
@@ -1422,12 +1505,39 @@ int sph_target_objective(sph_state *st, double xc, double yc, double sigma, doub
     launch_objective_f64(s, st->X, N, xc, yc, sigma, st->obj_part, st->obj_nb, st->obj_J,
                          seed ? st->seed_slot : nullptr);
     HIPC(hipMemcpyAsync(J, st->obj_J, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (seed) {
+    if (seed && dev) {
+      launch_scatter2(s, st->seed_slot, st->ids, N, reinterpret_cast<double2 *>(seed));
+    } else if (seed) {
       launch_scatter2(s, st->seed_slot, st->ids, N, st->tmp2);
       HIPC(hipMemcpyAsync(seed, st->tmp2, sizeof(double2) * N, hipMemcpyDeviceToHost, s));
     }
     HIPC(hipStreamSynchronize(s));
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_backward(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const double *seedX,
+                 const double *seedV, double skin, int64_t interval, int64_t maxnb, int layout) {
+  return backward_impl(ws, tape, theta, seedX, seedV, skin, interval, maxnb, layout, false);
+}
+
+int sph_backward_dev(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
+                     const double *seedX, const double *seedV, double skin, int64_t interval,
+                     int64_t maxnb, int layout) {
+  return backward_impl(ws, tape, theta, seedX, seedV, skin, interval, maxnb, layout, true);
+}
+
+int sph_target_objective(sph_state *st, double xc, double yc, double sigma, double *J,
+                         double *seed) {
+  return objective_impl(st, xc, yc, sigma, J, seed, false);
+}
+
+int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
+                             double *seed) {
+  return objective_impl(st, xc, yc, sigma, J, seed, true);
 }
 
 }  // extern "C"
@@ -1584,6 +1694,7 @@ struct sph_ckpt {
   sph_params p{};
   int precision = SPH_F64;
   int64_t N = 0, nsteps = 0, nsnap = 0, interval = 4, maxnb = 64, E = 0;
+  int64_t seg = 4;                      // steps per schedule unit (whole epochs)
   double skin = 0.2;
   sph_state *work = nullptr;
   sph_tape *mini = nullptr;
@@ -1602,13 +1713,15 @@ struct sph_ckpt {
   const sph_field *theta = nullptr;
   sph_workspace *ws = nullptr;
   int64_t fwd_advanced = 0, fwd_taped = 0;
+  int *iota = nullptr;                  // particle-id order as a slot order
 };
 
 namespace {
 
+// steps of schedule unit e (the last one may be short)
 int64_t ck_steps(const sph_ckpt *ck, int64_t e) {
-  int64_t rest = ck->nsteps - e * ck->interval;
-  return rest < ck->interval ? rest : ck->interval;
+  int64_t rest = ck->nsteps - e * ck->seg;
+  return rest < ck->seg ? rest : ck->seg;
 }
 
 void ck_copy(sph_ckpt *ck, bool to_snap, int slot) {
@@ -1706,6 +1819,7 @@ void ck_free(sph_ckpt *ck) {
     dfree(q.V32);
     dfree(q.ids);
   }
+  dfree(ck->iota);
   if (ck->work) sph_state_destroy(ck->work);
   if (ck->mini) sph_tape_destroy(ck->mini);
 }
@@ -1729,7 +1843,8 @@ int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3) {
 }
 
 int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, int64_t nsteps,
-                    int64_t nsnap, double skin, int64_t interval, int64_t maxnb, sph_ckpt **out) {
+                    int64_t nsnap, int64_t seg_epochs, double skin, int64_t interval,
+                    int64_t maxnb, sph_ckpt **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
     check_params(p);
@@ -1737,6 +1852,7 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
     if (nsteps < 1) fail(SPH_E_ARG, "nsteps must be >= 1");
     if (nsnap < 0) fail(SPH_E_ARG, "nsnap must be >= 0");
     if (interval < 1) fail(SPH_E_ARG, "interval must be >= 1");
+    if (seg_epochs < 1) fail(SPH_E_ARG, "seg_epochs must be >= 1");
     auto *ck = new sph_ckpt;
     try {
       ck->ctx = ctx;
@@ -1747,16 +1863,18 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
       ck->interval = interval;
       ck->maxnb = maxnb;
       ck->skin = skin;
-      ck->E = (nsteps + interval - 1) / interval;
+      ck->seg = seg_epochs * interval;
+      ck->E = (nsteps + ck->seg - 1) / ck->seg;
       ck->nsnap = nsnap < ck->E ? nsnap : ck->E;
       std::vector<double> zeros((size_t)N * 2, 0.0);
       int rc = sph_state_create(ctx, p, precision, N, zeros.data(), zeros.data(), skin, interval,
                                 maxnb, SPH_LAYOUT_SLOT, &ck->work);
       if (rc != SPH_OK) throw Error(rc, g_err);
-      rc = sph_tape_create(ctx, N, interval, precision, &ck->mini);
+      const int64_t cap = ck->seg < nsteps ? ck->seg : nsteps;
+      rc = sph_tape_create(ctx, N, cap, precision, &ck->mini);
       if (rc != SPH_OK) throw Error(rc, g_err);
       ck->mini->keep_pi = true;
-      const size_t tot = (size_t)(interval + 1) * (size_t)N;
+      const size_t tot = (size_t)(cap + 1) * (size_t)N;
       if (precision == SPH_F32C) ck->mini->PI32 = dalloc<float2>(tot);
       else ck->mini->PI = dalloc<double2>(tot);
       ck->snaps.resize((size_t)ck->nsnap + 1);
@@ -1790,8 +1908,14 @@ int sph_ckpt_destroy(sph_ckpt *ck) {
   });
 }
 
-// forward over nsteps from (X0, V0) (host, id order); snapshots per the schedule
-int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta) {
+}  // extern "C"
+
+namespace {
+
+// forward over nsteps from (X0, V0) (id order; host memory, or device memory
+// when `dev`); snapshots per the schedule
+int ckpt_forward_impl(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta,
+                      bool dev) {
   return guarded([&] {
     if (!ck || !X0 || !V0 || !theta) fail(SPH_E_ARG, "NULL argument");
     if (theta->ngy != ck->p.ngy || theta->ngx != ck->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
@@ -1799,17 +1923,27 @@ int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph
     hipStream_t s = ck->ctx->stream;
     const size_t n = (size_t)ck->N;
     // reset the work state to (X0, V0) in particle-id order
-    std::vector<int> io(n);
-    for (size_t i = 0; i < n; ++i) io[i] = (int)i;
-    HIPC(hipMemcpyAsync(st->ids, io.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    HIPC(hipMemcpyAsync(st->X, X0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-    if (ck->precision == SPH_F32C) {
-      HIPC(hipMemcpyAsync(st->tmp2, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
-      launch_d2_to_f2(s, st->tmp2, (int)n, st->V32);
-    } else {
-      HIPC(hipMemcpyAsync(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    if (!ck->iota) {
+      std::vector<int> io(n);
+      for (size_t i = 0; i < n; ++i) io[i] = (int)i;
+      ck->iota = dalloc<int>(n);
+      HIPC(hipMemcpyAsync(ck->iota, io.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+      HIPC(hipStreamSynchronize(s));
     }
-    HIPC(hipStreamSynchronize(s));
+    HIPC(hipMemcpyAsync(st->ids, ck->iota, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPC(hipMemcpyAsync(st->X, X0, sizeof(double2) * n, kind, s));
+    if (ck->precision == SPH_F32C) {
+      const double2 *v0 = reinterpret_cast<const double2 *>(V0);
+      if (!dev) {
+        HIPC(hipMemcpyAsync(st->tmp2, V0, sizeof(double2) * n, kind, s));
+        v0 = st->tmp2;
+      }
+      launch_d2_to_f2(s, v0, (int)n, st->V32);
+    } else {
+      HIPC(hipMemcpyAsync(st->V, V0, sizeof(double2) * n, kind, s));
+    }
+    if (!dev) HIPC(hipStreamSynchronize(s));  // the host buffers are released on return
     st->age = kAgeNever;
     st->aux_state = 0;
     st->epoch += 1;
@@ -1832,15 +1966,8 @@ int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph
   });
 }
 
-int sph_ckpt_state(sph_ckpt *ck, sph_state **st) {
-  return guarded([&] {
-    if (!ck || !st) fail(SPH_E_ARG, "NULL argument");
-    *st = ck->work;
-  });
-}
-
-int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta, const double *seedX,
-                      const double *seedV) {
+int ckpt_backward_impl(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
+                       const double *seedX, const double *seedV, bool dev) {
   return guarded([&] {
     if (!ck) fail(SPH_E_ARG, "NULL argument");
     check_bw_args(ws, theta, seedX, seedV);
@@ -1851,7 +1978,7 @@ int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta, c
     ck->theta = theta;
     ck->ws = ws;
     ensure_ws_list(ws, ck->maxnb);
-    bw_seed(ws, seedX, seedV);
+    bw_seed(ws, seedX, seedV, dev);
     ck->sched.advanced = ck->sched.taped = 0;
     ck->sched.live = (int64_t)ck->sched.chain.size() - 1;
     ck->sched.backward();
@@ -1859,6 +1986,36 @@ int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta, c
     ck->forward_done = false;
     HIPC(hipStreamSynchronize(ws->ctx->stream));
     prof_collect(ws->ctx);
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_ckpt_forward(sph_ckpt *ck, const double *X0, const double *V0, const sph_field *theta) {
+  return ckpt_forward_impl(ck, X0, V0, theta, false);
+}
+
+int sph_ckpt_forward_dev(sph_ckpt *ck, const double *X0, const double *V0,
+                         const sph_field *theta) {
+  return ckpt_forward_impl(ck, X0, V0, theta, true);
+}
+
+int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta, const double *seedX,
+                      const double *seedV) {
+  return ckpt_backward_impl(ck, ws, theta, seedX, seedV, false);
+}
+
+int sph_ckpt_backward_dev(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
+                          const double *seedX, const double *seedV) {
+  return ckpt_backward_impl(ck, ws, theta, seedX, seedV, true);
+}
+
+int sph_ckpt_state(sph_ckpt *ck, sph_state **st) {
+  return guarded([&] {
+    if (!ck || !st) fail(SPH_E_ARG, "NULL argument");
+    *st = ck->work;
   });
 }
 

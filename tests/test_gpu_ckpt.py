@@ -39,10 +39,10 @@ def full_tape(be, sc, nsteps, interval, cX, cV, precision):
     return X, V, ws.gX, ws.gV, ws.gtheta
 
 
-def checkpointed(be, sc, nsteps, interval, snapshots, cX, cV, precision, repeat=1):
+def checkpointed(be, sc, nsteps, interval, snapshots, seg, cX, cV, precision, repeat=1):
     pg = S.SPHParams(**sc.params)
-    ck = S.Checkpointer(be, sc.N, nsteps, pg, snapshots=snapshots, interval=interval,
-                        precision=precision)
+    ck = S.Checkpointer(be, sc.N, nsteps, pg, snapshots=snapshots, seg_epochs=seg,
+                        interval=interval, precision=precision)
     ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
     out = []
     for _ in range(repeat):
@@ -54,29 +54,33 @@ def checkpointed(be, sc, nsteps, interval, snapshots, cX, cV, precision, repeat=
 
 
 CASES = [
-    # (steps, interval, snapshots): ragged last epoch, no snapshot at all,
-    # one step per epoch, more snapshots than epochs, a single epoch
-    (37, 4, 3),
-    (30, 4, 0),
-    (25, 1, 2),
-    (16, 4, 10),
-    (3, 4, 2),
+    # (steps, interval, snapshots, epochs per segment): ragged last epoch, no
+    # snapshot at all, one step per epoch, more snapshots than segments, a
+    # single epoch, multi-epoch segments (ragged, exact, every segment stored)
+    (37, 4, 3, 1),
+    (30, 4, 0, 1),
+    (25, 1, 2, 1),
+    (16, 4, 10, 1),
+    (3, 4, 2, 1),
+    (37, 4, 2, 3),
+    (40, 2, 1, 4),
+    (37, 4, 20, 2),
 ]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32c"])
-@pytest.mark.parametrize("nsteps,interval,snapshots", CASES)
-def test_checkpointed_equals_full_tape(be, precision, nsteps, interval, snapshots):
-    say("ckpt vs full tape", precision, nsteps, interval, snapshots)
+@pytest.mark.parametrize("nsteps,interval,snapshots,seg", CASES)
+def test_checkpointed_equals_full_tape(be, precision, nsteps, interval, snapshots, seg):
+    say("ckpt vs full tape", precision, nsteps, interval, snapshots, seg)
     sc = scenes.brinkman_56x37(nsteps=nsteps)
     rng = np.random.default_rng(nsteps * 31 + snapshots)
     cX = rng.standard_normal((2, sc.N))
     cV = rng.standard_normal((2, sc.N))
     want = full_tape(be, sc, nsteps, interval, cX, cV, precision)
-    got, stats = checkpointed(be, sc, nsteps, interval, snapshots, cX, cV, precision)
+    got, stats = checkpointed(be, sc, nsteps, interval, snapshots, seg, cX, cV, precision)
     for a, b, name in zip(got[0], want, ("X", "V", "gX", "gV", "gtheta")):
         np.testing.assert_array_equal(a, b, err_msg=name)
-    E = -(-nsteps // interval)
+    E = -(-nsteps // (interval * seg))
     plan = S.ckpt_plan(E, min(snapshots, E))
     assert stats["epochs"] == E
     assert stats["forward_epochs"] + stats["backward_forward_epochs"] == plan["forward_epochs"]
@@ -154,3 +158,36 @@ def test_torch_autograd_matches_adjoint(be):
     np.testing.assert_array_equal(res[None][1].numpy(), ws.gtheta)
     np.testing.assert_array_equal(res[None][2].numpy(), ws.gX)
     np.testing.assert_array_equal(res[None][3].numpy(), ws.gV)
+
+
+def test_device_memory_variants(be):
+    say("device-memory inputs and seeds")
+    sc = scenes.brinkman_56x37(nsteps=20)
+    pg = S.SPHParams(**sc.params)
+    for precision in ("f64", "f32c"):
+        ck = S.Checkpointer(be, sc.N, 20, pg, snapshots=2, seg_epochs=2, precision=precision)
+        ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
+        st = ck.forward(sc.X0, sc.V0, sc.theta)
+        J, seed = S.target_objective(st, 0.3, 0.2, 0.1)
+        ck.backward(ws, seedX=seed, seedV=np.zeros_like(seed))
+        host = (J, seed, ws.gX, ws.gV, ws.gtheta)
+        X0 = S.DeviceArray(be, sc.X0)
+        V0 = S.DeviceArray(be, sc.V0)
+        zero = S.DeviceArray(be, N=sc.N)
+        np.testing.assert_array_equal(X0.numpy(), sc.X0)
+        st = ck.forward(X0, V0, sc.theta)
+        Jd, seedd = S.target_objective(st, 0.3, 0.2, 0.1, device=True)
+        ck.backward(ws, seedX=seedd, seedV=zero)
+        assert Jd == host[0]
+        np.testing.assert_array_equal(seedd.numpy(), host[1])
+        np.testing.assert_array_equal(ws.gX, host[2])
+        np.testing.assert_array_equal(ws.gV, host[3])
+        np.testing.assert_array_equal(ws.gtheta, host[4])
+        # full tape with device seeds
+        st = S.State(be, sc.X0, sc.V0, pg, precision=precision)
+        tape = S.Tape(be, sc.N, 20, pg, precision=precision)
+        S.simulate_(st, sc.theta, pg, be, 20, tape=tape)
+        Jt, seedt = S.target_objective(st, 0.3, 0.2, 0.1, device=True)
+        S.backward_(ws, tape, sc.theta, pg, be, seedX=seedt, seedV=zero)
+        assert Jt == host[0]
+        np.testing.assert_array_equal(ws.gtheta, host[4])
