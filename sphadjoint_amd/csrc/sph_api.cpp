@@ -287,8 +287,10 @@ struct sph_state {
   // current layout; for the f32c form scell holds the binning cell of each slot
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
   float2 *V32 = nullptr, *V32n = nullptr, *xr = nullptr, *xrn = nullptr, *PI32 = nullptr;
-  // transposed neighbour list of the current layout (entry m of slot s at nl[m*N + s])
+  // neighbour list of the current layout: FP64 form transposed (entry m of
+  // slot s at nl[m*N + s]); f32c form row bitmasks (nm)
   int *nl = nullptr, *nbcnt = nullptr;
+  uint32_t *nm = nullptr;
   double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
   double2 *Xref = nullptr, *Xref2 = nullptr;
   double2 *PI = nullptr;
@@ -353,6 +355,7 @@ struct sph_workspace {
   int *inv = nullptr;
   double4 *tpart = nullptr;  // per theta cell partial sums of the gtheta gather
   int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
+  uint32_t *nm = nullptr;                                  // f32c form: row bitmasks
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
   int *iota = nullptr;     // particle-id order as a slot order
@@ -374,6 +377,7 @@ void state_free(sph_state *st) {
   for (auto pp : d2) dfree(*pp);
   float2 **f2[] = {&st->V32, &st->V32n, &st->xr, &st->xrn, &st->PI32};
   for (auto pp : f2) dfree(*pp);
+  dfree(st->nm);
   double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
                    &st->tmp1, &st->obj_part, &st->obj_J};
   for (auto pp : d1) dfree(*pp);
@@ -408,9 +412,10 @@ void materialise_aux(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
-    launch_density_f32c(s, st->xrn, st->nl, st->nbcnt, st->g, st->P32, st->PI32, st->rho_slot);
-    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->nl,
-                      st->nbcnt, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
+    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->g, st->P32, st->PI32,
+                        st->rho_slot);
+    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->nm,
+                      st->cstart, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
                       st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
@@ -451,9 +456,8 @@ void do_build_neighbors(sph_state *st) {
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C)
-      launch_nl_build_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32,
-                           (float)(st->kc.cs * st->kc.cs), (int)st->maxnb, st->nl, st->nbcnt,
-                           st->flags);
+      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32,
+                           (float)(st->kc.cs * st->kc.cs), st->nm, st->flags);
     else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
@@ -524,10 +528,10 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->nl, st->nbcnt, st->g, st->P32, PI32, nullptr);
+      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->g, st->P32, PI32, nullptr);
     });
     timed(st->ctx, K_FORCE, [&] {
-      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->nl, st->nbcnt,
+      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->nm, st->cstart,
                         st->g, st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
@@ -779,6 +783,7 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       ny = ny < 1 ? 1 : ny;
       if (nx * ny > (1LL << 30)) fail(SPH_E_ARG, "cell grid %lldx%lld too large", nx, ny);
       st->g.nx = (int)nx;
+      st->g.inv_nx = 1.0 / (double)nx;
       st->g.ny = (int)ny;
       st->g.ncell = (int)(nx * ny);
       st->g.N = (int)N;
@@ -808,8 +813,12 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
         st->Vn = dalloc<double2>(n);
         st->PI = dalloc<double2>(n);
       }
-      st->nl = dalloc<int>(n * (size_t)maxnb);
-      st->nbcnt = dalloc<int>(n);
+      if (f32c) {
+        st->nm = dalloc<uint32_t>(n * 3 * kMaskWords);
+      } else {
+        st->nl = dalloc<int>(n * (size_t)maxnb);
+        st->nbcnt = dalloc<int>(n);
+      }
       st->cellof = dalloc<int>(n);
       st->counts = dalloc<int>(nc);
       st->cl_cstart = dalloc<int>(nc + 1);
@@ -1171,6 +1180,7 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
   dfree(ws->galpha32); dfree(ws->grho32); dfree(ws->tpart);
   dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags); dfree(ws->ids_cur); dfree(ws->iota);
+  dfree(ws->nm);
 }
 
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
@@ -1282,6 +1292,10 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
 namespace {
 
 void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
+  if (ws->precision == SPH_F32C) {
+    if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
+    return;
+  }
   if (ws->nl_cap < maxnb) {
     dfree(ws->nl);
     ws->nl = dalloc<int>((size_t)ws->N * (size_t)maxnb);
@@ -1294,7 +1308,7 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
 // tape's first entry, whose ids are copied to ws->ids_cur.  When `lnl` is
 // given it is the already built neighbour list of the tape's last epoch.
 void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int *ids_in,
-            int maxnb, const int *lnl, const int *lcnt) {
+            int maxnb, const int *lnl, const int *lcnt, const uint32_t *lnm) {
   hipStream_t s = ws->ctx->stream;
   sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
@@ -1321,9 +1335,10 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
     }
   });
   // the given list belongs to the tape's last epoch
-  const bool use_given = lnl != nullptr;
+  const bool use_given = f32c ? lnm != nullptr : lnl != nullptr;
   const int *nl = use_given ? lnl : ws->nl;
   const int *nbcnt = use_given ? lcnt : ws->nbcnt;
+  const uint32_t *nm = use_given ? lnm : ws->nm;
   int listed = use_given ? (int)tape->epochs.size() - 1 : -1;  // epoch whose list `nl` holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
@@ -1336,8 +1351,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
           launch_xrel_f32c(s, X0, E.scell, g, P32, ws->xr32);
-          launch_nl_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P32,
-                               (float)(tape->rc * tape->rc), maxnb, ws->nl, ws->nbcnt, ws->flags);
+          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P32,
+                               (float)(tape->rc * tape->rc), ws->nm, ws->flags);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);
@@ -1346,6 +1361,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       listed = e;
       nl = ws->nl;
       nbcnt = ws->nbcnt;
+      nm = ws->nm;
     }
     if (f32c) {
       const float2 *Vk = tape->V32 + off;
@@ -1372,19 +1388,20 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
                               ws->gV32, ws->abar32, ws->galpha32);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, g, P32, ws->xr32);
-          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, nl, nbcnt, g, P32, ws->PI32,
-                            ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
+          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, g, P32,
+                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
         }
       });
       timed(c, K_THETA, [&] {
         launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
       });
       timed(c, K_ADJ_P1, [&] {
-        launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, nl, nbcnt,
-                           g, P32, ws->gXs32, ws->gV32, ws->grho32);
+        launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
+                           nm, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, nl, nbcnt, g, P32, ws->gX32);
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, nm, E.cstart, g, P32,
+                           ws->gX32);
       });
     } else {
       const double2 *Vk = tape->V + off;
@@ -1489,7 +1506,7 @@ int backward_impl(sph_workspace *ws, sph_tape *tape, const sph_field *theta, con
     hipStream_t s = ws->ctx->stream;
     ensure_ws_list(ws, maxnb);
     bw_seed(ws, seedX, seedV, dev);
-    if (tape->n > 0) bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr);
+    if (tape->n > 0) bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr, nullptr);
     bw_finish(ws);
     HIPC(hipStreamSynchronize(s));
     prof_collect(ws->ctx);
@@ -1808,7 +1825,7 @@ void ck_bind(sph_ckpt *ck) {
     // the work state has not stepped since the epoch was taped: its list is
     // the epoch's list
     bw_run(ck->ws, ck->mini, ck->theta, ck->ws->ids_cur, (int)ck->maxnb, ck->work->nl,
-           ck->work->nbcnt);
+           ck->work->nbcnt, ck->work->nm);
   };
 }
 

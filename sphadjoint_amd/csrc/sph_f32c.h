@@ -14,14 +14,16 @@ struct PhysF32 {
   int ngx, ngy;
 };
 
-void launch_nl_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          Grid g, PhysF32 P, float rc2, int maxnb, int *nl, int *nbcnt,
-                          int *flags);
-void launch_density_f32c(hipStream_t s, const float2 *xr, const int *nl, const int *nbcnt, Grid g,
-                         PhysF32 P, float2 *PI, double *rho);
+// bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)
+constexpr int kMaskWords = 4;
+
+void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
+                          Grid g, PhysF32 P, float rc2, uint32_t *nm, int *flags);
+void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
+                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho);
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
-                       const float2 *PI, const double *theta, const int *bcell, const int *nl,
-                       const int *nbcnt, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
+                       const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
+                       const int *cstart, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
                        double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out);
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
                       float2 *xr);
@@ -30,17 +32,20 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          double dmax2, double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn,
                          int *bcell, int *flags);
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
-                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
-                       float2 *PI, float2 *gX, float2 *gV, float2 *abar, float *galpha);
+                       const double *theta, const int *bcell, const uint32_t *nm,
+                       const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
+                       float2 *abar, float *galpha);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha);
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
-                        const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
-                        float2 *gXs, float2 *gV, float *grho);
+                        const double *theta, const int *bcell, const uint32_t *nm,
+                        const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
+                        float *grho);
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *nl, const int *nbcnt, Grid g, PhysF32 P, float2 *gX);
+                        const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
+                        float2 *gX);
 
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);

@@ -12,14 +12,16 @@
 //     J in FP64;
 //   * square roots use v_sqrt_f32 / v_rsq_f32 (1 ulp) instead of the correctly
 //     rounded expansions, and 1/r is an rsq instead of a division.
-// Each rebuild builds a transposed neighbour list (entry m of slot s at
-// nl[m*N + s]) of the 3x3 cell block around the particle's own cell, in the
-// reference's order (ddy outer, ddx inner, ascending id inside a cell); an
-// entry packs the neighbour slot with the code of its cell in the block,
-// (j << 4) | code, from which the exact FP32 shift between the two cell
-// frames follows.  Every sum has a fixed order: repeated runs are
-// bit-identical.  Physics follows src/forward.jl:25-147 and
-// src/adjoint.jl:30-228.
+// Each rebuild builds a bitmask neighbour list: for each slot and each of the
+// three cell rows of its 3x3 block, the row's cells form one contiguous slot
+// run [rb, rb+len), and bit b of the row's mask flags slot rb+b as a list
+// neighbour (r2min < r2 < rc2).  Walking the set bits in ascending order
+// visits the reference's order (ddy outer, ddx inner, ascending id inside a
+// cell).  The cell of bit b inside the row, and with it the exact FP32 shift
+// between the two cell frames, follows from the run's cell boundaries.  A
+// mask costs 8 bytes per row instead of 4 bytes per entry (~24 entries).
+// Every sum has a fixed order: repeated runs are bit-identical.  Physics
+// follows src/forward.jl:25-147 and src/adjoint.jl:30-228.
 #include "sph_f32c.h"
 
 namespace sph {
@@ -94,103 +96,164 @@ __device__ __forceinline__ float interp_alpha(const double *__restrict__ th, con
 }
 
 // origin of 0-based cell (cx, cy): ((cx-1) cs32, (cy-1) cs32) in FP64
+__device__ __forceinline__ int2 cell_xy(int c, const Grid &g);
 __device__ __forceinline__ float2 rel_coord(double2 x, int cell, const Grid &g, const PhysF32 &P) {
-  int cy = cell / g.nx;
-  int cx = cell - cy * g.nx;
+  const int2 c = cell_xy(cell, g);
+  const int cx = c.x, cy = c.y;
   double ox = (double)(cx - 1) * (double)P.cs;
   double oy = (double)(cy - 1) * (double)P.cs;
   return make_float2((float)(x.x - ox), (float)(x.y - oy));
 }
 
-// shift that moves an xr of the cell with code (dy+1)*3 + (dx+1), where
-// (dx, dy) is its offset from the sweeping particle's cell, into the frame of
-// that particle (exact multiples of cs in FP32)
-__device__ __forceinline__ float2 shift_of(int code, float cs) {
-  const int row = (code * 11) >> 5;            // code / 3 for code < 9
-  const int col = code - row * 3;
-  return make_float2((float)(col - 1) * cs, (float)(row - 1) * cs);
+// ---- raw buffer access -------------------------------------------------------
+// Neighbour-side reads go through buffer loads with a 32-bit byte offset: one
+// offset serves every per-slot array of the pair, the 128-bit descriptor lives
+// in scalar registers, and a read past the array returns 0 instead of
+// faulting.  Descriptor word 3 for gfx9 raw buffers: 0x00020000.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float2 ld2(Rsrc r, uint32_t off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float ld1(Rsrc r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
 
-struct Pair32 {
-  float dx, dy, r2;
-  int j;
+// (cx, cy) of a 0-based cell; the quotient estimate in FP64 is exact to one
+// unit for cells < 2^30 and is corrected once
+__device__ __forceinline__ int2 cell_xy(int c, const Grid &g) {
+  int cy = (int)((double)c * g.inv_nx);
+  int cx = c - cy * g.nx;
+  if (cx < 0) { cy -= 1; cx += g.nx; }
+  if (cx >= g.nx) { cy += 1; cx -= g.nx; }
+  return make_int2(cx, cy);
+}
+
+// One row of the 3x3 sweep around cell (cx, row jy): the slot run of cells
+// max(cx-1,0) .. min(cx+1,nx-1), and the offsets in it where cells cx and
+// cx+1 begin (bits below s0 lie in cell cx-1, bits from s1 on in cell cx+1).
+struct RowRun {
+  int rb, len, s0, s1;
 };
-// pair of the sweeping particle (own frame xi) with list entry e
-__device__ __forceinline__ Pair32 pair32(float2 xi, const float2 *__restrict__ xr, int e,
-                                         float cs) {
-  const int j = e >> 4;
-  const float2 sh = shift_of(e & 15, cs);
-  const float2 xj = xr[j];
-  Pair32 p;
-  p.dx = (xi.x - sh.x) - xj.x;
-  p.dy = (xi.y - sh.y) - xj.y;
-  p.r2 = p.dx * p.dx + p.dy * p.dy;
-  p.j = j;
-  return p;
+__device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &g) {
+  const int base = jy * g.nx;
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
+  RowRun r;
+  r.rb = (int)ldu(cst, (uint32_t)(base + x0) * 4u);
+  r.len = (int)ldu(cst, (uint32_t)(base + x1 + 1) * 4u) - r.rb;
+  r.s0 = (int)ldu(cst, (uint32_t)(base + cx) * 4u) - r.rb;
+  r.s1 = (int)ldu(cst, (uint32_t)(base + cx + 1) * 4u) - r.rb;
+  return r;
 }
 
-// neighbour list of the 3x3 block around the binning cell, cutoff rc; N < 2^27
-__global__ void __launch_bounds__(kBlock)
-k_nl_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-             const int *__restrict__ cstart, Grid g, PhysF32 P, float rc2, int maxnb,
-             int *__restrict__ nl, int *__restrict__ nbcnt, int *__restrict__ flags) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
-  const float2 xi = xr[s];
-  const int bc = bcell[s];
-  const int by = bc / g.nx;
-  const int bx = bc - by * g.nx;
-  int m = 0;
-  bool over = false;
+// Bitmask list: word w (32 bits) of row r of slot s sits at
+// nm[(w*3 + r)*N + s]; words beyond ceil(len/32) are neither written nor read.
+// Runs longer than kMaskRun slots are cut there and raise the overflow flag.
+constexpr int kMaskRun = 32 * kMaskWords;
+
+// Calls fn(j, o8, dx, dy, r2) for every list neighbour j (byte offset o8 = 8j
+// into the 8-byte per-slot arrays) of slot s, whose own-frame position is xi
+// and binning cell bc, in the reference's sweep order.
+template <class Fn>
+__device__ __forceinline__ void for_each_nb(Rsrc nmr, Rsrc cst, Rsrc xrr, int s, int bc,
+                                            float2 xi, const Grid &g, float cs, Fn &&fn) {
+  const int2 c = cell_xy(bc, g);
+  const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
-  for (int dyi = -1; dyi <= 1; ++dyi) {
-    const int jy = by + dyi;
+  for (int r = 0; r < 3; ++r) {
+    const int jy = c.y + r - 1;
     if (jy < 0 || jy >= g.ny) continue;
-    const int base = jy * g.nx;
-#pragma unroll
-    for (int dxi = -1; dxi <= 1; ++dxi) {
-      const int jx = bx + dxi;
-      if (jx < 0 || jx >= g.nx) continue;
-      const int code = (dyi + 1) * 3 + (dxi + 1);
-      const float2 sh = shift_of(code, P.cs);
-      const float xs = xi.x - sh.x, ys = xi.y - sh.y;
-      const int e = cstart[base + jx + 1];
-      for (int j = cstart[base + jx]; j < e; ++j) {
-        const float2 xj = xr[j];
-        const float dx = xs - xj.x, dy = ys - xj.y;
-        const float r2 = dx * dx + dy * dy;
-        if (r2 < rc2 && r2 > P.r2min) {
-          if (m < maxnb) {
-            nl[(size_t)m * g.N + s] = (j << 4) | code;
-            ++m;
-          } else {
-            over = true;
-          }
-        }
+    const RowRun run = row_run(cst, c.x, jy, g);
+    const float yi = xi.y - (float)(r - 1) * cs;
+    const int nw = (run.len + 31) >> 5;
+    const uint32_t rb8 = (uint32_t)run.rb * 8u;
+    for (int w = 0; w < nw && w < kMaskWords; ++w) {
+      uint32_t m = ldu(nmr, ((uint32_t)(w * 3 + r) * (uint32_t)g.N + (uint32_t)s) * 4u);
+      while (m) {
+        const int b = __builtin_ctz(m) + 32 * w;
+        m &= m - 1;
+        const uint32_t o8 = rb8 + (uint32_t)b * 8u;
+        const float2 xj = ld2(xrr, o8);
+        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+        const float dx = xs - xj.x;
+        const float dy = yi - xj.y;
+        fn(run.rb + b, o8, dx, dy, dx * dx + dy * dy);
       }
     }
   }
-  nbcnt[s] = m;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
+             const int *__restrict__ cstart, Grid g, PhysF32 P, float rc2,
+             uint32_t *__restrict__ nm, int *__restrict__ flags) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= g.N) return;
+  const Rsrc xrr = rsrc(xr, (uint32_t)g.N * 8u);
+  const Rsrc cst = rsrc(cstart, (uint32_t)(g.ncell + 1) * 4u);
+  const float2 xi = xr[s];
+  const int2 c = cell_xy(bcell[s], g);
+  const float xm = xi.x + P.cs, xp = xi.x - P.cs;
+  bool over = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int jy = c.y + r - 1;
+    if (jy < 0 || jy >= g.ny) continue;
+    const RowRun run = row_run(cst, c.x, jy, g);
+    const float yi = xi.y - (float)(r - 1) * P.cs;
+    int len = run.len;
+    if (len > kMaskRun) {
+      over = true;
+      len = kMaskRun;
+    }
+    const uint32_t rb8 = (uint32_t)run.rb * 8u;
+    for (int w = 0; w * 32 < len; ++w) {
+      uint32_t m = 0;
+      const int e = len - 32 * w < 32 ? len - 32 * w : 32;
+      for (int k = 0; k < e; ++k) {
+        const int b = 32 * w + k;
+        const float2 xj = ld2(xrr, rb8 + (uint32_t)b * 8u);
+        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+        const float dx = xs - xj.x;
+        const float dy = yi - xj.y;
+        const float r2 = dx * dx + dy * dy;
+        if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
+      }
+      nm[(size_t)(w * 3 + r) * g.N + s] = m;
+    }
+  }
   if (over) flags[0] = 1;
 }
 
-__device__ __forceinline__ float2 density_pair(float2 xi, const float2 *__restrict__ xr,
-                                               const NbRow &row, const PhysF32 &P) {
+__device__ __forceinline__ float2 density_pair(Rsrc nmr, Rsrc cst, Rsrc xrr, int s, int bc,
+                                               float2 xi, const Grid &g, const PhysF32 &P) {
   CSum acc;
-  for (int m = 0; m < row.n; ++m) {
-    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
-    if (pr.r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+  for_each_nb(nmr, cst, xrr, s, bc, xi, g, P.cs, [&](int, uint32_t, float, float, float r2) {
+    if (r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
       float u = 1.f - 0.5f * q;
       float u2 = u * u;
       acc.add(P.mA * (u2 * u2 * (2.f * q + 1.f)));
     }
-  }
+  });
   acc.add(P.mA);                  // self term
   // rho - rho0 from the compensated pair (exact subtraction, then the tail)
   float drho = (acc.s - P.rho0) + acc.c;
   return make_float2(P.rho0 + drho, drho);
 }
+
+// descriptors of the list and the per-slot arrays of one state
+#define NB_RSRC                                                             \
+  const Rsrc nmr = rsrc(nm, (uint32_t)g.N * (uint32_t)(3 * kMaskWords) * 4u); \
+  const Rsrc cst = rsrc(cstart, (uint32_t)(g.ncell + 1) * 4u);               \
+  const Rsrc xrr = rsrc(xr, (uint32_t)g.N * 8u)
 
 __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
   float inv = 1.f / rd.x;
@@ -198,12 +261,13 @@ __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_density32(const float2 *__restrict__ xr, const int *__restrict__ nl,
-            const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ PI,
-            double *__restrict__ rho_out) {
+k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
+            const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+            float2 *__restrict__ PI, double *__restrict__ rho_out) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
-  float2 rd = density_pair(xr[s], xr, nb_row(nl, nbcnt, s, g.N), P);
+  NB_RSRC;
+  float2 rd = density_pair(nmr, cst, xrr, s, bcell[s], xr[s], g, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -212,7 +276,7 @@ __global__ void __launch_bounds__(kBlock)
 k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ V, const float2 *__restrict__ PI,
           const double *__restrict__ theta, const int *__restrict__ bcell,
-          const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF32 P,
+          const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
           double2 *__restrict__ Xn, float2 *__restrict__ xrn, float2 *__restrict__ Vn,
           double2 *__restrict__ tapeX, float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
           double *__restrict__ alpha_out) {
@@ -220,22 +284,25 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   if (s >= g.N) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  const int bc = bcell[s];
+  NB_RSRC;
+  const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
   CSum ax, ay;
-  for (int m = 0; m < row.n; ++m) {
-    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
-    if (pr.r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+  for_each_nb(nmr, cst, xrr, s, bc, xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
+                                                      float r2) {
+    if (r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
       float u = 1.f - 0.5f * q;
       float F = P.Fc * u * u * u;
-      float2 pij = PI[pr.j];
+      float2 pij = ld2(pir, o8);
       float PF = P.m * (pii.x + pij.x) * F;
       float C = P.twomumass * pii.y * pij.y * F;
-      float2 vj = V[pr.j];
-      ax.add(C * (vi.x - vj.x) - PF * pr.dx);
-      ay.add(C * (vi.y - vj.y) - PF * pr.dy);
+      float2 vj = ld2(vr, o8);
+      ax.add(C * (vi.x - vj.x) - PF * dx);
+      ay.add(C * (vi.y - vj.y) - PF * dy);
     }
-  }
+  });
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
   float axv = ax.value() + (P.gx + wall_accel((float)x64.x, P.Lx, P.kw));
@@ -251,7 +318,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   Vn[s] = make_float2(v1, v2);
   double2 xn = make_double2(x64.x + P.dt64 * (double)v1, x64.y + P.dt64 * (double)v2);
   Xn[s] = xn;
-  xrn[s] = rel_coord(xn, bcell[s], g, P);
+  xrn[s] = rel_coord(xn, bc, g, P);
   if (tapeX) {
     tapeX[s] = x64;
     tapeV[s] = vi;
@@ -291,13 +358,15 @@ __global__ void k_permute32(const int *__restrict__ perm, const int *__restrict_
 __global__ void __launch_bounds__(kBlock)
 k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ Vnext, const double *__restrict__ theta,
-          const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF32 P,
-          float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
-          float2 *__restrict__ abar, float *__restrict__ galpha) {
+          const int *__restrict__ bcell, const uint32_t *__restrict__ nm,
+          const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ PI,
+          float2 *__restrict__ gX, float2 *__restrict__ gV, float2 *__restrict__ abar,
+          float *__restrict__ galpha) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
-  float2 rd = density_pair(xr[s], xr, nb_row(nl, nbcnt, s, g.N), P);
+  NB_RSRC;
+  float2 rd = density_pair(nmr, cst, xrr, s, bcell[s], xr[s], g, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
@@ -339,32 +408,33 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const float2 *__restrict__ V, const float2 *__restrict__ PI,
            const float2 *__restrict__ abar, const float *__restrict__ galpha,
-           const double *__restrict__ theta, const int *__restrict__ nl,
-           const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ gXs,
-           float2 *__restrict__ gV, float *__restrict__ grho) {
+           const double *__restrict__ theta, const int *__restrict__ bcell,
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+           float2 *__restrict__ gXs, float2 *__restrict__ gV, float *__restrict__ grho) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const float pti = pii.x, iri = pii.y;
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  NB_RSRC;
+  const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
+  const Rsrc abr = rsrc(abar, (uint32_t)g.N * 8u);
   CSum gx, gy, gvx, gvy, grh, sumS;
-  for (int m = 0; m < row.n; ++m) {
-    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
-    if (pr.r2 < P.r2max) {
-      const int j = pr.j;
-      const float dx = pr.dx, dy = pr.dy;
-      float rinv = __builtin_amdgcn_rsqf(pr.r2);
-      float q = (pr.r2 * rinv) * P.invh;
+  for_each_nb(nmr, cst, xrr, s, bcell[s], xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
+                                                            float r2) {
+    if (r2 < P.r2max) {
+      float rinv = __builtin_amdgcn_rsqf(r2);
+      float q = (r2 * rinv) * P.invh;
       float u = 1.f - 0.5f * q;
       float F = P.Fc * u * u * u;
       float G = P.Gc * u * u;
-      float2 pij = PI[j];
+      float2 pij = ld2(pir, o8);
       float Pij = P.m * (pti + pij.x);
       float Cbase = P.twomumass * iri * pij.y;
       float C = Cbase * F;
-      float2 abj = abar[j];
-      float2 vj = V[j];
+      float2 abj = ld2(abr, o8);
+      float2 vj = ld2(vr, o8);
       float dax = ab.x - abj.x;
       float day = ab.y - abj.y;
       float dvx = vi.x - vj.x;
@@ -381,7 +451,7 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
       grh.add(-(C * ad_v * iri));
       sumS.add(F * ad_d);
     }
-  }
+  });
   float S = sumS.value();
   float grv = grh.value() + (-2.f * P.m * pti * iri) * S + (P.c2 * P.m * iri * iri) * S;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
@@ -401,24 +471,26 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 // adjoint P2: adj_pass2_kernel! and gX += gXs (src/adjoint.jl:126-156,187-191)
 __global__ void __launch_bounds__(kBlock)
 k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
-           const float2 *__restrict__ gXs, const int *__restrict__ nl,
-           const int *__restrict__ nbcnt, Grid g, PhysF32 P, float2 *__restrict__ gX) {
+           const float2 *__restrict__ gXs, const int *__restrict__ bcell,
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+           float2 *__restrict__ gX) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const float2 xi = xr[s];
   const float gri = grho[s];
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  NB_RSRC;
+  const Rsrc grr = rsrc(grho, (uint32_t)g.N * 4u);
   CSum gx, gy;
-  for (int m = 0; m < row.n; ++m) {
-    const Pair32 pr = pair32(xi, xr, row.at(m), P.cs);
-    if (pr.r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(pr.r2) * P.invh;
+  for_each_nb(nmr, cst, xrr, s, bcell[s], xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
+                                                            float r2) {
+    if (r2 < P.r2max) {
+      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
       float u = 1.f - 0.5f * q;
-      float wgt = P.m * (P.Fc * u * u * u) * (gri + grho[pr.j]);
-      gx.add(wgt * pr.dx);
-      gy.add(wgt * pr.dy);
+      float wgt = P.m * (P.Fc * u * u * u) * (gri + ld1(grr, o8 >> 1));
+      gx.add(wgt * dx);
+      gy.add(wgt * dy);
     }
-  }
+  });
   float2 gs = gXs[s];
   float2 go = gX[s];
   gX[s] = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
@@ -426,24 +498,22 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
 
 }  // namespace
 
-void launch_nl_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          Grid g, PhysF32 P, float rc2, int maxnb, int *nl, int *nbcnt,
-                          int *flags) {
+void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
+                          Grid g, PhysF32 P, float rc2, uint32_t *nm, int *flags) {
   if (g.N > 0)
-    k_nl_build32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, rc2, maxnb, nl, nbcnt,
-                                                 flags);
+    k_nm_build32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, rc2, nm, flags);
 }
-void launch_density_f32c(hipStream_t s, const float2 *xr, const int *nl, const int *nbcnt, Grid g,
-                         PhysF32 P, float2 *PI, double *rho) {
-  if (g.N > 0) k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, nl, nbcnt, g, P, PI, rho);
+void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
+                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho) {
+  if (g.N > 0) k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, nm, cstart, g, P, PI, rho);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
-                       const float2 *PI, const double *theta, const int *bcell, const int *nl,
-                       const int *nbcnt, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
+                       const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
+                       const int *cstart, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
                        double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out) {
   if (g.N > 0)
-    k_force32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nl, nbcnt, g, P, Xn, xrn,
-                                              Vn, tapeX, tapeV, a_out, alpha_out);
+    k_force32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, g, P, Xn,
+                                              xrn, Vn, tapeX, tapeV, a_out, alpha_out);
 }
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
                       float2 *xr) {
@@ -458,11 +528,12 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                                                 Vn, xr, Xrefn, bcell, flags);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
-                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
-                       float2 *PI, float2 *gX, float2 *gV, float2 *abar, float *galpha) {
+                       const double *theta, const int *bcell, const uint32_t *nm,
+                       const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
+                       float2 *abar, float *galpha) {
   if (g.N > 0)
-    k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV,
-                                              abar, galpha);
+    k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, g, P, PI,
+                                              gX, gV, abar, galpha);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
@@ -473,15 +544,18 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
-                        const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF32 P,
-                        float2 *gXs, float2 *gV, float *grho) {
+                        const double *theta, const int *bcell, const uint32_t *nm,
+                        const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
+                        float *grho) {
   if (g.N > 0)
-    k_adj_p132<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, nl, nbcnt, g, P,
-                                               gXs, gV, grho);
+    k_adj_p132<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                               cstart, g, P, gXs, gV, grho);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *nl, const int *nbcnt, Grid g, PhysF32 P, float2 *gX) {
-  if (g.N > 0) k_adj_p232<<<nblocks(g.N), kBlock, 0, s>>>(xr, grho, gXs, nl, nbcnt, g, P, gX);
+                        const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
+                        float2 *gX) {
+  if (g.N > 0)
+    k_adj_p232<<<nblocks(g.N), kBlock, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, g, P, gX);
 }
 
 }  // namespace sph

@@ -17,9 +17,10 @@ namespace sph {
 
 constexpr int kBlock = 256;
 
-// Cell grid of a state (0-based cells c = cx + cy*nx).
+// Cell grid of a state (0-based cells c = cx + cy*nx); inv_nx = 1/nx in FP64.
 struct Grid {
   int nx, ny, ncell, N;
+  double inv_nx;
 };
 
 // Constants of the FP64 kernels.  Every value is formed on the host with the
