@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsphadj.so")
+# SPHADJ_LIB names another build of the same library (e.g. a variant under test)
+LIB_PATH = os.environ.get("SPHADJ_LIB") or os.path.join(_HERE, "libsphadj.so")
 
 SPH_OK, SPH_E_ARG, SPH_E_CAPACITY, SPH_E_HIP, SPH_E_STATE, SPH_E_NOMEM = range(6)
 SPH_F64, SPH_F32C = 0, 1

@@ -291,6 +291,7 @@ struct sph_state {
   // slot s at nl[m*N + s]); f32c form row bitmasks (nm)
   int *nl = nullptr, *nbcnt = nullptr;
   uint32_t *nm = nullptr;
+  uint32_t *em = nullptr;  // f32c form: exact list of the last density pass
   double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
   double2 *Xref = nullptr, *Xref2 = nullptr;
   double2 *PI = nullptr;
@@ -356,6 +357,7 @@ struct sph_workspace {
   double4 *tpart = nullptr;  // per theta cell partial sums of the gtheta gather
   int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
   uint32_t *nm = nullptr;                                  // f32c form: row bitmasks
+  uint32_t *em = nullptr;                                  // f32c form: exact list of a step
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
   int *iota = nullptr;     // particle-id order as a slot order
@@ -378,6 +380,7 @@ void state_free(sph_state *st) {
   float2 **f2[] = {&st->V32, &st->V32n, &st->xr, &st->xrn, &st->PI32};
   for (auto pp : f2) dfree(*pp);
   dfree(st->nm);
+  dfree(st->em);
   double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
                    &st->tmp1, &st->obj_part, &st->obj_J};
   for (auto pp : d1) dfree(*pp);
@@ -413,8 +416,8 @@ void materialise_aux(sph_state *st) {
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
     launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->g, st->P32, st->PI32,
-                        st->rho_slot);
-    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->nm,
+                        st->rho_slot, st->em);
+    launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->em,
                       st->cstart, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
                       st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
@@ -528,10 +531,11 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->g, st->P32, PI32, nullptr);
+      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->g, st->P32, PI32, nullptr,
+                          st->em);
     });
     timed(st->ctx, K_FORCE, [&] {
-      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->nm, st->cstart,
+      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->em, st->cstart,
                         st->g, st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
@@ -815,6 +819,7 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       }
       if (f32c) {
         st->nm = dalloc<uint32_t>(n * 3 * kMaskWords);
+        st->em = dalloc<uint32_t>(n * 3 * kMaskWords);
       } else {
         st->nl = dalloc<int>(n * (size_t)maxnb);
         st->nbcnt = dalloc<int>(n);
@@ -1181,6 +1186,7 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->galpha32); dfree(ws->grho32); dfree(ws->tpart);
   dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags); dfree(ws->ids_cur); dfree(ws->iota);
   dfree(ws->nm);
+  dfree(ws->em);
 }
 
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
@@ -1294,6 +1300,7 @@ namespace {
 void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
   if (ws->precision == SPH_F32C) {
     if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
+    if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     return;
   }
   if (ws->nl_cap < maxnb) {
@@ -1389,18 +1396,25 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         } else {
           launch_xrel_f32c(s, Xk, E.scell, g, P32, ws->xr32);
           launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, g, P32,
-                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32);
+                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em);
         }
       });
       timed(c, K_THETA, [&] {
         launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
       });
       timed(c, K_ADJ_P1, [&] {
-        launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                           nm, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32);
+        // the density recompute wrote the exact list of these positions; with
+        // stored (pterm, invrho) P1 tests the pairs and writes it for P2
+        if (tape->keep_pi)
+          launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
+                             nm, false, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32, ws->em);
+        else
+          launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
+                             ws->em, true, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32,
+                             nullptr);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, nm, E.cstart, g, P32,
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, g, P32,
                            ws->gX32);
       });
     } else {

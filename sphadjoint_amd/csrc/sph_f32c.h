@@ -19,8 +19,12 @@ constexpr int kMaskWords = 4;
 
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           Grid g, PhysF32 P, float rc2, uint32_t *nm, int *flags);
+// density + EOS over the mask list nm; when em is given, the exact list of
+// these positions (pairs with r2 < (2h)^2) is written there for the force pass
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho);
+                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho,
+                         uint32_t *em);
+// force + integrator over the exact list nm of the density pass
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
@@ -34,15 +38,18 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha);
+                       float2 *abar, float *galpha, uint32_t *em);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha);
+// P1 over nm: the exact list when `exact`, else the mask list, whose exact
+// subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
-                        const double *theta, const int *bcell, const uint32_t *nm,
+                        const double *theta, const int *bcell, const uint32_t *nm, bool exact,
                         const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
-                        float *grho);
+                        float *grho, uint32_t *em);
+// P2 over the exact list nm
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
                         float2 *gX);

@@ -24,6 +24,12 @@
 // follows src/forward.jl:25-147 and src/adjoint.jl:30-228.
 #include "sph_f32c.h"
 
+// No implicit contraction: every FMA below is written out, so the rounding of
+// each expression is fixed by the source and identical in every kernel that
+// evaluates it (the checkpointed and full-tape reverse sweeps must agree bit
+// for bit), and the compensated sums see the roundings they assume.
+#pragma clang fp contract(off)
+
 namespace sph {
 
 namespace {
@@ -43,6 +49,33 @@ struct CSum {
   }
   __device__ __forceinline__ float value() const { return s + c; }
 };
+
+// Fast2Sum accumulator: the error term is exact when |s| >= |t| for every
+// added t (density terms are positive and none exceeds the self term, which
+// is added first).
+struct FSum {
+  float s, c;
+  __device__ __forceinline__ explicit FSum(float s0) : s(s0), c(0.f) {}
+  __device__ __forceinline__ void add(float t) {
+    float n = s + t;
+    c += t - (n - s);
+    s = n;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// plain FP32 accumulator
+struct PSum {
+  float s;
+  __device__ __forceinline__ PSum() : s(0.f) {}
+  __device__ __forceinline__ void add(float t) { s += t; }
+  __device__ __forceinline__ float value() const { return s; }
+};
+
+// Adjoint pair sums (P1, P2) accumulate in plain FP32: their rounding error
+// is far below the error the FP32 state itself carries into the gradients
+// (measured against FP64 with and without TwoSum: the same to two digits).
+typedef PSum ASum;
 
 __device__ __forceinline__ float wall_accel(float x, float L, float kw) {
   float lo = fmaxf(0.f, -x);
@@ -158,12 +191,31 @@ __device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &
 // Runs longer than kMaskRun slots are cut there and raise the overflow flag.
 constexpr int kMaskRun = 32 * kMaskWords;
 
-// Calls fn(j, o8, dx, dy, r2) for every list neighbour j (byte offset o8 = 8j
-// into the 8-byte per-slot arrays) of slot s, whose own-frame position is xi
-// and binning cell bc, in the reference's sweep order.
-template <class Fn>
-__device__ __forceinline__ void for_each_nb(Rsrc nmr, Rsrc cst, Rsrc xrr, int s, int bc,
-                                            float2 xi, const Grid &g, float cs, Fn &&fn) {
+// Calls fn(d, dx, dy, r2) for every neighbour flagged in the mask list m_in
+// of slot s (own-frame position xi, binning cell bc), in the reference's sweep
+// order, where d = ld(o8) holds the neighbour's per-slot data, o8 = 8*j being
+// its byte offset into the 8-byte per-slot arrays (d.x must be its xr).  fn
+// returns whether the pair is a true neighbour (r2 < (2h)^2); when em is given,
+// those pairs form a mask list of the same layout, written to em, that later
+// passes over the same positions iterate without testing r2.  The loads of the
+// next neighbour are issued before the current pair is evaluated (two
+// registers sets in turn), so a word exposes one memory round trip, not one per
+// pair.
+// one pair of for_each_nb: bit b of a word whose cells cx-1 / cx / cx+1 start
+// at bits <0 / s0 / s1; xm, x0, xp are the own x in those cells' frames
+template <class D, class Fn>
+__device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float xm, float x0,
+                                        float xp, float yi, uint32_t &hit, Fn &fn) {
+  const float xs = b < s0 ? xm : (b < s1 ? x0 : xp);
+  const float dx = xs - d.x.x;
+  const float dy = yi - d.x.y;
+  if (fn(d, dx, dy, fmaf(dx, dx, dy * dy))) hit |= 1u << b;
+}
+
+template <class Ld, class Fn>
+__device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
+                                            int bc, float2 xi, const Grid &g, float cs, Ld &&ld,
+                                            Fn &&fn) {
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
@@ -172,23 +224,52 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, Rsrc cst, Rsrc xrr, int s,
     if (jy < 0 || jy >= g.ny) continue;
     const RowRun run = row_run(cst, c.x, jy, g);
     const float yi = xi.y - (float)(r - 1) * cs;
-    const int nw = (run.len + 31) >> 5;
-    const uint32_t rb8 = (uint32_t)run.rb * 8u;
-    for (int w = 0; w < nw && w < kMaskWords; ++w) {
-      uint32_t m = ldu(nmr, ((uint32_t)(w * 3 + r) * (uint32_t)g.N + (uint32_t)s) * 4u);
-      while (m) {
-        const int b = __builtin_ctz(m) + 32 * w;
+    int nw = (run.len + 31) >> 5;
+    nw = nw < kMaskWords ? nw : kMaskWords;
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t widx = (uint32_t)(w * 3 + r) * (uint32_t)g.N + (uint32_t)s;
+      uint32_t m = ldu(nmr, widx * 4u);
+      uint32_t hit = 0;
+      if (m != 0) {
+        const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
+        const uint32_t wb8 = (uint32_t)(run.rb + 32 * w) * 8u;
+        int b0 = __builtin_ctz(m);
         m &= m - 1;
-        const uint32_t o8 = rb8 + (uint32_t)b * 8u;
-        const float2 xj = ld2(xrr, o8);
-        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
-        const float dx = xs - xj.x;
-        const float dy = yi - xj.y;
-        fn(run.rb + b, o8, dx, dy, dx * dx + dy * dy);
+        auto d0 = ld(wb8 + (uint32_t)b0 * 8u);
+        for (;;) {
+          const bool more1 = m != 0;
+          const int b1 = more1 ? __builtin_ctz(m) : b0;
+          m &= m - 1;
+          auto d1 = ld(wb8 + (uint32_t)b1 * 8u);
+          nb_pair(d0, b0, s0, s1, xm, xi.x, xp, yi, hit, fn);
+          if (!more1) break;
+          const bool more0 = m != 0;
+          b0 = more0 ? __builtin_ctz(m) : b1;
+          m &= m - 1;
+          d0 = ld(wb8 + (uint32_t)b0 * 8u);
+          nb_pair(d1, b1, s0, s1, xm, xi.x, xp, yi, hit, fn);
+          if (!more0) break;
+        }
       }
+      if (em) em[widx] = hit;
     }
   }
 }
+
+// neighbour data of the pair passes
+struct NbX {
+  float2 x;
+};
+struct NbXPV {
+  float2 x, pi, v;
+};
+struct NbXPAV {
+  float2 x, pi, ab, v;
+};
+struct NbXG {
+  float2 x;
+  float gr;
+};
 
 __global__ void __launch_bounds__(kBlock)
 k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
@@ -223,7 +304,7 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
         const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
         const float dx = xs - xj.x;
         const float dy = yi - xj.y;
-        const float r2 = dx * dx + dy * dy;
+        const float r2 = fmaf(dx, dx, dy * dy);
         if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
       }
       nm[(size_t)(w * 3 + r) * g.N + s] = m;
@@ -232,18 +313,23 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   if (over) flags[0] = 1;
 }
 
-__device__ __forceinline__ float2 density_pair(Rsrc nmr, Rsrc cst, Rsrc xrr, int s, int bc,
-                                               float2 xi, const Grid &g, const PhysF32 &P) {
-  CSum acc;
-  for_each_nb(nmr, cst, xrr, s, bc, xi, g, P.cs, [&](int, uint32_t, float, float, float r2) {
+__device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, Rsrc xrr, int s,
+                                               int bc, float2 xi, const Grid &g,
+                                               const PhysF32 &P) {
+  // Fast2Sum from the self term, the largest term of the sum: every later
+  // term is positive and smaller, so each step's error term is exact
+  FSum acc(P.mA);
+  for_each_nb(nmr, em, cst, s, bc, xi, g, P.cs, [&](uint32_t o8) __attribute__((always_inline)) { return NbX{ld2(xrr, o8)}; },
+              [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-      float u = 1.f - 0.5f * q;
+      float u = fmaf(-0.5f, q, 1.f);
       float u2 = u * u;
-      acc.add(P.mA * (u2 * u2 * (2.f * q + 1.f)));
+      acc.add(P.mA * (u2 * u2 * fmaf(2.f, q, 1.f)));
+      return true;
     }
+    return false;
   });
-  acc.add(P.mA);                  // self term
   // rho - rho0 from the compensated pair (exact subtraction, then the tail)
   float drho = (acc.s - P.rho0) + acc.c;
   return make_float2(P.rho0 + drho, drho);
@@ -263,11 +349,11 @@ __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
 __global__ void __launch_bounds__(kBlock)
 k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
             const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
-            float2 *__restrict__ PI, double *__restrict__ rho_out) {
+            float2 *__restrict__ PI, double *__restrict__ rho_out, uint32_t *__restrict__ em) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   NB_RSRC;
-  float2 rd = density_pair(nmr, cst, xrr, s, bcell[s], xr[s], g, P);
+  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -285,23 +371,25 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
   const int bc = bcell[s];
+  const float tmi = P.twomumass * pii.y;
   NB_RSRC;
   const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
   CSum ax, ay;
-  for_each_nb(nmr, cst, xrr, s, bc, xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
-                                                      float r2) {
-    if (r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-      float u = 1.f - 0.5f * q;
-      float F = P.Fc * u * u * u;
-      float2 pij = ld2(pir, o8);
-      float PF = P.m * (pii.x + pij.x) * F;
-      float C = P.twomumass * pii.y * pij.y * F;
-      float2 vj = ld2(vr, o8);
-      ax.add(C * (vi.x - vj.x) - PF * dx);
-      ay.add(C * (vi.y - vj.y) - PF * dy);
-    }
+  // nm is the exact list of the density pass over these positions
+  for_each_nb(nmr, nullptr, cst, s, bc, xi, g, P.cs,
+              [&](uint32_t o8) __attribute__((always_inline)) { return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)}; },
+              [&, P](const NbXPV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+    float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
+    float u = fmaf(-0.5f, q, 1.f);
+    float F = P.Fc * u * u * u;
+    const float2 pij = d.pi;
+    float PF = P.m * (pii.x + pij.x) * F;
+    float C = tmi * pij.y * F;
+    const float2 vj = d.v;
+    ax.add(fmaf(C, vi.x - vj.x, -(PF * dx)));
+    ay.add(fmaf(C, vi.y - vj.y, -(PF * dy)));
+    return true;
   });
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
@@ -361,12 +449,12 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const int *__restrict__ bcell, const uint32_t *__restrict__ nm,
           const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ PI,
           float2 *__restrict__ gX, float2 *__restrict__ gV, float2 *__restrict__ abar,
-          float *__restrict__ galpha) {
+          float *__restrict__ galpha, uint32_t *__restrict__ em) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
   NB_RSRC;
-  float2 rd = density_pair(nmr, cst, xrr, s, bcell[s], xr[s], g, P);
+  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
@@ -403,14 +491,18 @@ k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
   gV[s] = make_float2(D * gvx, D * gvy);
 }
 
-// adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185)
+// adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185).
+// kExact: nm is already the exact list of these positions; otherwise pairs are
+// tested and the exact list is written to em for P2.
+template <bool kExact>
 __global__ void __launch_bounds__(kBlock)
 k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const float2 *__restrict__ V, const float2 *__restrict__ PI,
            const float2 *__restrict__ abar, const float *__restrict__ galpha,
            const double *__restrict__ theta, const int *__restrict__ bcell,
            const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
-           float2 *__restrict__ gXs, float2 *__restrict__ gV, float *__restrict__ grho) {
+           float2 *__restrict__ gXs, float2 *__restrict__ gV, float *__restrict__ grho,
+           uint32_t *__restrict__ em) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
@@ -420,37 +512,42 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
   const Rsrc abr = rsrc(abar, (uint32_t)g.N * 8u);
-  CSum gx, gy, gvx, gvy, grh, sumS;
-  for_each_nb(nmr, cst, xrr, s, bcell[s], xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
-                                                            float r2) {
-    if (r2 < P.r2max) {
+  ASum gx, gy, gvx, gvy, grh, sumS;
+  for_each_nb(nmr, kExact ? nullptr : em, cst, s, bcell[s], xi, g, P.cs,
+              [&](uint32_t o8) __attribute__((always_inline)) {
+                return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
+              },
+              [&, P](const NbXPAV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+    if (kExact || r2 < P.r2max) {
       float rinv = __builtin_amdgcn_rsqf(r2);
       float q = (r2 * rinv) * P.invh;
-      float u = 1.f - 0.5f * q;
+      float u = fmaf(-0.5f, q, 1.f);
       float F = P.Fc * u * u * u;
       float G = P.Gc * u * u;
-      float2 pij = ld2(pir, o8);
+      const float2 pij = d.pi;
       float Pij = P.m * (pti + pij.x);
       float Cbase = P.twomumass * iri * pij.y;
       float C = Cbase * F;
-      float2 abj = ld2(abr, o8);
-      float2 vj = ld2(vr, o8);
+      const float2 abj = d.ab;
+      const float2 vj = d.v;
       float dax = ab.x - abj.x;
       float day = ab.y - abj.y;
       float dvx = vi.x - vj.x;
       float dvy = vi.y - vj.y;
-      float ad_d = -(dax * dx + day * dy);
-      float ad_v = dax * dvx + day * dvy;
-      float Fbar = Pij * ad_d + Cbase * ad_v;
+      float ad_d = -fmaf(dax, dx, day * dy);
+      float ad_v = fmaf(dax, dvx, day * dvy);
+      float Fbar = fmaf(Pij, ad_d, Cbase * ad_v);
       float wgt = Fbar * G * P.invh * rinv;
       float PF = Pij * F;
-      gx.add(wgt * dx - PF * dax);
-      gy.add(wgt * dy - PF * day);
+      gx.add(fmaf(wgt, dx, -(PF * dax)));
+      gy.add(fmaf(wgt, dy, -(PF * day)));
       gvx.add(C * dax);
       gvy.add(C * day);
       grh.add(-(C * ad_v * iri));
       sumS.add(F * ad_d);
+      return true;
     }
+    return false;
   });
   float S = sumS.value();
   float grv = grh.value() + (-2.f * P.m * pti * iri) * S + (P.c2 * P.m * iri * iri) * S;
@@ -480,16 +577,17 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   const float gri = grho[s];
   NB_RSRC;
   const Rsrc grr = rsrc(grho, (uint32_t)g.N * 4u);
-  CSum gx, gy;
-  for_each_nb(nmr, cst, xrr, s, bcell[s], xi, g, P.cs, [&](int, uint32_t o8, float dx, float dy,
-                                                            float r2) {
-    if (r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-      float u = 1.f - 0.5f * q;
-      float wgt = P.m * (P.Fc * u * u * u) * (gri + ld1(grr, o8 >> 1));
-      gx.add(wgt * dx);
-      gy.add(wgt * dy);
-    }
+  ASum gx, gy;
+  // nm is the exact list of P1 over these positions
+  for_each_nb(nmr, nullptr, cst, s, bcell[s], xi, g, P.cs,
+              [&](uint32_t o8) __attribute__((always_inline)) { return NbXG{ld2(xrr, o8), ld1(grr, o8 >> 1)}; },
+              [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+    float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
+    float u = fmaf(-0.5f, q, 1.f);
+    float wgt = P.m * (P.Fc * u * u * u) * (gri + d.gr);
+    gx.add(wgt * dx);
+    gy.add(wgt * dy);
+    return true;
   });
   float2 gs = gXs[s];
   float2 go = gX[s];
@@ -504,8 +602,10 @@ void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, con
     k_nm_build32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, rc2, nm, flags);
 }
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho) {
-  if (g.N > 0) k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, nm, cstart, g, P, PI, rho);
+                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho,
+                         uint32_t *em) {
+  if (g.N > 0)
+    k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, nm, cstart, g, P, PI, rho, em);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
@@ -530,10 +630,10 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha) {
+                       float2 *abar, float *galpha, uint32_t *em) {
   if (g.N > 0)
     k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, g, P, PI,
-                                              gX, gV, abar, galpha);
+                                              gX, gV, abar, galpha, em);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
@@ -544,12 +644,16 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
-                        const double *theta, const int *bcell, const uint32_t *nm,
+                        const double *theta, const int *bcell, const uint32_t *nm, bool exact,
                         const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
-                        float *grho) {
-  if (g.N > 0)
-    k_adj_p132<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
-                                               cstart, g, P, gXs, gV, grho);
+                        float *grho, uint32_t *em) {
+  if (g.N <= 0) return;
+  if (exact)
+    k_adj_p132<true><<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                                     cstart, g, P, gXs, gV, grho, nullptr);
+  else
+    k_adj_p132<false><<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell,
+                                                      nm, cstart, g, P, gXs, gV, grho, em);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
