@@ -187,7 +187,9 @@ int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3);
 /* --- measurement: per-kernel GPU time with events on the context's stream.
  * No counterpart in the reference (its stage timing lives in
  * scripts/06_metal_profile.jl:158-290); used by bench.py for the roofline. */
-int sph_profile_enable(sph_ctx *ctx, int on);     /* also resets the counters */
+/* on = 0: off; 1: every kernel; 2 + k: kernel id k only (two events per
+ * launch of that kernel, none elsewhere).  Also resets the counters. */
+int sph_profile_enable(sph_ctx *ctx, int on);
 int sph_profile_get(sph_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
 const char *sph_profile_name(int kernel);          /* NULL past the last kernel id */
 

@@ -557,9 +557,27 @@ def target_objective(st: State, xc, yc, sigma, device=False):
     return J.value, np.ascontiguousarray(seed.T)
 
 
-def profile_enable(backend: MI355XBackend, on: bool = True):
-    """Per-kernel GPU timing with events on the backend's stream (resets counters)."""
-    L.call("sph_profile_enable", backend._h, 1 if on else 0)
+def profile_enable(backend: MI355XBackend, on: bool = True, only: str = None):
+    """Per-kernel GPU timing with events on the backend's stream (resets counters).
+    With ``only`` (a kernel name of profile_stats), that kernel alone is timed."""
+    mode = 1 if on else 0
+    if on and only is not None:
+        names = kernel_names()
+        if only not in names:
+            raise ValueError(f"unknown kernel {only!r}; known: {names}")
+        mode = 2 + names.index(only)
+    L.call("sph_profile_enable", backend._h, mode)
+
+
+def kernel_names():
+    lib = L.load()
+    out, k = [], 0
+    while True:
+        name = lib.sph_profile_name(k)
+        if name is None:
+            return out
+        out.append(name.decode())
+        k += 1
 
 
 def profile_stats(backend: MI355XBackend):

@@ -195,6 +195,7 @@ struct sph_ctx {
   int refs = 1;
   // optional per-kernel timing with events on this stream
   bool prof = false;
+  int prof_only = -1;  // >= 0: time only this kernel id
   std::vector<ProfRec> pending;
   std::vector<hipEvent_t> evpool;
   double tot_ms[K_COUNT] = {};
@@ -232,7 +233,7 @@ hipEvent_t prof_event(sph_ctx *c) {
 // with events and books the interval under kernel id kid.
 template <class F>
 void timed(sph_ctx *c, int kid, F &&f) {
-  if (!c->prof) {
+  if (!c->prof || (c->prof_only >= 0 && c->prof_only != kid)) {
     f();
     return;
   }
@@ -358,6 +359,8 @@ struct sph_workspace {
   int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
   uint32_t *nm = nullptr;                                  // f32c form: row bitmasks
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
+  int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
+  double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
   int *iota = nullptr;     // particle-id order as a slot order
@@ -613,7 +616,9 @@ int sph_profile_enable(sph_ctx *ctx, int on) {
     if (!ctx) fail(SPH_E_ARG, "ctx is NULL");
     HIPC(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    if (on < 0 || on >= 2 + K_COUNT) fail(SPH_E_ARG, "profile mode %d out of range", on);
     ctx->prof = on != 0;
+    ctx->prof_only = on >= 2 ? on - 2 : -1;
     for (int k = 0; k < K_COUNT; ++k) {
       ctx->tot_ms[k] = 0.0;
       ctx->launches[k] = 0;
@@ -1187,6 +1192,8 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags); dfree(ws->ids_cur); dfree(ws->iota);
   dfree(ws->nm);
   dfree(ws->em);
+  dfree(ws->tkey);
+  dfree(ws->tfxy);
 }
 
 int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
@@ -1301,6 +1308,8 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
   if (ws->precision == SPH_F32C) {
     if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
+    if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);
+    if (!ws->tfxy) ws->tfxy = dalloc<double2>((size_t)ws->N);
     return;
   }
   if (ws->nl_cap < maxnb) {
@@ -1392,15 +1401,17 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
           launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, g, P32, ws->xr32, ws->gX32,
-                              ws->gV32, ws->abar32, ws->galpha32);
+                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, g, P32, ws->xr32);
           launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, g, P32,
-                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em);
+                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em,
+                            ws->tkey, ws->tfxy);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_f32(s, Xk, ws->galpha32, E.cstart, g, T, ws->tpart, ws->gtheta);
+        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, g, T,
+                                      ws->tpart, ws->gtheta);
       });
       timed(c, K_ADJ_P1, [&] {
         // the density recompute wrote the exact list of these positions; with

@@ -35,13 +35,15 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          const float2 *V, const double2 *Xref, int check, Grid g, PhysF32 P,
                          double dmax2, double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn,
                          int *bcell, int *flags);
+// R passes also write each slot's interp-cell key and FP64 fractions for the
+// keyed gtheta gather
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha, uint32_t *em);
+                       float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha);
+                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,

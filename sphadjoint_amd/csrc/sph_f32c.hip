@@ -91,6 +91,7 @@ __device__ __forceinline__ float wall_daccel(float x, float L, float kw) {
 struct InterpF {
   int l00;
   float fx, fy, ax, ay;
+  double fx64, fy64;   // the fractions before rounding to FP32
 };
 
 // interp_cell (src/params.jl:53-68) from the FP64 position
@@ -107,8 +108,10 @@ __device__ __forceinline__ InterpF interp_cell(double x, double y, const PhysF32
   int j0 = (fyf > -1e9 && fyf < 1e9) ? (int)fyf : -1000000000;
   i0 = i0 < 0 ? 0 : (i0 > P.ngx - 2 ? P.ngx - 2 : i0);
   j0 = j0 < 0 ? 0 : (j0 > P.ngy - 2 ? P.ngy - 2 : j0);
-  r.fx = (float)(gxc - (double)i0);
-  r.fy = (float)(gyc - (double)j0);
+  r.fx64 = gxc - (double)i0;
+  r.fy64 = gyc - (double)j0;
+  r.fx = (float)r.fx64;
+  r.fy = (float)r.fy64;
   r.l00 = j0 + i0 * P.ngy;
   return r;
 }
@@ -449,7 +452,8 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const int *__restrict__ bcell, const uint32_t *__restrict__ nm,
           const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ PI,
           float2 *__restrict__ gX, float2 *__restrict__ gV, float2 *__restrict__ abar,
-          float *__restrict__ galpha, uint32_t *__restrict__ em) {
+          float *__restrict__ galpha, uint32_t *__restrict__ em, int *__restrict__ tkey,
+          double2 *__restrict__ tfxy) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
@@ -457,6 +461,8 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
+  tkey[s] = ic.l00 - ic.l00 / P.ngy;          // j0 + i0*(ngy-1)
+  tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
@@ -474,12 +480,15 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
             const float2 *__restrict__ Vnext, const double *__restrict__ theta, Grid g, PhysF32 P,
             float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
-            float2 *__restrict__ abar, float *__restrict__ galpha) {
+            float2 *__restrict__ abar, float *__restrict__ galpha, int *__restrict__ tkey,
+            double2 *__restrict__ tfxy) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   const double2 x64 = X[s];
   xr[s] = rel_coord(x64, bcell[s], g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
+  tkey[s] = ic.l00 - ic.l00 / P.ngy;          // j0 + i0*(ngy-1)
+  tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
@@ -630,17 +639,17 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha, uint32_t *em) {
+                       float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy) {
   if (g.N > 0)
     k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, g, P, PI,
-                                              gX, gV, abar, galpha, em);
+                                              gX, gV, abar, galpha, em, tkey, tfxy);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha) {
+                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy) {
   if (g.N > 0)
     k_adj_rpw32<<<nblocks(g.N), kBlock, 0, s>>>(X, bcell, Vnext, theta, g, P, xr, gX, gV, abar,
-                                                galpha);
+                                                galpha, tkey, tfxy);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,

@@ -177,5 +177,10 @@ void launch_theta_gather_f64(hipStream_t s, const double2 *X, const double *galp
 void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galpha,
                              const int *cstart, Grid g, ThetaGrid T, double4 *part,
                              double *gtheta);
+// the gather over per-slot interp-cell keys and FP64 (fx, fy) of the f32c
+// adjoint's pointwise pass
+void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
+                                   const float *galpha, const int *cstart, Grid g, ThetaGrid T,
+                                   double4 *part, double *gtheta);
 
 }  // namespace sph

@@ -94,6 +94,53 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
   if (lane == 0) part[j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
 }
 
+// The same gather over per-slot keys written by the adjoint's pointwise pass:
+// key[p] = j0 + i0*(ngy-1) of the slot's interp cell and fxy[p] = (fx, fy) in
+// FP64, so that a candidate costs one 4-byte load and a compare; the
+// contributions and their order are those of k_theta_cells.
+template <class GA>
+__global__ void __launch_bounds__(kBlock)
+k_theta_cells_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
+                    const GA *__restrict__ galpha, const int *__restrict__ cstart, Grid g,
+                    ThetaGrid T, double4 *__restrict__ part) {
+  const int ncx = T.ngx - 1, ncy = T.ngy - 1;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int t = (xcd_block() * blockDim.x + threadIdx.x) / kLanes;
+  if (t >= ncx * ncy) return;      // the kLanes lanes of a group leave together
+  const int i0 = t % ncx;
+  const int j0 = t / ncx;
+  const int mykey = j0 + i0 * ncy;
+  int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - 1;
+  int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + 1;
+  int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - 1;
+  int cy1 = (j0 == ncy - 1) ? g.ny - 1 : cell_of((double)(j0 + 1) * T.hy, T.cs, g.ny) + 1;
+  cx0 = cx0 < 0 ? 0 : cx0;
+  cy0 = cy0 < 0 ? 0 : cy0;
+  cx1 = cx1 > g.nx - 1 ? g.nx - 1 : cx1;
+  cy1 = cy1 > g.ny - 1 ? g.ny - 1 : cy1;
+  double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
+  for (int cy = cy0; cy <= cy1; ++cy) {
+    const int b = cstart[cy * g.nx + cx0], e = cstart[cy * g.nx + cx1 + 1];
+    for (int p = b + lane; p < e; p += kLanes) {
+      if (key[p] != mykey) continue;
+      const double2 f = fxy[p];
+      const double ga = (double)galpha[p];
+      s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
+      s10 += (ga * f.x) * (1.0 - f.y);
+      s01 += (ga * (1.0 - f.x)) * f.y;
+      s11 += (ga * f.x) * f.y;
+    }
+  }
+#pragma unroll
+  for (int m = kLanes / 2; m > 0; m >>= 1) {
+    s00 += __shfl_xor(s00, m, kLanes);
+    s10 += __shfl_xor(s10, m, kLanes);
+    s01 += __shfl_xor(s01, m, kLanes);
+    s11 += __shfl_xor(s11, m, kLanes);
+  }
+  if (lane == 0) part[j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
+}
+
 __global__ void k_theta_nodes(const double4 *__restrict__ part, ThetaGrid T,
                               double *__restrict__ gtheta) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,6 +170,15 @@ void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galph
                              double *gtheta) {
   const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
   k_theta_cells<float><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T, part);
+  k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
+}
+
+void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
+                                   const float *galpha, const int *cstart, Grid g, ThetaGrid T,
+                                   double4 *part, double *gtheta) {
+  const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
+  k_theta_cells_keyed<float><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(key, fxy, galpha, cstart,
+                                                                          g, T, part);
   k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
 }
 
