@@ -150,6 +150,32 @@ int sph_backward_dev(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
 int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
                              double *seed);
 
+/* --- batched scenes (the FP32-compensated form).  No counterpart in the
+ * reference, which runs one scene per State; its line search evaluates the
+ * design candidates one after another (scripts/03_optimize.jl:314-341).  A
+ * batched state holds nscenes independent scenes side by side (each with its
+ * own params, hence its own resolution, N and design grid); every kernel
+ * launch covers all of them, each workgroup working inside one scene.
+ * X0, V0: the scenes' particles in order, (2, sum N) interleaved.  The design
+ * field of a batch is one sph_field holding the scenes' (ngy, ngx)
+ * column-major fields one after the other (e.g. created with ngy = total
+ * nodes, ngx = 1).  sph_step, sph_simulate, sph_tape_*, sph_backward and
+ * sph_state_get/set work on batched states as on single ones; gradients and
+ * gtheta come back concatenated in scene order. --- */
+int sph_batch_create(sph_ctx *ctx, int64_t nscenes, const sph_params *params, const int64_t *N,
+                     const double *X0, const double *V0, double skin, int64_t interval,
+                     int64_t maxnb, sph_state **out);
+int sph_batch_workspace_create(sph_ctx *ctx, const sph_state *st, sph_workspace **out);
+/* *nscenes; when out != NULL, out[4k..4k+3] = {first particle, particle count,
+ * first design node, design nodes} of scene k */
+int sph_batch_info(const sph_state *st, int64_t *nscenes, int64_t *out);
+/* target_objective per scene: J[nscenes] (host), seed (2, sum N) in id order
+ * (host for sph_batch_objective, device for the _dev variant), or NULL */
+int sph_batch_objective(sph_state *st, double xc, double yc, double sigma, double *J,
+                        double *seed);
+int sph_batch_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
+                            double *seed);
+
 /* --- binomial checkpointing of the time loop.  No counterpart in the
  * reference, whose Tape keeps every step (src/driver.jl:38-65) and whose README
  * recommends recompute checkpointing for long horizons.  The horizon is cut

@@ -115,6 +115,8 @@ class Field:
 
 
 def _field(theta, p: SPHParams, backend: MI355XBackend) -> Field:
+    if isinstance(theta, Field) and getattr(theta, "batched", False):
+        return theta                      # the library checks the node count
     if isinstance(theta, Field):
         f = theta
     else:
@@ -538,6 +540,90 @@ def ckpt_plan(epochs, snapshots):
     o = np.zeros(3, np.int64)
     L.call("sph_ckpt_plan", int(epochs), int(snapshots), _ptr(o))
     return dict(forward_epochs=int(o[0]), taped=int(o[1]), peak=int(o[2]))
+
+
+class BatchField(Field):
+    """The design fields of a batch's scenes, one after the other (each (ngy, ngx)
+    column-major), as one device field."""
+    batched = True
+
+    def __init__(self, backend: MI355XBackend, thetas):
+        flat = np.concatenate([np.asarray(t, dtype=np.float64).ravel(order="F") for t in thetas])
+        super().__init__(backend, flat.reshape(-1, 1))
+        self.shapes = [np.shape(t) for t in thetas]
+
+
+class BatchState(State):
+    """Several independent scenes in one f32c state (sph_batch_create): every
+    kernel launch covers all of them.  ``scenes`` is a list of (SPHParams, X0,
+    V0); X/V of the state are the scenes' particles in order."""
+
+    def __init__(self, backend: MI355XBackend, scenes, *, skin=0.2, interval=4, maxnb=64):
+        params = (L.sph_params * len(scenes))(*[sp.c_struct() for sp, _, _ in scenes])
+        Ns = np.array([np.shape(x0)[1] for _, x0, _ in scenes], np.int64)
+        x0 = np.ascontiguousarray(np.concatenate([_as_2n(x) for _, x, _ in scenes]))
+        v0 = np.ascontiguousarray(np.concatenate([_as_2n(v) for _, _, v in scenes]))
+        self.backend, self.p = backend, scenes[0][0]
+        self.skin = float(skin)
+        self.rc = (2 * self.p.h) * (1 + self.skin)
+        self.precision = L.SPH_F32C
+        self.N = int(Ns.sum())
+        h = C.c_void_p()
+        L.call("sph_batch_create", backend._h, len(scenes), params, _ptr(Ns), _ptr(x0), _ptr(v0),
+               self.skin, int(interval), int(maxnb), C.byref(h))
+        self._h = h
+        self.cl = CellList(self)
+        self.nl = NeighborList(self)
+        self.params = [sp for sp, _, _ in scenes]
+        n = C.c_int64()
+        info = np.zeros(4 * len(scenes), np.int64)
+        L.call("sph_batch_info", self._h, C.byref(n), _ptr(info))
+        self.layout_info = info.reshape(-1, 4)
+        self.nscenes = int(n.value)
+
+    def split(self, A):
+        """(2, N) array -> list of the scenes' (2, N_k) arrays."""
+        return [A[:, o:o + n] for o, n, _, _ in self.layout_info]
+
+    def field(self, thetas):
+        return BatchField(self.backend, thetas)
+
+
+class BatchWorkspace(AdjointWorkspace):
+    """AdjointWorkspace of a BatchState; gtheta is the list of the scenes' fields."""
+
+    def __init__(self, backend: MI355XBackend, st: BatchState):
+        self.backend, self.N, self.p = backend, st.N, st.p
+        self._st_info = st.layout_info
+        self._shapes = [(sp.ngy, sp.ngx) for sp in st.params]
+        h = C.c_void_p()
+        L.call("sph_batch_workspace_create", backend._h, st._h, C.byref(h))
+        self._h = h
+
+    def _get(self, which):
+        if which == L.SPH_GTHETA:
+            tot = int(sum(a * b for a, b in self._shapes))
+            out = np.empty(tot)
+            L.call("sph_workspace_get", self._h, which, _ptr(out))
+            res, o = [], 0
+            for ny, nx in self._shapes:
+                res.append(out[o:o + ny * nx].reshape((ny, nx), order="F").copy())
+                o += ny * nx
+            return res
+        return super()._get(which)
+
+
+def batch_objective(st: BatchState, xc, yc, sigma, device=False):
+    """target_objective per scene: (J of shape (nscenes,), dJ/dX_T of shape (2, N))."""
+    J = np.zeros(st.nscenes)
+    if device:
+        buf = device if isinstance(device, DeviceArray) else DeviceArray(st.backend, N=st.N)
+        L.call("sph_batch_objective_dev", st._h, float(xc), float(yc), float(sigma), _ptr(J),
+               buf.ptr)
+        return J, buf
+    seed = np.empty((st.N, 2))
+    L.call("sph_batch_objective", st._h, float(xc), float(yc), float(sigma), _ptr(J), _ptr(seed))
+    return J, np.ascontiguousarray(seed.T)
 
 
 def target_objective(st: State, xc, yc, sigma, device=False):

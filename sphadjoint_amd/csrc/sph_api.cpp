@@ -162,6 +162,105 @@ ThetaGrid make_theta_grid(const sph_params &p, double rc) {
   return T;
 }
 
+// Device copy of an f32c state's scene table and workgroup -> scene maps.
+struct BatchStore {
+  Batch B{};
+  std::vector<Scene32> host;
+  Scene32 *sc = nullptr;
+  int *b2s = nullptr, *tb2s = nullptr, *nb2s = nullptr;
+};
+
+int blocks_of(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// One scene of an f32c state: its grid, constants and design grid, with the
+// offsets left at zero (batch_build lays the scenes out).
+Scene32 make_scene32(const sph_params &p, int64_t N, double skin) {
+  Scene32 c{};
+  const double rc = (2.0 * p.h) * (1.0 + skin);
+  long long nx = (long long)std::ceil(p.Lx / rc) + 2;
+  long long ny = (long long)std::ceil(p.Ly / rc) + 2;
+  nx = nx < 1 ? 1 : nx;
+  ny = ny < 1 ? 1 : ny;
+  if (nx * ny > (1LL << 30)) fail(SPH_E_ARG, "cell grid %lldx%lld too large", nx, ny);
+  if (nx >= (1LL << 24) || ny >= (1LL << 24)) fail(SPH_E_ARG, "cell grid too wide");
+  c.g.nx = (int)nx;
+  c.g.ny = (int)ny;
+  c.g.ncell = (int)(nx * ny);
+  c.g.N = (int)N;
+  c.g.inv_nx = 1.0 / (double)nx;
+  c.P = make_phys32(p, rc);
+  c.T = make_theta_grid(p, rc);
+  c.cs64 = rc;
+  c.dmax2 = (p.h * skin) * (p.h * skin);
+  return c;
+}
+
+void batch_free(BatchStore &bs) {
+  dfree(bs.sc);
+  dfree(bs.b2s);
+  dfree(bs.tb2s);
+  dfree(bs.nb2s);
+}
+
+// Lays the scenes out (slots, cells, design nodes and cells, workgroups in
+// order) and uploads the table and the workgroup maps.
+void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
+  long long off = 0, coff = 0, toff = 0, tcoff = 0, boff = 0, tboff = 0, nboff = 0;
+  for (auto &c : scenes) {
+    const long long nodes = (long long)c.T.ngx * c.T.ngy;
+    const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
+    c.off = (int)off;
+    c.coff = (int)coff;
+    c.toff = (int)toff;
+    c.tcoff = (int)tcoff;
+    c.boff = (int)boff;
+    c.tboff = (int)tboff;
+    c.nboff = (int)nboff;
+    off += c.g.N;
+    coff += c.g.ncell;
+    toff += nodes;
+    tcoff += tcells;
+    boff += blocks_of(c.g.N);
+    tboff += blocks_of(tcells * kThetaLanes);
+    nboff += blocks_of(nodes);
+  }
+  if (off >= (1LL << 27)) fail(SPH_E_ARG, "the f32c form packs slot offsets in 27 bits: N=%lld", off);
+  if (coff >= (1LL << 30)) fail(SPH_E_ARG, "too many cells in all scenes (%lld)", coff);
+  std::vector<int> b2s((size_t)boff), tb2s((size_t)tboff), nb2s((size_t)nboff);
+  for (int k = 0; k < (int)scenes.size(); ++k) {
+    const Scene32 &c = scenes[k];
+    const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
+    for (int b = 0; b < blocks_of(c.g.N); ++b) b2s[c.boff + b] = k;
+    for (int b = 0; b < blocks_of(tcells * kThetaLanes); ++b) tb2s[c.tboff + b] = k;
+    for (int b = 0; b < blocks_of((long long)c.T.ngx * c.T.ngy); ++b) nb2s[c.nboff + b] = k;
+  }
+  batch_free(bs);
+  bs.host = scenes;
+  bs.sc = dalloc<Scene32>(scenes.size());
+  bs.b2s = dalloc<int>(b2s.size() + 1);
+  bs.tb2s = dalloc<int>(tb2s.size() + 1);
+  bs.nb2s = dalloc<int>(nb2s.size() + 1);
+  HIPC(hipMemcpy(bs.sc, scenes.data(), sizeof(Scene32) * scenes.size(), hipMemcpyHostToDevice));
+  if (!b2s.empty()) HIPC(hipMemcpy(bs.b2s, b2s.data(), sizeof(int) * b2s.size(), hipMemcpyHostToDevice));
+  if (!tb2s.empty())
+    HIPC(hipMemcpy(bs.tb2s, tb2s.data(), sizeof(int) * tb2s.size(), hipMemcpyHostToDevice));
+  if (!nb2s.empty())
+    HIPC(hipMemcpy(bs.nb2s, nb2s.data(), sizeof(int) * nb2s.size(), hipMemcpyHostToDevice));
+  Batch &B = bs.B;
+  B.sc = bs.sc;
+  B.b2s = bs.b2s;
+  B.tb2s = bs.tb2s;
+  B.nb2s = bs.nb2s;
+  B.nscenes = (int)scenes.size();
+  B.nblk = (int)boff;
+  B.ntblk = (int)tboff;
+  B.nnblk = (int)nboff;
+  B.N = (int)off;
+  B.ncell = (int)coff;
+  B.ntnode = (int)toff;
+  B.ntcell = (int)tcoff;
+}
+
 void check_params(const sph_params *p) {
   if (!p) fail(SPH_E_ARG, "params is NULL");
   if (!(p->h > 0) || !(p->dt > 0) || !(p->Lx > 0) || !(p->Ly > 0))
@@ -279,6 +378,8 @@ struct sph_state {
   Grid g{};
   PhysF64 P{};
   PhysF32 P32{};
+  BatchStore bs;          // f32c form: scene table (one entry unless batched)
+  int64_t ntnode = 0;     // design-field nodes over all scenes
   CellConstF64 kc{};
   double skin = 0.2;
   int64_t interval = 4, maxnb = 64, sm = 0, si = 0;
@@ -344,6 +445,7 @@ struct sph_tape {
   double rc = 0.0;  // cell width of the recorded epochs
   CellConstF64 kc{};
   int ncell = 0;
+  BatchStore bs;  // f32c form: the scene table of the recorded state
 };
 
 struct sph_workspace {
@@ -362,6 +464,7 @@ struct sph_workspace {
   int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
   double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
+  int64_t ntnode = 0, ntcell = 0;  // design-field nodes and cells (all scenes)
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
   int *iota = nullptr;     // particle-id order as a slot order
   // f32c form
@@ -384,6 +487,7 @@ void state_free(sph_state *st) {
   for (auto pp : f2) dfree(*pp);
   dfree(st->nm);
   dfree(st->em);
+  batch_free(st->bs);
   double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
                    &st->tmp1, &st->obj_part, &st->obj_J};
   for (auto pp : d1) dfree(*pp);
@@ -396,7 +500,8 @@ void do_build(sph_state *st) {
   sph_ctx *c = st->ctx;
   timed(c, K_ASSIGN, [&] {
     launch_fill_i32(s, st->counts, st->g.ncell, 0);
-    launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts);
+    if (st->precision == SPH_F32C) launch_assign_f32c(s, st->X, st->bs.B, st->cellof, st->counts);
+    else launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts);
   });
   timed(c, K_SCAN, [&] {
     launch_scan_excl(s, st->counts, st->g.ncell, st->cl_cstart, st->cursor, st->bsum);
@@ -418,10 +523,10 @@ void materialise_aux(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
-    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->g, st->P32, st->PI32,
+    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->bs.B, st->PI32,
                         st->rho_slot, st->em);
     launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->em,
-                      st->cstart, st->g, st->P32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      st->cstart, st->bs.B, nullptr, nullptr, nullptr, nullptr, nullptr,
                       st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
@@ -444,8 +549,8 @@ void do_build_neighbors(sph_state *st) {
   int check = (st->age != kAgeNever) ? 1 : 0;
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_PERMUTE, [&] {
-      launch_permute_f32c(s, st->perm, st->cellof, st->X, st->V32, st->Xref, check, st->g, st->P32,
-                          st->kc.dmax2, st->Xn, st->V32n, st->xr, st->Xref2, st->scell, st->flags);
+      launch_permute_f32c(s, st->perm, st->cellof, st->X, st->V32, st->Xref, check, st->bs.B,
+                          st->Xn, st->V32n, st->xr, st->Xref2, st->scell, st->flags);
     });
     std::swap(st->V32, st->V32n);
   } else {
@@ -462,8 +567,7 @@ void do_build_neighbors(sph_state *st) {
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C)
-      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->g, st->P32,
-                           (float)(st->kc.cs * st->kc.cs), st->nm, st->flags);
+      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.B, st->nm, st->flags);
     else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
@@ -485,6 +589,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
     tp->rc = st->kc.cs;
     tp->kc = st->kc;
     tp->ncell = st->g.ncell;
+    if (st->precision == SPH_F32C) batch_build(tp->bs, st->bs.host);
     tp->has_geom = true;
   } else if (tp->ncell != st->g.ncell || tp->g.nx != st->g.nx) {
     fail(SPH_E_ARG, "tape already holds steps of a state with another cell grid");
@@ -534,12 +639,12 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->g, st->P32, PI32, nullptr,
+      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->bs.B, PI32, nullptr,
                           st->em);
     });
     timed(st->ctx, K_FORCE, [&] {
       launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->em, st->cstart,
-                        st->g, st->P32, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
+                        st->bs.B, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
     std::swap(st->xr, st->xrn);
@@ -559,15 +664,24 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   st->aux_state = 1;
 }
 
+bool is_batched(const sph_state *st) {
+  return st->precision == SPH_F32C && st->bs.B.nscenes > 1;
+}
+
 void need_precision(int precision) {
   if (precision != SPH_F64 && precision != SPH_F32C) fail(SPH_E_ARG, "unknown precision %d", precision);
 }
 
 void copy_theta(sph_state *st, const sph_field *th) {
   if (!th) fail(SPH_E_ARG, "theta is NULL");
-  if (th->ngy != st->p.ngy || th->ngx != st->p.ngx)
+  if (st->bs.B.nscenes > 1) {
+    if (th->ngy * th->ngx != st->ntnode)
+      fail(SPH_E_ARG, "batched theta has %lld nodes, the scenes have %lld",
+           (long long)(th->ngy * th->ngx), (long long)st->ntnode);
+  } else if (th->ngy != st->p.ngy || th->ngx != st->p.ngx) {
     fail(SPH_E_ARG, "theta is (%lld, %lld) but params want (%lld, %lld)", (long long)th->ngy,
          (long long)th->ngx, (long long)st->p.ngy, (long long)st->p.ngx);
+  }
   HIPC(hipMemcpyAsync(st->theta_last, th->d, sizeof(double) * th->ngy * th->ngx,
                       hipMemcpyDeviceToDevice, st->ctx->stream));
 }
@@ -644,7 +758,7 @@ const char *sph_profile_name(int kernel) {
 int sph_field_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, const double *host, sph_field **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
-    if (ngy < 2 || ngx < 2) fail(SPH_E_ARG, "field must be at least 2x2");
+    if (ngy < 1 || ngx < 1 || ngy * ngx < 4) fail(SPH_E_ARG, "field must hold at least 4 nodes");
     HIPC(hipSetDevice(ctx->device));
     auto *f = new sph_field;
     f->ctx = ctx;
@@ -751,13 +865,28 @@ int sph_buffer_write(sph_buffer *b, const void *host, int64_t nbytes) {
   });
 }
 
-int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, const double *X0,
-                     const double *V0, double skin, int64_t interval, int64_t maxnb, int layout,
-                     sph_state **out) {
-  return guarded([&] {
-    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
-    check_params(p);
+}  // extern "C"
+
+namespace {
+
+// State(backend, X0, V0, p; ...) for nsc scenes side by side (nsc > 1: the
+// f32c form's batched state); X0, V0 hold the scenes' particles in order.
+void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const int64_t *Ns,
+                       int precision, const double *X0, const double *V0, double skin,
+                       int64_t interval, int64_t maxnb, int layout, sph_state **out) {
+  {
+    if (!ctx || !out || !ps || !Ns) fail(SPH_E_ARG, "NULL argument");
+    if (nsc < 1) fail(SPH_E_ARG, "a state needs at least one scene");
     need_precision(precision);
+    if (nsc > 1 && precision != SPH_F32C) fail(SPH_E_ARG, "batched states use the f32c form");
+    int64_t N = 0;
+    for (int64_t k = 0; k < nsc; ++k) {
+      check_params(&ps[k]);
+      if (Ns[k] < 1) fail(SPH_E_ARG, "scene %lld: N=%lld out of range", (long long)k,
+                          (long long)Ns[k]);
+      N += Ns[k];
+    }
+    const sph_params *p = &ps[0];
     if (N < 1 || N > (int64_t)1 << 30) fail(SPH_E_ARG, "N=%lld out of range", (long long)N);
     if (precision == SPH_F32C && N >= (int64_t)1 << 27)
       fail(SPH_E_ARG, "the f32c form packs slot indices in 27 bits: N=%lld too large", (long long)N);
@@ -801,6 +930,19 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       st->kc.rc2 = rc * rc;
       st->kc.r2min = (DBL_EPSILON * p->h) * p->h;
       st->kc.dmax2 = dmax * dmax;
+      st->ntnode = p->ngx * p->ngy;
+      if (precision == SPH_F32C) {
+        std::vector<Scene32> scenes;
+        for (int64_t k = 0; k < nsc; ++k) scenes.push_back(make_scene32(ps[k], Ns[k], skin));
+        batch_build(st->bs, scenes);
+        if (nsc > 1) {
+          // the global arrays span every scene; per-scene grids live in the table
+          st->g.nx = st->g.ny = 0;
+          st->g.inv_nx = 0.0;
+          st->g.ncell = st->bs.B.ncell;
+          st->ntnode = st->bs.B.ntnode;
+        }
+      }
       const size_t n = (size_t)N, nc = (size_t)st->g.ncell;
       st->tmp2 = dalloc<double2>(n);
       st->ids = dalloc<int>(n);
@@ -846,12 +988,12 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
       st->rho_id = dalloc<double>(n);
       st->alpha_slot = dalloc<double>(n);
       st->rho_slot = dalloc<double>(n);
-      st->theta_last = dalloc<double>((size_t)(p->ngx * p->ngy));
+      st->theta_last = dalloc<double>((size_t)st->ntnode);
       st->seed_slot = dalloc<double2>(n);
       st->tmp1 = dalloc<double>(n);
       st->obj_nb = objective_blocks((int)N);
       st->obj_part = dalloc<double>(st->obj_nb + 2);
-      st->obj_J = dalloc<double>(1);
+      st->obj_J = dalloc<double>((size_t)nsc);
       hipStream_t s = ctx->stream;
       // State ctor: X, V copied in; a, rho, pterm, invrho, alpha start at zero
       std::vector<int> iota(n);
@@ -880,6 +1022,28 @@ int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N
     }
     ctx_retain(ctx);
     *out = st;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_state_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, const double *X0,
+                     const double *V0, double skin, int64_t interval, int64_t maxnb, int layout,
+                     sph_state **out) {
+  return guarded([&] {
+    if (!p) fail(SPH_E_ARG, "NULL argument");
+    state_create_impl(ctx, p, 1, &N, precision, X0, V0, skin, interval, maxnb, layout, out);
+  });
+}
+
+int sph_batch_create(sph_ctx *ctx, int64_t nscenes, const sph_params *params, const int64_t *N,
+                     const double *X0, const double *V0, double skin, int64_t interval,
+                     int64_t maxnb, sph_state **out) {
+  return guarded([&] {
+    state_create_impl(ctx, params, nscenes, N, SPH_F32C, X0, V0, skin, interval, maxnb,
+                      SPH_LAYOUT_SLOT, out);
   });
 }
 
@@ -955,7 +1119,7 @@ int sph_state_set(sph_state *st, int field, const double *host) {
         launch_gather2f(s, st->tmp2, st->ids, (int)st->N, st->V32);
       } else {
         launch_gather2(s, st->tmp2, st->ids, (int)st->N, st->X);
-        if (st->age != kAgeNever) launch_xrel_f32c(s, st->X, st->scell, st->g, st->P32, st->xr);
+        if (st->age != kAgeNever) launch_xrel_f32c(s, st->X, st->scell, st->bs.B, st->xr);
       }
     } else {
       launch_gather2(s, st->tmp2, st->ids, (int)st->N, field == SPH_X ? st->X : st->V);
@@ -983,6 +1147,7 @@ int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *
                       int32_t *order) {
   return guarded([&] {
     if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (is_batched(st)) fail(SPH_E_ARG, "cell lists of batched states are not exported");
     if (st->cl_view == 0) fail(SPH_E_STATE, "no cell list has been built");
     hipStream_t s = st->ctx->stream;
     const size_t n = (size_t)st->N, nc = (size_t)st->g.ncell;
@@ -1006,6 +1171,7 @@ int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *
 int sph_neighbors_get(sph_state *st, int32_t *indices, int32_t *counts) {
   return guarded([&] {
     if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (is_batched(st)) fail(SPH_E_ARG, "neighbour lists of batched states are not exported");
     if (st->age == kAgeNever) fail(SPH_E_STATE, "the neighbour list has not been built");
     hipStream_t s = st->ctx->stream;
     const size_t n = (size_t)st->N;
@@ -1122,6 +1288,7 @@ int sph_tape_destroy(sph_tape *tp) {
     dfree(tp->V32);
     dfree(tp->PI);
     dfree(tp->PI32);
+    batch_free(tp->bs);
     sph_ctx *c = tp->ctx;
     delete tp;
     ctx_release(c);
@@ -1196,9 +1363,15 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->tfxy);
 }
 
-int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
-                         sph_workspace **out) {
-  return guarded([&] {
+}  // extern "C"
+
+namespace {
+
+// AdjointWorkspace for N particles; ntnode, ntcell > 0 size the design-field
+// gradient for the scenes of a batched state
+void workspace_create_impl(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
+                           int64_t ntnode, int64_t ntcell, sph_workspace **out) {
+  {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
     check_params(p);
     need_precision(precision);
@@ -1210,13 +1383,19 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       ws->N = N;
       ws->p = *p;
       ws->precision = precision;
+      ws->ntnode = ntnode;
+      ws->ntcell = ntcell;
       const size_t n = (size_t)N;
       ws->t2 = dalloc<double2>(n);
       ws->gX_id = dalloc<double2>(n);
       ws->gV_id = dalloc<double2>(n);
-      ws->gtheta = dalloc<double>((size_t)(p->ngx * p->ngy));
+      if (ws->ntnode == 0) {
+        ws->ntnode = p->ngx * p->ngy;
+        ws->ntcell = (p->ngx - 1) * (p->ngy - 1);
+      }
+      ws->gtheta = dalloc<double>((size_t)ws->ntnode);
       ws->inv = dalloc<int>(n);
-      ws->tpart = dalloc<double4>((size_t)((p->ngx - 1) * (p->ngy - 1)));
+      ws->tpart = dalloc<double4>((size_t)ws->ntcell);
       ws->nbcnt = dalloc<int>(n);
       ws->flags = dalloc<int>(2);
       HIPC(hipMemset(ws->flags, 0, sizeof(int) * 2));
@@ -1250,7 +1429,7 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
       }
       HIPC(hipMemset(ws->gX_id, 0, sizeof(double2) * n));
       HIPC(hipMemset(ws->gV_id, 0, sizeof(double2) * n));
-      HIPC(hipMemset(ws->gtheta, 0, sizeof(double) * p->ngx * p->ngy));
+      HIPC(hipMemset(ws->gtheta, 0, sizeof(double) * ws->ntnode));
       HIPC(hipDeviceSynchronize());
     } catch (...) {
       ws_free(ws);
@@ -1259,6 +1438,45 @@ int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int preci
     }
     ctx_retain(ctx);
     *out = ws;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_workspace_create(sph_ctx *ctx, int64_t N, const sph_params *p, int precision,
+                         sph_workspace **out) {
+  return guarded([&] { workspace_create_impl(ctx, N, p, precision, 0, 0, out); });
+}
+
+int sph_batch_workspace_create(sph_ctx *ctx, const sph_state *st, sph_workspace **out) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (st->precision != SPH_F32C) fail(SPH_E_ARG, "batched workspaces use the f32c form");
+    workspace_create_impl(ctx, st->N, &st->p, SPH_F32C, st->bs.B.ntnode, st->bs.B.ntcell, out);
+  });
+}
+
+// per-scene layout of a batched state: for each scene k, out[4k..4k+3] =
+// {first particle, particle count, first design node, design nodes}
+int sph_batch_info(const sph_state *st, int64_t *nscenes, int64_t *out) {
+  return guarded([&] {
+    if (!st || !nscenes) fail(SPH_E_ARG, "NULL argument");
+    const int nsc = st->precision == SPH_F32C ? st->bs.B.nscenes : 1;
+    *nscenes = nsc;
+    if (!out) return;
+    if (st->precision != SPH_F32C) {
+      out[0] = 0; out[1] = st->N; out[2] = 0; out[3] = st->ntnode;
+      return;
+    }
+    for (int k = 0; k < nsc; ++k) {
+      const Scene32 &c = st->bs.host[k];
+      out[4 * k + 0] = c.off;
+      out[4 * k + 1] = c.g.N;
+      out[4 * k + 2] = c.toff;
+      out[4 * k + 3] = (int64_t)c.T.ngx * c.T.ngy;
+    }
   });
 }
 
@@ -1285,7 +1503,7 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
         HIPC(hipMemcpyAsync(host, ws->gV_id, sizeof(double2) * ws->N, hipMemcpyDeviceToHost, s));
         break;
       case SPH_GTHETA:
-        HIPC(hipMemcpyAsync(host, ws->gtheta, sizeof(double) * ws->p.ngx * ws->p.ngy,
+        HIPC(hipMemcpyAsync(host, ws->gtheta, sizeof(double) * ws->ntnode,
                             hipMemcpyDeviceToHost, s));
         break;
       default:
@@ -1332,7 +1550,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const bool f32c = ws->precision == SPH_F32C;
   const Grid g = tape->g;
   const PhysF64 P = make_phys(ws->p);
-  const PhysF32 P32 = make_phys32(ws->p, tape->rc);
+  const Batch &B = tape->bs.B;   // f32c form
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
   int cur = tape->entry_epoch[tape->n];
   // the adjoint state into the slot order of the final entry
@@ -1366,9 +1584,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       timed(c, K_NLBUILD, [&] {
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
-          launch_xrel_f32c(s, X0, E.scell, g, P32, ws->xr32);
-          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, g, P32,
-                               (float)(tape->rc * tape->rc), ws->nm, ws->flags);
+          launch_xrel_f32c(s, X0, E.scell, B, ws->xr32);
+          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, B, ws->nm, ws->flags);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);
@@ -1400,17 +1617,17 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
-          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, g, P32, ws->xr32, ws->gX32,
+          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, B, ws->xr32, ws->gX32,
                               ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy);
         } else {
-          launch_xrel_f32c(s, Xk, E.scell, g, P32, ws->xr32);
-          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, g, P32,
+          launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
+          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, B,
                             ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em,
                             ws->tkey, ws->tfxy);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, g, T,
+        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, B,
                                       ws->tpart, ws->gtheta);
       });
       timed(c, K_ADJ_P1, [&] {
@@ -1418,14 +1635,14 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         // stored (pterm, invrho) P1 tests the pairs and writes it for P2
         if (tape->keep_pi)
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             nm, false, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32, ws->em);
+                             nm, false, E.cstart, B, ws->gXs32, ws->gV32, ws->grho32, ws->em);
         else
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             ws->em, true, E.cstart, g, P32, ws->gXs32, ws->gV32, ws->grho32,
+                             ws->em, true, E.cstart, B, ws->gXs32, ws->gV32, ws->grho32,
                              nullptr);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, g, P32,
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, B,
                            ws->gX32);
       });
     } else {
@@ -1475,7 +1692,7 @@ void bw_seed(sph_workspace *ws, const double *seedX, const double *seedV, bool d
   hipStream_t s = ws->ctx->stream;
   const int N = (int)ws->N;
   const size_t n = (size_t)N;
-  HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->p.ngx * ws->p.ngy, s));
+  HIPC(hipMemsetAsync(ws->gtheta, 0, sizeof(double) * ws->ntnode, s));
   if (ws->precision == SPH_F32C && dev) {
     launch_d2_to_f2(s, reinterpret_cast<const double2 *>(seedX), N, ws->gX32);
     launch_d2_to_f2(s, reinterpret_cast<const double2 *>(seedV), N, ws->gV32);
@@ -1509,7 +1726,7 @@ void bw_finish(sph_workspace *ws) {
 void check_bw_args(sph_workspace *ws, const sph_field *theta, const double *seedX,
                    const double *seedV) {
   if (!ws || !theta || !seedX || !seedV) fail(SPH_E_ARG, "NULL argument");
-  if (theta->ngy != ws->p.ngy || theta->ngx != ws->p.ngx) fail(SPH_E_ARG, "theta shape mismatch");
+  if (theta->ngy * theta->ngx != ws->ntnode) fail(SPH_E_ARG, "theta shape mismatch");
 }
 
 }  // namespace
@@ -1542,6 +1759,7 @@ int objective_impl(sph_state *st, double xc, double yc, double sigma, double *J,
                    bool dev) {
   return guarded([&] {
     if (!st || !J) fail(SPH_E_ARG, "NULL argument");
+    if (is_batched(st)) fail(SPH_E_ARG, "batched states have one J per scene: sph_batch_objective");
     hipStream_t s = st->ctx->stream;
     const int N = (int)st->N;
     launch_objective_f64(s, st->X, N, xc, yc, sigma, st->obj_part, st->obj_nb, st->obj_J,
@@ -1580,6 +1798,50 @@ int sph_target_objective(sph_state *st, double xc, double yc, double sigma, doub
 int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
                              double *seed) {
   return objective_impl(st, xc, yc, sigma, J, seed, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+// target_objective per scene of a (batched) f32c state: J[k] in FP64 and the
+// seeds of every scene, scene by scene as the single-scene objective forms them
+int batch_objective_impl(sph_state *st, double xc, double yc, double sigma, double *J,
+                         double *seed, bool dev) {
+  return guarded([&] {
+    if (!st || !J) fail(SPH_E_ARG, "NULL argument");
+    if (st->precision != SPH_F32C) fail(SPH_E_ARG, "batched objectives use the f32c form");
+    hipStream_t s = st->ctx->stream;
+    const int nsc = st->bs.B.nscenes;
+    for (int k = 0; k < nsc; ++k) {
+      const Scene32 &c = st->bs.host[k];
+      launch_objective_f64(s, st->X + c.off, c.g.N, xc, yc, sigma, st->obj_part,
+                           objective_blocks(c.g.N), st->obj_J + k,
+                           seed ? st->seed_slot + c.off : nullptr);
+    }
+    HIPC(hipMemcpyAsync(J, st->obj_J, sizeof(double) * nsc, hipMemcpyDeviceToHost, s));
+    if (seed && dev) {
+      launch_scatter2(s, st->seed_slot, st->ids, (int)st->N, reinterpret_cast<double2 *>(seed));
+    } else if (seed) {
+      launch_scatter2(s, st->seed_slot, st->ids, (int)st->N, st->tmp2);
+      HIPC(hipMemcpyAsync(seed, st->tmp2, sizeof(double2) * st->N, hipMemcpyDeviceToHost, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_batch_objective(sph_state *st, double xc, double yc, double sigma, double *J,
+                        double *seed) {
+  return batch_objective_impl(st, xc, yc, sigma, J, seed, false);
+}
+
+int sph_batch_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
+                            double *seed) {
+  return batch_objective_impl(st, xc, yc, sigma, J, seed, true);
 }
 
 }  // extern "C"

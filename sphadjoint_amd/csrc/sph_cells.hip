@@ -23,23 +23,6 @@ __global__ void k_fill_i32(int *p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
-// Run heads of equal keys within a wavefront.  Lanes whose key is -1 are
-// inactive.  Returns the ballot of heads; `head_lane` is the head of this
-// lane's run and `run_len` is set on heads.
-__device__ __forceinline__ unsigned long long wave_runs(int key, int &head_lane, int &run_len) {
-  const int lane = threadIdx.x & 63;
-  int prev = __shfl_up(key, 1, 64);
-  bool head = (lane == 0) || (prev != key);
-  unsigned long long heads = __ballot(head);
-  // next head strictly above this lane (or 64)
-  unsigned long long above = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-  int next = above ? __ffsll((long long)above) - 1 : 64;
-  unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
-  head_lane = 63 - __clzll((long long)below);
-  run_len = next - lane;
-  return heads;
-}
-
 __global__ void k_assign(const double2 *__restrict__ X, Grid g, double cs, int *__restrict__ cellof,
                          int *__restrict__ counts) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;

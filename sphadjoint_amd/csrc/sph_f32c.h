@@ -14,47 +14,84 @@ struct PhysF32 {
   int ngx, ngy;
 };
 
+// One scene of an f32c state.  A state holds one or more independent scenes
+// (batched line-search candidates, say) side by side: scene k owns the slots
+// [off, off + g.N), the cells [coff, coff + g.ncell) of the global cell
+// numbering (cstart holds global slots), the design-field nodes
+// [toff, toff + ngx*ngy) and the design cells [tcoff, tcoff + (ngx-1)*(ngy-1)).
+// Every per-slot launch covers all scenes at once: its workgroups are dealt to
+// the scenes in order (boff = the scene's first workgroup), so a workgroup
+// never straddles two scenes and reads its scene's constants once, through
+// scalar loads.
+struct Scene32 {
+  Grid g;                  // the scene's own cell grid; g.N = its particle count
+  PhysF32 P;
+  ThetaGrid T;
+  double cs64;             // cell width in FP64 (binning by the division rule)
+  double dmax2;            // squared displacement allowed between rebuilds (skin*h)^2
+  int off, coff, boff;     // first slot, first cell, first workgroup of slot launches
+  int toff, tcoff, tboff;  // first theta node, first theta cell, first workgroup of
+                           // theta-cell launches (kThetaLanes lanes per cell)
+  int nboff;               // first workgroup of theta-node launches
+};
+
+// Launch geometry of an f32c state: device scene table, workgroup -> scene
+// maps of the three launch shapes, and totals.
+struct Batch {
+  const Scene32 *sc;
+  const int *b2s, *tb2s, *nb2s;
+  int nscenes, nblk, ntblk, nnblk;
+  int N, ncell, ntnode, ntcell;
+};
+constexpr int kThetaLanes = 8;   // lanes that share one theta cell in the gtheta gather
+
 // bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)
 constexpr int kMaskWords = 4;
 
+void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
+                        int *counts);
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          Grid g, PhysF32 P, float rc2, uint32_t *nm, int *flags);
+                          const Batch &B, uint32_t *nm, int *flags);
 // density + EOS over the mask list nm; when em is given, the exact list of
 // these positions (pairs with r2 < (2h)^2) is written there for the force pass
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho,
+                         const int *cstart, const Batch &B, float2 *PI, double *rho,
                          uint32_t *em);
 // force + integrator over the exact list nm of the density pass
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
+                       const int *cstart, const Batch &B, double2 *Xn, float2 *xrn, float2 *Vn,
                        double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out);
-void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
+void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
                       float2 *xr);
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
-                         const float2 *V, const double2 *Xref, int check, Grid g, PhysF32 P,
-                         double dmax2, double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn,
-                         int *bcell, int *flags);
+                         const float2 *V, const double2 *Xref, int check, const Batch &B,
+                         double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
+                         int *flags);
 // R passes also write each slot's interp-cell key and FP64 fractions for the
 // keyed gtheta gather
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
+                       const int *cstart, const Batch &B, float2 *PI, float2 *gX, float2 *gV,
                        float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
-                         const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
+                         const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
                         const double *theta, const int *bcell, const uint32_t *nm, bool exact,
-                        const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
+                        const int *cstart, const Batch &B, float2 *gXs, float2 *gV,
                         float *grho, uint32_t *em);
 // P2 over the exact list nm
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
+                        const int *bcell, const uint32_t *nm, const int *cstart, const Batch &B,
                         float2 *gX);
+// deterministic gtheta gather over the keys of the R passes (sph_theta.hip)
+void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
+                                   const float *galpha, const int *cstart, const Batch &B,
+                                   double4 *part, double *gtheta);
 
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);

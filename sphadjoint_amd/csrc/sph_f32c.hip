@@ -161,6 +161,23 @@ __device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
 }
 
+// The scene of this workgroup and the slot s of this thread (threads past the
+// scene's last slot leave); g and P are the scene's grid and constants.
+#define SCENE_SLOT(B)                                                   \
+  const int blk_ = xcd_block();                                         \
+  const Scene32 S = (B).sc[(B).b2s[blk_]];                              \
+  const int sl_ = (blk_ - S.boff) * kBlock + (int)threadIdx.x;          \
+  if (sl_ >= S.g.N) return;                                             \
+  const int s = S.off + sl_;                                            \
+  const Grid &g = S.g;                                                  \
+  const PhysF32 &P = S.P
+
+// descriptors of the list and the per-slot arrays of one state
+#define NB_RSRC                                                                \
+  const Rsrc nmr = rsrc(nm, (uint32_t)B.N * (uint32_t)(3 * kMaskWords) * 4u);  \
+  const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);                  \
+  const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u)
+
 // (cx, cy) of a 0-based cell; the quotient estimate in FP64 is exact to one
 // unit for cells < 2^30 and is corrected once
 __device__ __forceinline__ int2 cell_xy(int c, const Grid &g) {
@@ -177,8 +194,8 @@ __device__ __forceinline__ int2 cell_xy(int c, const Grid &g) {
 struct RowRun {
   int rb, len, s0, s1;
 };
-__device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &g) {
-  const int base = jy * g.nx;
+__device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &g, int coff) {
+  const int base = coff + jy * g.nx;
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
   RowRun r;
@@ -217,20 +234,20 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
 
 template <class Ld, class Fn>
 __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
-                                            int bc, float2 xi, const Grid &g, float cs, Ld &&ld,
-                                            Fn &&fn) {
+                                            int bc, float2 xi, const Grid &g, int coff,
+                                            uint32_t Ntot, float cs, Ld &&ld, Fn &&fn) {
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const int jy = c.y + r - 1;
     if (jy < 0 || jy >= g.ny) continue;
-    const RowRun run = row_run(cst, c.x, jy, g);
+    const RowRun run = row_run(cst, c.x, jy, g, coff);
     const float yi = xi.y - (float)(r - 1) * cs;
     int nw = (run.len + 31) >> 5;
     nw = nw < kMaskWords ? nw : kMaskWords;
     for (int w = 0; w < nw; ++w) {
-      const uint32_t widx = (uint32_t)(w * 3 + r) * (uint32_t)g.N + (uint32_t)s;
+      const uint32_t widx = (uint32_t)(w * 3 + r) * Ntot + (uint32_t)s;
       uint32_t m = ldu(nmr, widx * 4u);
       uint32_t hit = 0;
       if (m != 0) {
@@ -276,12 +293,12 @@ struct NbXG {
 
 __global__ void __launch_bounds__(kBlock)
 k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-             const int *__restrict__ cstart, Grid g, PhysF32 P, float rc2,
-             uint32_t *__restrict__ nm, int *__restrict__ flags) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
-  const Rsrc xrr = rsrc(xr, (uint32_t)g.N * 8u);
-  const Rsrc cst = rsrc(cstart, (uint32_t)(g.ncell + 1) * 4u);
+             const int *__restrict__ cstart, Batch B, uint32_t *__restrict__ nm,
+             int *__restrict__ flags) {
+  SCENE_SLOT(B);
+  const float rc2 = (float)(S.cs64 * S.cs64);
+  const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
+  const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   const float2 xi = xr[s];
   const int2 c = cell_xy(bcell[s], g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
@@ -290,7 +307,7 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   for (int r = 0; r < 3; ++r) {
     const int jy = c.y + r - 1;
     if (jy < 0 || jy >= g.ny) continue;
-    const RowRun run = row_run(cst, c.x, jy, g);
+    const RowRun run = row_run(cst, c.x, jy, g, S.coff);
     const float yi = xi.y - (float)(r - 1) * P.cs;
     int len = run.len;
     if (len > kMaskRun) {
@@ -310,19 +327,20 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
         const float r2 = fmaf(dx, dx, dy * dy);
         if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
       }
-      nm[(size_t)(w * 3 + r) * g.N + s] = m;
+      nm[(size_t)(w * 3 + r) * B.N + s] = m;
     }
   }
   if (over) flags[0] = 1;
 }
 
 __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, Rsrc xrr, int s,
-                                               int bc, float2 xi, const Grid &g,
-                                               const PhysF32 &P) {
+                                               int bc, float2 xi, const Grid &g, int coff,
+                                               uint32_t Ntot, const PhysF32 &P) {
   // Fast2Sum from the self term, the largest term of the sum: every later
   // term is positive and smaller, so each step's error term is exact
   FSum acc(P.mA);
-  for_each_nb(nmr, em, cst, s, bc, xi, g, P.cs, [&](uint32_t o8) __attribute__((always_inline)) { return NbX{ld2(xrr, o8)}; },
+  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs,
+              [&](uint32_t o8) __attribute__((always_inline)) { return NbX{ld2(xrr, o8)}; },
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
@@ -338,11 +356,6 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
   return make_float2(P.rho0 + drho, drho);
 }
 
-// descriptors of the list and the per-slot arrays of one state
-#define NB_RSRC                                                             \
-  const Rsrc nmr = rsrc(nm, (uint32_t)g.N * (uint32_t)(3 * kMaskWords) * 4u); \
-  const Rsrc cst = rsrc(cstart, (uint32_t)(g.ncell + 1) * 4u);               \
-  const Rsrc xrr = rsrc(xr, (uint32_t)g.N * 8u)
 
 __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
   float inv = 1.f / rd.x;
@@ -351,12 +364,11 @@ __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
 
 __global__ void __launch_bounds__(kBlock)
 k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-            const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+            const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
             float2 *__restrict__ PI, double *__restrict__ rho_out, uint32_t *__restrict__ em) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   NB_RSRC;
-  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, P);
+  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, S.coff, B.N, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -365,22 +377,21 @@ __global__ void __launch_bounds__(kBlock)
 k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ V, const float2 *__restrict__ PI,
           const double *__restrict__ theta, const int *__restrict__ bcell,
-          const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+          const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
           double2 *__restrict__ Xn, float2 *__restrict__ xrn, float2 *__restrict__ Vn,
           double2 *__restrict__ tapeX, float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
           double *__restrict__ alpha_out) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
   const int bc = bcell[s];
   const float tmi = P.twomumass * pii.y;
   NB_RSRC;
-  const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
-  const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
+  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   CSum ax, ay;
   // nm is the exact list of the density pass over these positions
-  for_each_nb(nmr, nullptr, cst, s, bc, xi, g, P.cs,
+  for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs,
               [&](uint32_t o8) __attribute__((always_inline)) { return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)}; },
               [&, P](const NbXPV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
@@ -395,7 +406,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
     return true;
   });
   InterpF ic = interp_cell(x64.x, x64.y, P);
-  float al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float axv = ax.value() + (P.gx + wall_accel((float)x64.x, P.Lx, P.kw));
   float ayv = ay.value() + (P.gy + wall_accel((float)x64.y, P.Ly, P.kw));
   if (a_out) {
@@ -416,28 +427,30 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   }
 }
 
-__global__ void k_xrel(const double2 *__restrict__ X, const int *__restrict__ bcell, Grid g,
-                       PhysF32 P, float2 *__restrict__ xr) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < g.N) xr[s] = rel_coord(X[s], bcell[s], g, P);
+__global__ void k_xrel(const double2 *__restrict__ X, const int *__restrict__ bcell, Batch B,
+                       float2 *__restrict__ xr) {
+  SCENE_SLOT(B);
+  xr[s] = rel_coord(X[s], bcell[s], g, P);
 }
 
 __global__ void k_permute32(const int *__restrict__ perm, const int *__restrict__ cellof,
                             const double2 *__restrict__ X, const float2 *__restrict__ V,
-                            const double2 *__restrict__ Xref, int check, Grid g, PhysF32 P,
-                            double dmax2, double2 *__restrict__ Xn, float2 *__restrict__ Vn,
+                            const double2 *__restrict__ Xref, int check, Batch B,
+                            double2 *__restrict__ Xn, float2 *__restrict__ Vn,
                             float2 *__restrict__ xr, double2 *__restrict__ Xrefn,
                             int *__restrict__ bcell, int *__restrict__ flags) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.N) return;
+  // slots keep their scene through the reorder: scenes own contiguous cell
+  // ranges, and cells own contiguous slot ranges
+  SCENE_SLOT(B);
+  const int t = s;
   int o = perm[t];
   double2 x = X[o];
   if (check) {
     double2 r = Xref[o];
     double ddx = x.x - r.x, ddy = x.y - r.y;
-    if (ddx * ddx + ddy * ddy > dmax2) flags[1] = 1;
+    if (ddx * ddx + ddy * ddy > S.dmax2) flags[1] = 1;
   }
-  int c = cellof[o];
+  int c = cellof[o] - S.coff;     // the scene's own cell numbering
   Xn[t] = x;
   Vn[t] = V[o];
   Xrefn[t] = x;
@@ -445,25 +458,48 @@ __global__ void k_permute32(const int *__restrict__ perm, const int *__restrict_
   xr[t] = rel_coord(x, c, g, P);
 }
 
+// cell of every slot (the scene's grid, the division rule of cell_index,
+// src/neighbors.jl:38-42) in the global cell numbering, counts by
+// wave-aggregated atomics
+__global__ void __launch_bounds__(kBlock)
+k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
+           int *__restrict__ counts) {
+  const int blk = blockIdx.x;     // all lanes of the workgroup take part in the run search
+  const Scene32 S = B.sc[B.b2s[blk]];
+  const int sl = (blk - S.boff) * kBlock + (int)threadIdx.x;
+  int c = -1;
+  if (sl < S.g.N) {
+    const int s = S.off + sl;
+    const double2 x = X[s];
+    const int cx = cell_coord_div(x.x / S.cs64, S.g.nx);
+    const int cy = cell_coord_div(x.y / S.cs64, S.g.ny);
+    c = S.coff + cx + cy * S.g.nx;
+    cellof[s] = c;
+  }
+  int head_lane, run_len;
+  wave_runs(c, head_lane, run_len);
+  const int lane = threadIdx.x & 63;
+  if (c >= 0 && head_lane == lane) atomicAdd(&counts[c], run_len);
+}
+
 // adjoint R: density recompute, _adj_integrate!, _adj_drag! (src/adjoint.jl:193-228)
 __global__ void __launch_bounds__(kBlock)
 k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ Vnext, const double *__restrict__ theta,
           const int *__restrict__ bcell, const uint32_t *__restrict__ nm,
-          const int *__restrict__ cstart, Grid g, PhysF32 P, float2 *__restrict__ PI,
+          const int *__restrict__ cstart, Batch B, float2 *__restrict__ PI,
           float2 *__restrict__ gX, float2 *__restrict__ gV, float2 *__restrict__ abar,
           float *__restrict__ galpha, uint32_t *__restrict__ em, int *__restrict__ tkey,
           double2 *__restrict__ tfxy) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   const double2 x64 = X[s];
   NB_RSRC;
-  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, P);
+  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, S.coff, B.N, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
-  tkey[s] = ic.l00 - ic.l00 / P.ngy;          // j0 + i0*(ngy-1)
+  tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
   tfxy[s] = make_double2(ic.fx64, ic.fy64);
-  float al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
@@ -478,18 +514,17 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 // pterm/invrho); also forms this step's relative coordinates xr
 __global__ void __launch_bounds__(kBlock)
 k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
-            const float2 *__restrict__ Vnext, const double *__restrict__ theta, Grid g, PhysF32 P,
+            const float2 *__restrict__ Vnext, const double *__restrict__ theta, Batch B,
             float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
             float2 *__restrict__ abar, float *__restrict__ galpha, int *__restrict__ tkey,
             double2 *__restrict__ tfxy) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   const double2 x64 = X[s];
   xr[s] = rel_coord(x64, bcell[s], g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
-  tkey[s] = ic.l00 - ic.l00 / P.ngy;          // j0 + i0*(ngy-1)
+  tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
   tfxy[s] = make_double2(ic.fx64, ic.fy64);
-  float al = interp_alpha(theta, ic, P, nullptr, nullptr);
+  float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
@@ -509,20 +544,19 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const float2 *__restrict__ V, const float2 *__restrict__ PI,
            const float2 *__restrict__ abar, const float *__restrict__ galpha,
            const double *__restrict__ theta, const int *__restrict__ bcell,
-           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
            float2 *__restrict__ gXs, float2 *__restrict__ gV, float *__restrict__ grho,
            uint32_t *__restrict__ em) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const float pti = pii.x, iri = pii.y;
   NB_RSRC;
-  const Rsrc pir = rsrc(PI, (uint32_t)g.N * 8u);
-  const Rsrc vr = rsrc(V, (uint32_t)g.N * 8u);
-  const Rsrc abr = rsrc(abar, (uint32_t)g.N * 8u);
+  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
+  const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
   ASum gx, gy, gvx, gvy, grh, sumS;
-  for_each_nb(nmr, kExact ? nullptr : em, cst, s, bcell[s], xi, g, P.cs,
+  for_each_nb(nmr, kExact ? nullptr : em, cst, s, bcell[s], xi, g, S.coff, B.N, P.cs,
               [&](uint32_t o8) __attribute__((always_inline)) {
                 return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
               },
@@ -558,14 +592,14 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
     }
     return false;
   });
-  float S = sumS.value();
-  float grv = grh.value() + (-2.f * P.m * pti * iri) * S + (P.c2 * P.m * iri * iri) * S;
+  float Ssum = sumS.value();
+  float grv = grh.value() + (-2.f * P.m * pti * iri) * Ssum + (P.c2 * P.m * iri * iri) * Ssum;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
   float gyv = gy.value() + ab.y * wall_daccel((float)x64.y, P.Ly, P.kw);
   float ga = galpha[s];
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float dadx, dady;
-  interp_alpha(theta, ic, P, &dadx, &dady);
+  interp_alpha(theta + S.toff, ic, P, &dadx, &dady);
   gxv += ga * dadx;
   gyv += ga * dady;
   gXs[s] = make_float2(gxv, gyv);
@@ -578,17 +612,16 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 __global__ void __launch_bounds__(kBlock)
 k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
            const float2 *__restrict__ gXs, const int *__restrict__ bcell,
-           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Grid g, PhysF32 P,
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
            float2 *__restrict__ gX) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  SCENE_SLOT(B);
   const float2 xi = xr[s];
   const float gri = grho[s];
   NB_RSRC;
-  const Rsrc grr = rsrc(grho, (uint32_t)g.N * 4u);
+  const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
   ASum gx, gy;
   // nm is the exact list of P1 over these positions
-  for_each_nb(nmr, nullptr, cst, s, bcell[s], xi, g, P.cs,
+  for_each_nb(nmr, nullptr, cst, s, bcell[s], xi, g, S.coff, B.N, P.cs,
               [&](uint32_t o8) __attribute__((always_inline)) { return NbXG{ld2(xrr, o8), ld1(grr, o8 >> 1)}; },
               [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
@@ -605,70 +638,71 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
 
 }  // namespace
 
+void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
+                        int *counts) {
+  if (B.N > 0) k_assign32<<<B.nblk, kBlock, 0, s>>>(X, B, cellof, counts);
+}
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          Grid g, PhysF32 P, float rc2, uint32_t *nm, int *flags) {
-  if (g.N > 0)
-    k_nm_build32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, cstart, g, P, rc2, nm, flags);
+                          const Batch &B, uint32_t *nm, int *flags) {
+  if (B.N > 0) k_nm_build32<<<B.nblk, kBlock, 0, s>>>(xr, bcell, cstart, B, nm, flags);
 }
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, Grid g, PhysF32 P, float2 *PI, double *rho,
+                         const int *cstart, const Batch &B, float2 *PI, double *rho,
                          uint32_t *em) {
-  if (g.N > 0)
-    k_density32<<<nblocks(g.N), kBlock, 0, s>>>(xr, bcell, nm, cstart, g, P, PI, rho, em);
+  if (B.N > 0) k_density32<<<B.nblk, kBlock, 0, s>>>(xr, bcell, nm, cstart, B, PI, rho, em);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, Grid g, PhysF32 P, double2 *Xn, float2 *xrn, float2 *Vn,
+                       const int *cstart, const Batch &B, double2 *Xn, float2 *xrn, float2 *Vn,
                        double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out) {
-  if (g.N > 0)
-    k_force32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, g, P, Xn,
-                                              xrn, Vn, tapeX, tapeV, a_out, alpha_out);
+  if (B.N > 0)
+    k_force32<<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, B, Xn, xrn, Vn,
+                                        tapeX, tapeV, a_out, alpha_out);
 }
-void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, Grid g, PhysF32 P,
+void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
                       float2 *xr) {
-  if (g.N > 0) k_xrel<<<nblocks(g.N), kBlock, 0, s>>>(X, bcell, g, P, xr);
+  if (B.N > 0) k_xrel<<<B.nblk, kBlock, 0, s>>>(X, bcell, B, xr);
 }
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
-                         const float2 *V, const double2 *Xref, int check, Grid g, PhysF32 P,
-                         double dmax2, double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn,
-                         int *bcell, int *flags) {
-  if (g.N > 0)
-    k_permute32<<<nblocks(g.N), kBlock, 0, s>>>(perm, cellof, X, V, Xref, check, g, P, dmax2, Xn,
-                                                Vn, xr, Xrefn, bcell, flags);
+                         const float2 *V, const double2 *Xref, int check, const Batch &B,
+                         double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
+                         int *flags) {
+  if (B.N > 0)
+    k_permute32<<<B.nblk, kBlock, 0, s>>>(perm, cellof, X, V, Xref, check, B, Xn, Vn, xr, Xrefn,
+                                          bcell, flags);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, Grid g, PhysF32 P, float2 *PI, float2 *gX, float2 *gV,
+                       const int *cstart, const Batch &B, float2 *PI, float2 *gX, float2 *gV,
                        float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy) {
-  if (g.N > 0)
-    k_adj_r32<<<nblocks(g.N), kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, g, P, PI,
-                                              gX, gV, abar, galpha, em, tkey, tfxy);
+  if (B.N > 0)
+    k_adj_r32<<<B.nblk, kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, B, PI, gX, gV,
+                                        abar, galpha, em, tkey, tfxy);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
-                         const double *theta, Grid g, PhysF32 P, float2 *xr, float2 *gX,
+                         const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy) {
-  if (g.N > 0)
-    k_adj_rpw32<<<nblocks(g.N), kBlock, 0, s>>>(X, bcell, Vnext, theta, g, P, xr, gX, gV, abar,
-                                                galpha, tkey, tfxy);
+  if (B.N > 0)
+    k_adj_rpw32<<<B.nblk, kBlock, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
+                                          tkey, tfxy);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
                         const double *theta, const int *bcell, const uint32_t *nm, bool exact,
-                        const int *cstart, Grid g, PhysF32 P, float2 *gXs, float2 *gV,
+                        const int *cstart, const Batch &B, float2 *gXs, float2 *gV,
                         float *grho, uint32_t *em) {
-  if (g.N <= 0) return;
+  if (B.N <= 0) return;
   if (exact)
-    k_adj_p132<true><<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
-                                                     cstart, g, P, gXs, gV, grho, nullptr);
+    k_adj_p132<true><<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                               cstart, B, gXs, gV, grho, nullptr);
   else
-    k_adj_p132<false><<<nblocks(g.N), kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell,
-                                                      nm, cstart, g, P, gXs, gV, grho, em);
+    k_adj_p132<false><<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                                cstart, B, gXs, gV, grho, em);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *bcell, const uint32_t *nm, const int *cstart, Grid g, PhysF32 P,
+                        const int *bcell, const uint32_t *nm, const int *cstart, const Batch &B,
                         float2 *gX) {
-  if (g.N > 0)
-    k_adj_p232<<<nblocks(g.N), kBlock, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, g, P, gX);
+  if (B.N > 0) k_adj_p232<<<B.nblk, kBlock, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, B, gX);
 }
 
 }  // namespace sph

@@ -56,6 +56,24 @@ __device__ __forceinline__ int xcd_block() {
 
 // ---- integer helpers -------------------------------------------------------
 
+// Run heads of equal keys within a wavefront.  Lanes whose key is -1 are
+// inactive.  Returns the ballot of heads; `head_lane` is the head of this
+// lane's run and `run_len` is set on heads.
+__device__ __forceinline__ unsigned long long wave_runs(int key, int &head_lane, int &run_len) {
+  const int lane = threadIdx.x & 63;
+  int prev = __shfl_up(key, 1, 64);
+  bool head = (lane == 0) || (prev != key);
+  unsigned long long heads = __ballot(head);
+  // next head strictly above this lane (or 64)
+  unsigned long long above = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+  int next = above ? __ffsll((long long)above) - 1 : 64;
+  unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  head_lane = 63 - __clzll((long long)below);
+  run_len = next - lane;
+  return heads;
+}
+
+
 // unsafe_trunc(Int, floor(v)) + 2 clamped to [1, n], then made 0-based.
 // NaN / out-of-range inputs give the lowest cell, as x86-64's cvttsd2si
 // (typemin) followed by the clamp does in the reference.
@@ -177,10 +195,6 @@ void launch_theta_gather_f64(hipStream_t s, const double2 *X, const double *galp
 void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galpha,
                              const int *cstart, Grid g, ThetaGrid T, double4 *part,
                              double *gtheta);
-// the gather over per-slot interp-cell keys and FP64 (fx, fy) of the f32c
-// adjoint's pointwise pass
-void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, Grid g, ThetaGrid T,
-                                   double4 *part, double *gtheta);
+
 
 }  // namespace sph

@@ -15,7 +15,7 @@
 // Every contribution is the reference's product, bit for bit; only the order
 // of the sum is fixed.  FP64 throughout; compiled with -ffp-contract=off.
 #pragma clang fp contract(off)
-#include "sph_internal.h"
+#include "sph_f32c.h"
 
 namespace sph {
 
@@ -94,22 +94,29 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
   if (lane == 0) part[j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
 }
 
-// The same gather over per-slot keys written by the adjoint's pointwise pass:
-// key[p] = j0 + i0*(ngy-1) of the slot's interp cell and fxy[p] = (fx, fy) in
-// FP64, so that a candidate costs one 4-byte load and a compare; the
-// contributions and their order are those of k_theta_cells.
+// The same gather over per-slot keys written by the f32c adjoint's pointwise
+// pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
+// interp cell and fxy[p] = (fx, fy) in FP64, so that a candidate costs one
+// 4-byte load and a compare; the contributions and their order are those of
+// k_theta_cells.  Workgroups are dealt to the scenes of a batched state in
+// order (Scene32::tboff), kThetaLanes lanes per theta cell.
 template <class GA>
 __global__ void __launch_bounds__(kBlock)
 k_theta_cells_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
-                    const GA *__restrict__ galpha, const int *__restrict__ cstart, Grid g,
-                    ThetaGrid T, double4 *__restrict__ part) {
+                    const GA *__restrict__ galpha, const int *__restrict__ cstart, Batch B,
+                    double4 *__restrict__ part) {
+  const int blk = xcd_block();
+  const Scene32 &Sc = B.sc[B.tb2s[blk]];
+  const Grid g = Sc.g;
+  const ThetaGrid T = Sc.T;
+  const int coff = Sc.coff, tcoff = Sc.tcoff;
   const int ncx = T.ngx - 1, ncy = T.ngy - 1;
   const int lane = threadIdx.x & (kLanes - 1);
-  const int t = (xcd_block() * blockDim.x + threadIdx.x) / kLanes;
+  const int t = ((blk - Sc.tboff) * kBlock + (int)threadIdx.x) / kLanes;
   if (t >= ncx * ncy) return;      // the kLanes lanes of a group leave together
   const int i0 = t % ncx;
   const int j0 = t / ncx;
-  const int mykey = j0 + i0 * ncy;
+  const int mykey = tcoff + j0 + i0 * ncy;
   int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - 1;
   int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + 1;
   int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - 1;
@@ -120,7 +127,7 @@ k_theta_cells_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
   cy1 = cy1 > g.ny - 1 ? g.ny - 1 : cy1;
   double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
   for (int cy = cy0; cy <= cy1; ++cy) {
-    const int b = cstart[cy * g.nx + cx0], e = cstart[cy * g.nx + cx1 + 1];
+    const int b = cstart[coff + cy * g.nx + cx0], e = cstart[coff + cy * g.nx + cx1 + 1];
     for (int p = b + lane; p < e; p += kLanes) {
       if (key[p] != mykey) continue;
       const double2 f = fxy[p];
@@ -138,7 +145,26 @@ k_theta_cells_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
     s01 += __shfl_xor(s01, m, kLanes);
     s11 += __shfl_xor(s11, m, kLanes);
   }
-  if (lane == 0) part[j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
+  if (lane == 0) part[tcoff + j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
+}
+
+// node sums of the theta-cell partials, per scene (workgroups dealt in order
+// from Scene32::nboff)
+__global__ void k_theta_nodes_b(const double4 *__restrict__ part, Batch B,
+                                double *__restrict__ gtheta) {
+  const Scene32 &Sc = B.sc[B.nb2s[blockIdx.x]];
+  const int l = (blockIdx.x - Sc.nboff) * blockDim.x + threadIdx.x;
+  const int ngx = Sc.T.ngx, ngy = Sc.T.ngy;
+  if (l >= ngx * ngy) return;
+  const double4 *pt = part + Sc.tcoff;
+  const int j = l % ngy, i = l / ngy;
+  const int ncx = ngx - 1, ncy = ngy - 1;
+  double acc = 0.0;
+  if (i < ncx && j < ncy) acc += pt[j + i * ncy].x;                    // l00 corner
+  if (i > 0 && j < ncy) acc += pt[j + (i - 1) * ncy].y;                // l00 + ny
+  if (i < ncx && j > 0) acc += pt[(j - 1) + i * ncy].z;                // l00 + 1
+  if (i > 0 && j > 0) acc += pt[(j - 1) + (i - 1) * ncy].w;            // l00 + ny + 1
+  gtheta[Sc.toff + l] += acc;
 }
 
 __global__ void k_theta_nodes(const double4 *__restrict__ part, ThetaGrid T,
@@ -174,12 +200,11 @@ void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galph
 }
 
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, Grid g, ThetaGrid T,
+                                   const float *galpha, const int *cstart, const Batch &B,
                                    double4 *part, double *gtheta) {
-  const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
-  k_theta_cells_keyed<float><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(key, fxy, galpha, cstart,
-                                                                          g, T, part);
-  k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
+  static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
+  k_theta_cells_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, B, part);
+  k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta);
 }
 
 }  // namespace sph

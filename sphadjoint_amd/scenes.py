@@ -194,7 +194,11 @@ def batch_line_search(seed=0, nscenes=64, nmin=1000, nmax=64000, nsteps=300):
         nx = max(2, int(round(n / ny)))
         # the block spans half the tank width: nx*dx = 0.5
         dx = 0.5 / nx
-        p = wcsph_params(dx, 1.0, 0.6, 64, 32)
+        # viscosity: 0.05 (the dam-break scenes), capped so that the explicit
+        # viscous diffusion number nu*dt/h^2 stays at 0.037 or below
+        h, c = 1.2 * dx, 10.0 * math.sqrt(2.0 * G * 0.4)
+        nu = min(0.05, 0.037 * h * h / (0.25 * h / c))
+        p = wcsph_params(dx, 1.0, 0.6, 64, 32, nu=nu)
         X0 = lattice(nx, ny, dx) + rng.uniform(-1e-2 * dx, 1e-2 * dx, size=(2, nx * ny))
         V0 = np.zeros_like(X0)
         scenes.append(Scene(f"batch_{k}", p, X0, V0, theta0 + alphas[k] * d, nsteps))

@@ -1,0 +1,74 @@
+"""Batched scenes (one f32c state holding several independent scenes) on the GPU.
+
+Bar: every scene of a batch evolves, differentiates and scores exactly as the
+same scene run alone in its own f32c state -- X, V, J, gX, gV and gtheta bit
+for bit -- since each workgroup works inside one scene with that scene's
+constants and the sums keep their order.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import sphadjoint_amd as S
+from sphadjoint_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def be():
+    return S.MI355XBackend(0)
+
+
+def alone(be, sc, nsteps, seedV_scale):
+    pg = S.SPHParams(**sc.params)
+    st = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+    tape = S.Tape(be, sc.N, nsteps, pg, precision="f32c")
+    S.simulate_(st, sc.theta, pg, be, nsteps, tape=tape)
+    J, seed = S.target_objective(st, 0.3, 0.2, 0.1)
+    sv = seedV_scale * np.ones_like(seed)
+    ws = S.AdjointWorkspace(be, sc.N, pg, precision="f32c")
+    S.backward_(ws, tape, sc.theta, pg, be, seedX=seed, seedV=sv)
+    return st.X, st.V, J, ws.gX, ws.gV, ws.gtheta
+
+
+@pytest.mark.parametrize("nsteps", [1, 9, 30])
+def test_batch_equals_scenes_alone(be, nsteps):
+    say("batch of 4 vs alone, steps", nsteps)
+    scs = scenes.batch_line_search(seed=3, nscenes=4, nmin=300, nmax=5000, nsteps=nsteps)
+    bst = S.BatchState(be, [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs])
+    bth = bst.field([sc.theta for sc in scs])
+    tape = S.Tape(be, bst.N, nsteps, bst.p, precision="f32c")
+    S.simulate_(bst, bth, bst.p, be, nsteps, tape=tape)
+    J, seed = S.batch_objective(bst, 0.3, 0.2, 0.1)
+    ws = S.BatchWorkspace(be, bst)
+    S.backward_(ws, tape, bth, bst.p, be, seedX=seed, seedV=0.5 * np.ones_like(seed))
+    Xs, Vs, gXs, gVs = bst.split(bst.X), bst.split(bst.V), bst.split(ws.gX), bst.split(ws.gV)
+    gth = ws.gtheta
+    for k, sc in enumerate(scs):
+        X, V, Jk, gX, gV, gt = alone(be, sc, nsteps, 0.5)
+        np.testing.assert_array_equal(Xs[k], X, err_msg=f"X scene {k}")
+        np.testing.assert_array_equal(Vs[k], V, err_msg=f"V scene {k}")
+        assert J[k] == Jk
+        np.testing.assert_array_equal(gXs[k], gX, err_msg=f"gX scene {k}")
+        np.testing.assert_array_equal(gVs[k], gV, err_msg=f"gV scene {k}")
+        np.testing.assert_array_equal(gth[k], gt, err_msg=f"gtheta scene {k}")
+
+
+def test_batch_rejects_single_scene_only_calls(be):
+    say("batch API guards")
+    scs = scenes.batch_line_search(seed=1, nscenes=2, nmin=300, nmax=600, nsteps=2)
+    bst = S.BatchState(be, [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs])
+    with pytest.raises(Exception):
+        S.target_objective(bst, 0.3, 0.2, 0.1)
+    with pytest.raises(Exception):
+        bst.cl.arrays()
+    wrong = S.Field(be, np.zeros((5, 5)))
+    with pytest.raises(Exception):
+        S.step_(bst, wrong)
