@@ -173,6 +173,65 @@ class Runner:
         return t2 - t0, J, kern, extra
 
 
+def run_batched(nscenes, steps, seed=0):
+    """Batched line search (nscenes scenes, N log-uniform in [1k, 64k],
+    theta_k = theta0 + alpha_k d): one f32c state holds every scene.  After
+    the autotuner has picked the workgroup sizes (on a scratch state), times
+    forward + adjoint over a full tape, and forward-only candidate evaluation
+    (J per scene) run eagerly and replayed from a captured HIP graph."""
+    import sphadjoint_amd as S
+    from sphadjoint_amd import scenes
+    scs = scenes.batch_line_search(seed=seed, nscenes=nscenes, nsteps=steps)
+    be = S.MI355XBackend(0)
+    args = [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs]
+    scratch = S.BatchState(be, args)
+    th = scratch.field([sc.theta for sc in scs])
+    sizes = S.autotune_(scratch, th, steps=4)
+    del scratch
+    Ntot = sum(sc.N for sc in scs)
+    out = {"scenes": nscenes, "N_total": int(Ntot), "N_min": int(min(sc.N for sc in scs)),
+           "N_max": int(max(sc.N for sc in scs)), "steps": steps, "workgroup_sizes": sizes}
+
+    def fresh():
+        st = S.BatchState(be, args)
+        S.tuning(st, sizes)
+        return st
+
+    # forward + adjoint over a full tape (a warm-up pass, then the timed one)
+    for _ in range(2):
+        st = fresh()
+        tape = S.Tape(be, Ntot, steps, st.p, precision="f32c")
+        ws = S.BatchWorkspace(be, st)
+        zero = S.DeviceArray(be, N=Ntot)
+        be.synchronize()
+        t0 = time.perf_counter()
+        S.simulate_(st, th, st.p, be, steps, tape=tape)
+        J, seed_ = S.batch_objective(st, 0.3, 0.2, 0.1, device=True)
+        S.backward_(ws, tape, th, st.p, be, seedX=seed_, seedV=zero)
+        be.synchronize()
+        dt = time.perf_counter() - t0
+        del tape, ws
+    out["fwd_adj_s"] = round(dt, 5)
+    out["fwd_adj_scenes_per_s"] = round(nscenes / dt, 2)
+    out["fwd_adj_particle_steps_per_s"] = round(Ntot * steps / dt, 1)
+    # forward-only candidate evaluation: eager steps, then graph replays
+    for mode in ("eager", "graph"):
+        for _ in range(2):
+            st = fresh()
+            be.synchronize()
+            t0 = time.perf_counter()
+            if mode == "eager":
+                S.simulate_(st, th, st.p, be, steps)
+            else:
+                out["graph_launches"] = S.simulate_graph_(st, th, st.p, be, steps)
+            J, _ = S.batch_objective(st, 0.3, 0.2, 0.1)
+            be.synchronize()
+            dt = time.perf_counter() - t0
+        out[f"fwd_{mode}_scenes_per_s"] = round(nscenes / dt, 2)
+        out[f"fwd_{mode}_particle_steps_per_s"] = round(Ntot * steps / dt, 1)
+    return out
+
+
 def dominant(form, kern):
     table = ALG_BYTES[form.split("_")[0]]
     return max((k for k in kern if k in table), key=lambda k: kern[k]["total_ms"])
@@ -213,6 +272,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="f64,f32c_ckpt,f32c")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -257,6 +318,12 @@ def main():
     b = forms[best]
     roofline = roofline_of(best, b["timed"], N)
 
+    batched = None
+    if args.batch_scenes > 0:
+        log(f"batched line search: {args.batch_scenes} scenes, {args.batch_steps} steps")
+        batched = run_batched(args.batch_scenes, args.batch_steps)
+        log(f"batched: {batched}")
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} steps forward+adjoint on the oracle")
@@ -294,6 +361,7 @@ def main():
                             for f, r in forms.items() if f != "f64"}
                            if "f64" in forms else None),
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in b["kernels"].items()},
+        "batched_scenes_per_s": batched,
     }
     print(json.dumps(line), flush=True)
 

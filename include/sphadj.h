@@ -114,6 +114,12 @@ int sph_flags_clear(sph_state *st);
 int sph_step(sph_state *st, const sph_field *theta);
 /* simulate!(st, theta, p, backend, nsteps; tape) (src/driver.jl:92-115); tape may be NULL */
 int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape *tape);
+/* simulate! without a tape, replaying a captured HIP graph of two rebuild
+ * epochs (2*interval steps); results equal sph_simulate's bit for bit.  No
+ * counterpart in the reference (whose steps are individual kernel launches,
+ * src/forward.jl:161-173); for forward-only evaluations such as line-search
+ * candidates.  *replays (may be NULL) = graph launches made. */
+int sph_simulate_graph(sph_state *st, const sph_field *theta, int64_t nsteps, int64_t *replays);
 
 /* --- Tape (src/driver.jl:45-83) --- */
 int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out);
@@ -175,6 +181,18 @@ int sph_batch_objective(sph_state *st, double xc, double yc, double sigma, doubl
                         double *seed);
 int sph_batch_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
                             double *seed);
+
+/* Autotuner of the f32c form (no counterpart in the reference): picks the
+ * workgroup size (64, 128 or 256) of the neighbour-list build, density,
+ * force, adjoint R, P1 and P2 kernels by timing each candidate over `steps`
+ * steps with a tape and their reverse sweep, and keeps the choice with the
+ * state (tapes recorded afterwards inherit it).  Advances st by 6*steps
+ * steps.  chosen (may be NULL) receives the 6 sizes in that order. */
+int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *chosen);
+/* the tuned sizes of a state (6 entries, the order of sph_autotune), and
+ * their transfer to another state of the same kind */
+int sph_tuning_get(const sph_state *st, int32_t *sizes);
+int sph_tuning_set(sph_state *st, const int32_t *sizes);
 
 /* --- binomial checkpointing of the time loop.  No counterpart in the
  * reference, whose Tape keeps every step (src/driver.jl:38-65) and whose README

@@ -5,13 +5,14 @@ The public names mirror the reference's exports (src/SPHAdjoint.jl:39-40),
 with Julia's ``!`` written as a trailing underscore.
 """
 from .api import (AdjointWorkspace, BatchField, BatchState, BatchWorkspace, CellList,
-                  Checkpointer, DeviceArray, Field, MI355XBackend,
-                  NeighborList, SPHParams, State, Tape, backward_, build_, build_neighbors_,
-                  capacity, check_neighbor_flags, ckpt_plan, reset_, simulate_, step_,
-                  target_objective, batch_objective)
+                  Checkpointer, DeviceArray, Field, MI355XBackend, NeighborList, SPHParams,
+                  State, Tape, autotune_, backward_, batch_objective, build_, build_neighbors_,
+                  capacity, check_neighbor_flags, ckpt_plan, reset_, simulate_, simulate_graph_,
+                  step_, target_objective, tuning)
 
 __all__ = ["SPHParams", "State", "CellList", "NeighborList", "AdjointWorkspace", "Tape",
            "simulate_", "step_", "build_", "build_neighbors_", "backward_", "target_objective",
            "reset_", "capacity", "check_neighbor_flags", "MI355XBackend", "Field", "Checkpointer",
            "ckpt_plan", "DeviceArray", "BatchState", "BatchWorkspace", "BatchField",
-           "batch_objective"]
+           "batch_objective", "simulate_graph_", "autotune_",
+           "tuning"]

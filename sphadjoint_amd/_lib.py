@@ -55,6 +55,7 @@ SIGNATURES = {
     "sph_flags_clear": (C.c_int, [_vp]),
     "sph_step": (C.c_int, [_vp, _vp]),
     "sph_simulate": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "sph_simulate_graph": (C.c_int, [_vp, _vp, _i64, C.POINTER(_i64)]),
     "sph_tape_create": (C.c_int, [_vp, _i64, _i64, C.c_int, _pp]),
     "sph_tape_destroy": (C.c_int, [_vp]),
     "sph_tape_reset": (C.c_int, [_vp]),
@@ -80,6 +81,9 @@ SIGNATURES = {
     "sph_batch_info": (C.c_int, [_vp, C.POINTER(_i64), _vp]),
     "sph_batch_objective": (C.c_int, [_vp, _d, _d, _d, _vp, _vp]),
     "sph_batch_objective_dev": (C.c_int, [_vp, _d, _d, _d, _vp, _vp]),
+    "sph_autotune": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "sph_tuning_get": (C.c_int, [_vp, _vp]),
+    "sph_tuning_set": (C.c_int, [_vp, _vp]),
     "sph_ckpt_create": (C.c_int, [_vp, _PP, C.c_int, _i64, _i64, _i64, _i64, _d, _i64, _i64,
                                   _pp]),
     "sph_ckpt_destroy": (C.c_int, [_vp]),

@@ -404,6 +404,38 @@ def simulate_(st: State, theta, p: SPHParams, backend: MI355XBackend, nsteps, ta
     return st
 
 
+def simulate_graph_(st: State, theta, p: SPHParams, backend: MI355XBackend, nsteps):
+    """simulate! without a tape, its steps replayed from a captured HIP graph
+    (same results as simulate_).  Returns the number of graph launches."""
+    f = _field(theta, st.p, st.backend)
+    n = C.c_int64()
+    L.call("sph_simulate_graph", st._h, f._h, int(nsteps), C.byref(n))
+    check_neighbor_flags(st.nl)
+    return int(n.value)
+
+
+def autotune_(st: State, theta, steps=4):
+    """Picks the workgroup size of the f32c pair kernels on this state (advances
+    it by 6*steps steps).  Returns {kernel: workgroup size}."""
+    f = _field(theta, st.p, st.backend)
+    out = np.zeros(6, np.int32)
+    L.call("sph_autotune", st._h, f._h, int(steps), _ptr(out))
+    names = ("nl_build", "density", "force_integrate", "adjoint_r", "adjoint_p1", "adjoint_p2")
+    return dict(zip(names, out.tolist()))
+
+
+def tuning(st: State, sizes=None):
+    """The tuned workgroup sizes of an f32c state; with ``sizes`` (as returned
+    by autotune_ or tuning), sets them."""
+    names = ("nl_build", "density", "force_integrate", "adjoint_r", "adjoint_p1", "adjoint_p2")
+    if sizes is not None:
+        v = np.array([int(sizes[n]) for n in names], np.int32)
+        L.call("sph_tuning_set", st._h, _ptr(v))
+    out = np.zeros(6, np.int32)
+    L.call("sph_tuning_get", st._h, _ptr(out))
+    return dict(zip(names, out.tolist()))
+
+
 class AdjointWorkspace:
     """AdjointWorkspace (src/driver.jl:117-133): gX, gV, gtheta after backward_."""
 

@@ -162,12 +162,16 @@ ThetaGrid make_theta_grid(const sph_params &p, double rc) {
   return T;
 }
 
-// Device copy of an f32c state's scene table and workgroup -> scene maps.
+// Device copy of an f32c state's scene table and workgroup -> scene maps, for
+// each per-slot workgroup size, and the size each tuned kernel launches with.
 struct BatchStore {
-  Batch B{};
+  Batch B{};              // the 256-thread geometry (untuned kernels)
+  Batch Bv[3]{};          // per kSlotBlocks entry
+  int cfg[TK_COUNT] = {2, 2, 2, 2, 2, 2};
   std::vector<Scene32> host;
   Scene32 *sc = nullptr;
-  int *b2s = nullptr, *tb2s = nullptr, *nb2s = nullptr;
+  int *b2s[3] = {}, *boff[3] = {}, *tb2s = nullptr, *nb2s = nullptr;
+  const Batch &at(int k) const { return Bv[cfg[k]]; }
 };
 
 int blocks_of(long long n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -197,15 +201,19 @@ Scene32 make_scene32(const sph_params &p, int64_t N, double skin) {
 
 void batch_free(BatchStore &bs) {
   dfree(bs.sc);
-  dfree(bs.b2s);
+  for (int v = 0; v < 3; ++v) {
+    dfree(bs.b2s[v]);
+    dfree(bs.boff[v]);
+  }
   dfree(bs.tb2s);
   dfree(bs.nb2s);
 }
 
 // Lays the scenes out (slots, cells, design nodes and cells, workgroups in
-// order) and uploads the table and the workgroup maps.
+// order) and uploads the table and the workgroup maps; cfg keeps the tuned
+// workgroup sizes.
 void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
-  long long off = 0, coff = 0, toff = 0, tcoff = 0, boff = 0, tboff = 0, nboff = 0;
+  long long off = 0, coff = 0, toff = 0, tcoff = 0, tboff = 0, nboff = 0;
   for (auto &c : scenes) {
     const long long nodes = (long long)c.T.ngx * c.T.ngy;
     const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
@@ -213,52 +221,74 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     c.coff = (int)coff;
     c.toff = (int)toff;
     c.tcoff = (int)tcoff;
-    c.boff = (int)boff;
     c.tboff = (int)tboff;
     c.nboff = (int)nboff;
     off += c.g.N;
     coff += c.g.ncell;
     toff += nodes;
     tcoff += tcells;
-    boff += blocks_of(c.g.N);
     tboff += blocks_of(tcells * kThetaLanes);
     nboff += blocks_of(nodes);
   }
   if (off >= (1LL << 27)) fail(SPH_E_ARG, "the f32c form packs slot offsets in 27 bits: N=%lld", off);
   if (coff >= (1LL << 30)) fail(SPH_E_ARG, "too many cells in all scenes (%lld)", coff);
-  std::vector<int> b2s((size_t)boff), tb2s((size_t)tboff), nb2s((size_t)nboff);
-  for (int k = 0; k < (int)scenes.size(); ++k) {
+  const int nsc = (int)scenes.size();
+  std::vector<int> tb2s((size_t)tboff), nb2s((size_t)nboff);
+  for (int k = 0; k < nsc; ++k) {
     const Scene32 &c = scenes[k];
     const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
-    for (int b = 0; b < blocks_of(c.g.N); ++b) b2s[c.boff + b] = k;
     for (int b = 0; b < blocks_of(tcells * kThetaLanes); ++b) tb2s[c.tboff + b] = k;
     for (int b = 0; b < blocks_of((long long)c.T.ngx * c.T.ngy); ++b) nb2s[c.nboff + b] = k;
   }
+  int cfg[TK_COUNT];
+  std::copy(bs.cfg, bs.cfg + TK_COUNT, cfg);
   batch_free(bs);
+  std::copy(cfg, cfg + TK_COUNT, bs.cfg);
   bs.host = scenes;
   bs.sc = dalloc<Scene32>(scenes.size());
-  bs.b2s = dalloc<int>(b2s.size() + 1);
+  HIPC(hipMemcpy(bs.sc, scenes.data(), sizeof(Scene32) * scenes.size(), hipMemcpyHostToDevice));
   bs.tb2s = dalloc<int>(tb2s.size() + 1);
   bs.nb2s = dalloc<int>(nb2s.size() + 1);
-  HIPC(hipMemcpy(bs.sc, scenes.data(), sizeof(Scene32) * scenes.size(), hipMemcpyHostToDevice));
-  if (!b2s.empty()) HIPC(hipMemcpy(bs.b2s, b2s.data(), sizeof(int) * b2s.size(), hipMemcpyHostToDevice));
   if (!tb2s.empty())
     HIPC(hipMemcpy(bs.tb2s, tb2s.data(), sizeof(int) * tb2s.size(), hipMemcpyHostToDevice));
   if (!nb2s.empty())
     HIPC(hipMemcpy(bs.nb2s, nb2s.data(), sizeof(int) * nb2s.size(), hipMemcpyHostToDevice));
-  Batch &B = bs.B;
-  B.sc = bs.sc;
-  B.b2s = bs.b2s;
-  B.tb2s = bs.tb2s;
-  B.nb2s = bs.nb2s;
-  B.nscenes = (int)scenes.size();
-  B.nblk = (int)boff;
-  B.ntblk = (int)tboff;
-  B.nnblk = (int)nboff;
-  B.N = (int)off;
-  B.ncell = (int)coff;
-  B.ntnode = (int)toff;
-  B.ntcell = (int)tcoff;
+  for (int v = 0; v < 3; ++v) {
+    const int wg = kSlotBlocks[v];
+    std::vector<int> b2s, boff(nsc);
+    for (int k = 0; k < nsc; ++k) {
+      boff[k] = (int)b2s.size();
+      const int nb = (scenes[k].g.N + wg - 1) / wg;
+      for (int b = 0; b < nb; ++b) b2s.push_back(k);
+    }
+    bs.b2s[v] = dalloc<int>(b2s.size() + 1);
+    bs.boff[v] = dalloc<int>((size_t)nsc);
+    if (!b2s.empty())
+      HIPC(hipMemcpy(bs.b2s[v], b2s.data(), sizeof(int) * b2s.size(), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(bs.boff[v], boff.data(), sizeof(int) * nsc, hipMemcpyHostToDevice));
+    Batch &B = bs.Bv[v];
+    B.sc = bs.sc;
+    B.b2s = bs.b2s[v];
+    B.boff = bs.boff[v];
+    B.tb2s = bs.tb2s;
+    B.nb2s = bs.nb2s;
+    B.bs = wg;
+    B.nscenes = nsc;
+    B.nblk = (int)b2s.size();
+    B.ntblk = (int)tboff;
+    B.nnblk = (int)nboff;
+    B.N = (int)off;
+    B.ncell = (int)coff;
+    B.ntnode = (int)toff;
+    B.ntcell = (int)tcoff;
+  }
+  bs.B = bs.Bv[2];
+}
+
+// the tape keeps its own copy of a state's scene table (same tuned sizes)
+void batch_copy(BatchStore &dst, const BatchStore &src) {
+  std::copy(src.cfg, src.cfg + TK_COUNT, dst.cfg);
+  batch_build(dst, src.host);
 }
 
 void check_params(const sph_params *p) {
@@ -413,6 +443,10 @@ struct sph_state {
   int *nl_idx = nullptr, *nl_cnt = nullptr;
   // staging
   double2 *tmp2 = nullptr, *seed_slot = nullptr;
+  // captured forward cycle (two rebuild epochs) of sph_simulate_graph
+  hipGraphExec_t gexec = nullptr;
+  const double *gtheta_d = nullptr;   // design field the graph reads
+  std::vector<const void *> groles;   // buffer roles at the start of the captured cycle
   double *tmp1 = nullptr;
   double *obj_part = nullptr, *obj_J = nullptr;
   int obj_nb = 0;
@@ -488,6 +522,8 @@ void state_free(sph_state *st) {
   dfree(st->nm);
   dfree(st->em);
   batch_free(st->bs);
+  if (st->gexec) (void)hipGraphExecDestroy(st->gexec);
+  st->gexec = nullptr;
   double **d1[] = {&st->alpha_id, &st->rho_id, &st->alpha_slot, &st->rho_slot, &st->theta_last,
                    &st->tmp1, &st->obj_part, &st->obj_J};
   for (auto pp : d1) dfree(*pp);
@@ -523,10 +559,10 @@ void materialise_aux(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
-    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->bs.B, st->PI32,
+    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->bs.at(TK_DENSITY), st->PI32,
                         st->rho_slot, st->em);
     launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->em,
-                      st->cstart, st->bs.B, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      st->cstart, st->bs.at(TK_FORCE), nullptr, nullptr, nullptr, nullptr, nullptr,
                       st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
@@ -567,7 +603,8 @@ void do_build_neighbors(sph_state *st) {
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C)
-      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.B, st->nm, st->flags);
+      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
+                           st->flags);
     else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
@@ -589,7 +626,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
     tp->rc = st->kc.cs;
     tp->kc = st->kc;
     tp->ncell = st->g.ncell;
-    if (st->precision == SPH_F32C) batch_build(tp->bs, st->bs.host);
+    if (st->precision == SPH_F32C) batch_copy(tp->bs, st->bs);
     tp->has_geom = true;
   } else if (tp->ncell != st->g.ncell || tp->g.nx != st->g.nx) {
     fail(SPH_E_ARG, "tape already holds steps of a state with another cell grid");
@@ -639,12 +676,13 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->bs.B, PI32, nullptr,
+      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->bs.at(TK_DENSITY), PI32,
+                          nullptr,
                           st->em);
     });
     timed(st->ctx, K_FORCE, [&] {
       launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->em, st->cstart,
-                        st->bs.B, st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
+                        st->bs.at(TK_FORCE), st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
     std::swap(st->xr, st->xrn);
@@ -1212,6 +1250,77 @@ int sph_step(sph_state *st, const sph_field *theta) {
   });
 }
 
+// simulate! without a tape, the steps replayed from a captured HIP graph.
+// Two rebuild epochs (2*interval steps, the first one a rebuild) bring every
+// buffer role back to where it started, so one captured cycle replays as is;
+// the steps before the first rebuild and after the last whole cycle run
+// eagerly.  The graph holds kernel launches only (no memset or copy nodes) and
+// is kept with the state until the design field or the buffer roles change.
+int sph_simulate_graph(sph_state *st, const sph_field *theta, int64_t nsteps, int64_t *replays) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (nsteps < 0) fail(SPH_E_ARG, "nsteps must be >= 0");
+    copy_theta(st, theta);
+    hipStream_t s = st->ctx->stream;
+    int64_t nrep = 0;
+    auto eager = [&](int64_t n) {
+      for (int64_t k = 0; k < n; ++k) do_step(st, theta->d, nullptr);
+      nsteps -= n;
+    };
+    // up to a rebuild that checks displacements (a first build does not)
+    if (st->age == kAgeNever) eager(std::min<int64_t>(nsteps, st->interval));
+    if (st->age < st->interval) eager(std::min<int64_t>(nsteps, st->interval - st->age));
+    const int64_t cyc = 2 * st->interval;
+    // every buffer whose role a step or a rebuild swaps
+    auto roles = [&]() {
+      return std::vector<const void *>{st->X, st->Xn, st->V, st->Vn, st->V32, st->V32n, st->xr,
+                                       st->xrn, st->Xref, st->Xref2, st->ids, st->new_ids,
+                                       st->cstart, st->cl_cstart};
+    };
+    if (nsteps >= cyc && !st->ctx->prof) {
+      if (!st->gexec || st->gtheta_d != theta->d || st->groles != roles()) {
+        if (st->gexec) HIPC(hipGraphExecDestroy(st->gexec));
+        st->gexec = nullptr;
+        hipGraph_t graph = nullptr;
+        const std::vector<const void *> start = roles();
+        HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int64_t k = 0; k < cyc; ++k) do_step(st, theta->d, nullptr);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        HIPC(hipStreamEndCapture(s, &graph));
+        if (roles() != start) {
+          (void)hipGraphDestroy(graph);
+          fail(SPH_E_STATE, "a captured cycle did not return the buffers to their roles");
+        }
+        hipError_t e = hipGraphInstantiate(&st->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPC(e);
+        st->gtheta_d = theta->d;
+        st->groles = start;
+        // the capture advanced the host-side bookkeeping by one cycle; the
+        // first launch brings the device there
+        HIPC(hipGraphLaunch(st->gexec, s));
+        nsteps -= cyc;
+        nrep += 1;
+      }
+      while (nsteps >= cyc) {
+        HIPC(hipGraphLaunch(st->gexec, s));
+        st->epoch += 2;
+        nsteps -= cyc;
+        nrep += 1;
+      }
+    }
+    eager(nsteps);
+    if (replays) *replays = nrep;
+    HIPC(hipStreamSynchronize(s));
+    prof_collect(st->ctx);
+  });
+}
+
 int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape *tape) {
   return guarded([&] {
     if (!st) fail(SPH_E_ARG, "NULL argument");
@@ -1550,7 +1659,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const bool f32c = ws->precision == SPH_F32C;
   const Grid g = tape->g;
   const PhysF64 P = make_phys(ws->p);
-  const Batch &B = tape->bs.B;   // f32c form
+  const Batch &B = tape->bs.B;   // f32c form: untuned launches
+  const BatchStore &BS = tape->bs;
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
   int cur = tape->entry_epoch[tape->n];
   // the adjoint state into the slot order of the final entry
@@ -1585,7 +1695,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
           launch_xrel_f32c(s, X0, E.scell, B, ws->xr32);
-          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, B, ws->nm, ws->flags);
+          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
+                               ws->flags);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);
@@ -1617,11 +1728,11 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
-          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, B, ws->xr32, ws->gX32,
+          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32, ws->gX32,
                               ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
-          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, B,
+          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, BS.at(TK_ADJ_R),
                             ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em,
                             ws->tkey, ws->tfxy);
         }
@@ -1635,14 +1746,17 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         // stored (pterm, invrho) P1 tests the pairs and writes it for P2
         if (tape->keep_pi)
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             nm, false, E.cstart, B, ws->gXs32, ws->gV32, ws->grho32, ws->em);
+                             nm, false, E.cstart, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
+                             ws->grho32, ws->em);
         else
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             ws->em, true, E.cstart, B, ws->gXs32, ws->gV32, ws->grho32,
+                             ws->em, true, E.cstart, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
+                             ws->grho32,
                              nullptr);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, B,
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart,
+                           BS.at(TK_ADJ_P2),
                            ws->gX32);
       });
     } else {
@@ -2131,6 +2245,100 @@ void ck_free(sph_ckpt *ck) {
 }  // namespace
 
 extern "C" {
+
+// the tuned workgroup sizes of an f32c state (TK_COUNT entries)
+int sph_tuning_get(const sph_state *st, int32_t *sizes) {
+  return guarded([&] {
+    if (!st || !sizes) fail(SPH_E_ARG, "NULL argument");
+    if (st->precision != SPH_F32C) fail(SPH_E_ARG, "the f32c form has tuned kernels");
+    for (int k = 0; k < TK_COUNT; ++k) sizes[k] = kSlotBlocks[st->bs.cfg[k]];
+  });
+}
+
+int sph_tuning_set(sph_state *st, const int32_t *sizes) {
+  return guarded([&] {
+    if (!st || !sizes) fail(SPH_E_ARG, "NULL argument");
+    if (st->precision != SPH_F32C) fail(SPH_E_ARG, "the f32c form has tuned kernels");
+    int v[TK_COUNT];
+    for (int k = 0; k < TK_COUNT; ++k) {
+      v[k] = -1;
+      for (int j = 0; j < 3; ++j)
+        if (sizes[k] == kSlotBlocks[j]) v[k] = j;
+      if (v[k] < 0) fail(SPH_E_ARG, "workgroup size %d is not 64, 128 or 256", (int)sizes[k]);
+    }
+    for (int k = 0; k < TK_COUNT; ++k) st->bs.cfg[k] = v[k];
+  });
+}
+
+// Workgroup size of each tuned f32c kernel: for every candidate size, a
+// warm-up pass and a timed pass of `steps` steps with a tape plus the reverse
+// sweep over them (zero seeds: the same instruction stream), every kernel
+// timed with events; each kernel keeps its fastest size.  Advances st by
+// 6*steps steps.  chosen (may be NULL) receives the TK_COUNT sizes.
+int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *chosen) {
+  sph_tape *tape = nullptr;
+  sph_workspace *ws = nullptr;
+  int rc = guarded([&] {
+    if (!st || !theta) fail(SPH_E_ARG, "NULL argument");
+    if (st->precision != SPH_F32C) fail(SPH_E_ARG, "the autotuner tunes the f32c kernels");
+    if (steps < 1) fail(SPH_E_ARG, "steps must be >= 1");
+    sph_ctx *c = st->ctx;
+    if (c->prof) fail(SPH_E_STATE, "turn profiling off before tuning");
+    int r = sph_tape_create(c, st->N, steps, SPH_F32C, &tape);
+    if (r != SPH_OK) throw Error(r, g_err);
+    workspace_create_impl(c, st->N, &st->p, SPH_F32C, st->ntnode,
+                          st->bs.B.nscenes > 1 ? st->bs.B.ntcell
+                                               : (st->p.ngx - 1) * (st->p.ngy - 1),
+                          &ws);
+    std::vector<double> zeros((size_t)st->N * 2, 0.0);
+    const int kid[TK_COUNT] = {K_NLBUILD, K_DENSITY, K_FORCE, K_ADJ_R, K_ADJ_P1, K_ADJ_P2};
+    double best[TK_COUNT];
+    int pick[TK_COUNT];
+    for (int k = 0; k < TK_COUNT; ++k) {
+      best[k] = 1e300;
+      pick[k] = 2;
+    }
+    for (int v = 0; v < 3; ++v) {
+      for (int k = 0; k < TK_COUNT; ++k) st->bs.cfg[k] = v;
+      for (int pass = 0; pass < 2; ++pass) {
+        HIPC(hipStreamSynchronize(c->stream));
+        prof_collect(c);
+        for (int k = 0; k < K_COUNT; ++k) {
+          c->tot_ms[k] = 0.0;
+          c->launches[k] = 0;
+        }
+        c->prof = pass == 1;
+        // the reverse sweep launches with the tape's copy of the sizes
+        if (tape->has_geom) batch_copy(tape->bs, st->bs);
+        r = sph_tape_reset(tape);
+        if (r == SPH_OK) r = sph_simulate(st, theta, steps, tape);
+        if (r == SPH_OK)
+          r = sph_backward(ws, tape, theta, zeros.data(), zeros.data(), st->skin, st->interval,
+                           st->maxnb, SPH_LAYOUT_SLOT);
+        c->prof = false;
+        if (r != SPH_OK) throw Error(r, g_err);
+      }
+      for (int k = 0; k < TK_COUNT; ++k) {
+        const int64_t n = c->launches[kid[k]];
+        if (n == 0) continue;
+        const double t = c->tot_ms[kid[k]] / (double)n;
+        if (t < best[k]) {
+          best[k] = t;
+          pick[k] = v;
+        }
+      }
+    }
+    for (int k = 0; k < TK_COUNT; ++k) {
+      st->bs.cfg[k] = pick[k];
+      if (chosen) chosen[k] = kSlotBlocks[pick[k]];
+      c->tot_ms[kid[k]] = 0.0;
+      c->launches[kid[k]] = 0;
+    }
+  });
+  if (ws) sph_workspace_destroy(ws);
+  if (tape) sph_tape_destroy(tape);
+  return rc;
+}
 
 int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3) {
   return guarded([&] {

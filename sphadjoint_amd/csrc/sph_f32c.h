@@ -20,29 +20,35 @@ struct PhysF32 {
 // numbering (cstart holds global slots), the design-field nodes
 // [toff, toff + ngx*ngy) and the design cells [tcoff, tcoff + (ngx-1)*(ngy-1)).
 // Every per-slot launch covers all scenes at once: its workgroups are dealt to
-// the scenes in order (boff = the scene's first workgroup), so a workgroup
-// never straddles two scenes and reads its scene's constants once, through
-// scalar loads.
+// the scenes in order (Batch::boff[k] = scene k's first workgroup), so a
+// workgroup never straddles two scenes and reads its scene's constants once,
+// through scalar loads.
 struct Scene32 {
   Grid g;                  // the scene's own cell grid; g.N = its particle count
   PhysF32 P;
   ThetaGrid T;
   double cs64;             // cell width in FP64 (binning by the division rule)
   double dmax2;            // squared displacement allowed between rebuilds (skin*h)^2
-  int off, coff, boff;     // first slot, first cell, first workgroup of slot launches
+  int off, coff;           // first slot, first cell
   int toff, tcoff, tboff;  // first theta node, first theta cell, first workgroup of
                            // theta-cell launches (kThetaLanes lanes per cell)
   int nboff;               // first workgroup of theta-node launches
 };
 
 // Launch geometry of an f32c state: device scene table, workgroup -> scene
-// maps of the three launch shapes, and totals.
+// maps of the three launch shapes, and totals.  Per-slot launches use bs
+// threads per workgroup (one of kSlotBlocks; the autotuner picks it per
+// kernel), with b2s and boff built for that size.
 struct Batch {
   const Scene32 *sc;
-  const int *b2s, *tb2s, *nb2s;
+  const int *b2s, *boff, *tb2s, *nb2s;
+  int bs;
   int nscenes, nblk, ntblk, nnblk;
   int N, ncell, ntnode, ntcell;
 };
+constexpr int kSlotBlocks[3] = {64, 128, 256};
+// kernels whose workgroup size is tuned (sph_autotune)
+enum TunedKernel { TK_NLBUILD, TK_DENSITY, TK_FORCE, TK_ADJ_R, TK_ADJ_P1, TK_ADJ_P2, TK_COUNT };
 constexpr int kThetaLanes = 8;   // lanes that share one theta cell in the gtheta gather
 
 // bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)

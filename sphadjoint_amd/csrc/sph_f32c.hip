@@ -165,8 +165,9 @@ __device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
 // scene's last slot leave); g and P are the scene's grid and constants.
 #define SCENE_SLOT(B)                                                   \
   const int blk_ = xcd_block();                                         \
-  const Scene32 S = (B).sc[(B).b2s[blk_]];                              \
-  const int sl_ = (blk_ - S.boff) * kBlock + (int)threadIdx.x;          \
+  const int scn_ = (B).b2s[blk_];                                       \
+  const Scene32 S = (B).sc[scn_];                                       \
+  const int sl_ = (blk_ - (B).boff[scn_]) * (B).bs + (int)threadIdx.x;  \
   if (sl_ >= S.g.N) return;                                             \
   const int s = S.off + sl_;                                            \
   const Grid &g = S.g;                                                  \
@@ -465,8 +466,9 @@ __global__ void __launch_bounds__(kBlock)
 k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
            int *__restrict__ counts) {
   const int blk = blockIdx.x;     // all lanes of the workgroup take part in the run search
-  const Scene32 S = B.sc[B.b2s[blk]];
-  const int sl = (blk - S.boff) * kBlock + (int)threadIdx.x;
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const int sl = (blk - B.boff[scn]) * B.bs + (int)threadIdx.x;
   int c = -1;
   if (sl < S.g.N) {
     const int s = S.off + sl;
@@ -640,35 +642,35 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
 
 void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
                         int *counts) {
-  if (B.N > 0) k_assign32<<<B.nblk, kBlock, 0, s>>>(X, B, cellof, counts);
+  if (B.N > 0) k_assign32<<<B.nblk, B.bs, 0, s>>>(X, B, cellof, counts);
 }
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags) {
-  if (B.N > 0) k_nm_build32<<<B.nblk, kBlock, 0, s>>>(xr, bcell, cstart, B, nm, flags);
+  if (B.N > 0) k_nm_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, B, nm, flags);
 }
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
                          const int *cstart, const Batch &B, float2 *PI, double *rho,
                          uint32_t *em) {
-  if (B.N > 0) k_density32<<<B.nblk, kBlock, 0, s>>>(xr, bcell, nm, cstart, B, PI, rho, em);
+  if (B.N > 0) k_density32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, nm, cstart, B, PI, rho, em);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const Batch &B, double2 *Xn, float2 *xrn, float2 *Vn,
                        double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out) {
   if (B.N > 0)
-    k_force32<<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, B, Xn, xrn, Vn,
+    k_force32<<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, B, Xn, xrn, Vn,
                                         tapeX, tapeV, a_out, alpha_out);
 }
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
                       float2 *xr) {
-  if (B.N > 0) k_xrel<<<B.nblk, kBlock, 0, s>>>(X, bcell, B, xr);
+  if (B.N > 0) k_xrel<<<B.nblk, B.bs, 0, s>>>(X, bcell, B, xr);
 }
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                          const float2 *V, const double2 *Xref, int check, const Batch &B,
                          double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
                          int *flags) {
   if (B.N > 0)
-    k_permute32<<<B.nblk, kBlock, 0, s>>>(perm, cellof, X, V, Xref, check, B, Xn, Vn, xr, Xrefn,
+    k_permute32<<<B.nblk, B.bs, 0, s>>>(perm, cellof, X, V, Xref, check, B, Xn, Vn, xr, Xrefn,
                                           bcell, flags);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
@@ -676,14 +678,14 @@ void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                        const int *cstart, const Batch &B, float2 *PI, float2 *gX, float2 *gV,
                        float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy) {
   if (B.N > 0)
-    k_adj_r32<<<B.nblk, kBlock, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, B, PI, gX, gV,
+    k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, B, PI, gX, gV,
                                         abar, galpha, em, tkey, tfxy);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy) {
   if (B.N > 0)
-    k_adj_rpw32<<<B.nblk, kBlock, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
+    k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
                                           tkey, tfxy);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
@@ -693,16 +695,16 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
                         float *grho, uint32_t *em) {
   if (B.N <= 0) return;
   if (exact)
-    k_adj_p132<true><<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+    k_adj_p132<true><<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
                                                cstart, B, gXs, gV, grho, nullptr);
   else
-    k_adj_p132<false><<<B.nblk, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+    k_adj_p132<false><<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
                                                 cstart, B, gXs, gV, grho, em);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const Batch &B,
                         float2 *gX) {
-  if (B.N > 0) k_adj_p232<<<B.nblk, kBlock, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, B, gX);
+  if (B.N > 0) k_adj_p232<<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, B, gX);
 }
 
 }  // namespace sph

@@ -72,3 +72,28 @@ def test_batch_rejects_single_scene_only_calls(be):
     wrong = S.Field(be, np.zeros((5, 5)))
     with pytest.raises(Exception):
         S.step_(bst, wrong)
+
+
+def test_autotune_keeps_results(be):
+    say("autotune: any workgroup size gives the same numbers")
+    sc = scenes.brinkman_56x37(nsteps=0)
+    pg = S.SPHParams(**sc.params)
+    ref = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+    st = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
+    chosen = S.autotune_(st, sc.theta, steps=4)
+    assert set(chosen.values()) <= {64, 128, 256}
+    S.simulate_(ref, sc.theta, pg, be, 24)          # the tuner advanced st by 24 steps
+    np.testing.assert_array_equal(st.X, ref.X)
+    np.testing.assert_array_equal(st.V, ref.V)
+    # forward + adjoint with the tuned sizes equal the default ones
+    outs = []
+    for s_ in (ref, st):
+        tape = S.Tape(be, sc.N, 12, pg, precision="f32c")
+        S.simulate_(s_, sc.theta, pg, be, 12, tape=tape)
+        J, seed = S.target_objective(s_, 0.3, 0.2, 0.1)
+        ws = S.AdjointWorkspace(be, sc.N, pg, precision="f32c")
+        S.backward_(ws, tape, sc.theta, pg, be, seedX=seed, seedV=np.zeros_like(seed))
+        outs.append((J, ws.gX, ws.gV, ws.gtheta))
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        np.testing.assert_array_equal(a, b)
