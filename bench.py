@@ -71,27 +71,32 @@ def log(*a):
 def cpu_baseline(sc, steps, threads):
     """The oracle (C restatement of the reference's CPU path, OpenMP over
     particles) on a bounded sample of the same workload: `steps` steps forward
-    with a tape, then the adjoint over them."""
+    with a tape, then the adjoint over them, in Float64 (`value`) and Float32."""
     from oracle import oracle as O
     if threads:
         O.set_threads(threads)
-    p = O.SPHParams(**sc.params)
-    st = O.State(sc.X0, sc.V0, p)
-    tape = O.Tape(st.N, steps, p)
-    t0 = time.perf_counter()
-    O.simulate_(st, sc.theta, p, steps, tape)
-    t1 = time.perf_counter()
-    J, seed = O.target_objective(st.X, XC, YC, SIGMA)
-    ws = O.AdjointWorkspace(st.N, p)
-    O.backward_(ws, tape, sc.theta, p, seedX=seed, seedV=np.zeros_like(seed))
-    t2 = time.perf_counter()
-    N = st.N
+    rates = {}
+    for T in (np.float64, np.float32):
+        p = O.SPHParams(T=T, **sc.params)
+        st = O.State(sc.X0, sc.V0, p)
+        tape = O.Tape(st.N, steps, p)
+        t0 = time.perf_counter()
+        O.simulate_(st, sc.theta, p, steps, tape)
+        t1 = time.perf_counter()
+        J, seed = O.target_objective(st.X, XC, YC, SIGMA, T=T)
+        ws = O.AdjointWorkspace(st.N, p)
+        O.backward_(ws, tape, sc.theta, p, seedX=seed, seedV=np.zeros_like(seed))
+        t2 = time.perf_counter()
+        rates[T] = (st.N * steps / (t2 - t0), st.N * steps / (t1 - t0), t2 - t0)
+    v64, f64, s64 = rates[np.float64]
+    v32, f32, s32 = rates[np.float32]
     return {
-        "value": N * steps / (t2 - t0), "unit": "particle-steps/s", "cores": O.get_threads(),
-        "kind": "port",
-        "sample": f"oracle FP64 (the reference's CPU algorithm restated in C, OpenMP over "
-                  f"particles), the same 1M scene, first {steps} steps forward + adjoint "
-                  f"({t2 - t0:.1f} s); forward alone {N * steps / (t1 - t0):.4g} p-s/s",
+        "value": v64, "unit": "particle-steps/s", "cores": O.get_threads(), "kind": "port",
+        "sample": f"oracle (the reference's CPU algorithm restated in C, OpenMP over particles), "
+                  f"the same 1M scene, first {steps} steps forward + adjoint: Float64 "
+                  f"{v64:.4g} p-s/s ({s64:.1f} s; forward alone {f64:.4g}), Float32 {v32:.4g} "
+                  f"p-s/s ({s32:.1f} s; forward alone {f32:.4g})",
+        "float32_value": v32,
     }
 
 
@@ -268,7 +273,7 @@ def main():
     ap.add_argument("--snapshots", type=int, default=64, help="checkpoint snapshot budget")
     ap.add_argument("--seg-epochs", type=int, default=0,
                     help="epochs per checkpoint segment (0: fewest that fit the budget)")
-    ap.add_argument("--cpu-steps", type=int, default=64, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=100, help="steps of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="f64,f32c_ckpt,f32c")

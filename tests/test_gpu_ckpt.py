@@ -191,3 +191,40 @@ def test_device_memory_variants(be):
         S.backward_(ws, tape, sc.theta, pg, be, seedX=seedt, seedV=zero)
         assert Jt == host[0]
         np.testing.assert_array_equal(ws.gtheta, host[4])
+
+
+@pytest.mark.slow
+def test_long_horizon_5333_steps(be):
+    """N=2072 Brinkman scene, 5333 steps: the f32c form under a 64-snapshot
+    budget gives J bit-identical over 4 repeated runs, and its gradients stay
+    within the FP32-compensated form's error of a full-tape FP64 run."""
+    say("long horizon 5333 steps, 64 snapshots, f32c x4 vs f64 full tape")
+    n = 5333
+    sc = scenes.brinkman_56x37(nsteps=n)
+    pg = S.SPHParams(**sc.params)
+    # FP64 full tape
+    st = S.State(be, sc.X0, sc.V0, pg)
+    tape = S.Tape(be, sc.N, n, pg)
+    S.simulate_(st, sc.theta, pg, be, n, tape=tape)
+    J64, seed = S.target_objective(st, 0.6, 0.2, 0.1)
+    ws = S.AdjointWorkspace(be, sc.N, pg)
+    S.backward_(ws, tape, sc.theta, pg, be, seedX=seed, seedV=np.zeros_like(seed))
+    ref = (ws.gX, ws.gV, ws.gtheta)
+    del tape
+    ck = S.Checkpointer(be, sc.N, n, pg, snapshots=64, precision="f32c")
+    wsf = S.AdjointWorkspace(be, sc.N, pg, precision="f32c")
+    runs = []
+    for _ in range(4):
+        stf = ck.forward(sc.X0, sc.V0, sc.theta)
+        J, sd = S.target_objective(stf, 0.6, 0.2, 0.1)
+        ck.backward(wsf, seedX=sd, seedV=np.zeros_like(sd))
+        runs.append((J, wsf.gX, wsf.gV, wsf.gtheta))
+    for r in runs[1:]:
+        assert r[0] == runs[0][0]
+        for a, b in zip(r[1:], runs[0][1:]):
+            np.testing.assert_array_equal(a, b)
+    eJ = abs(runs[0][0] - J64) / abs(J64)
+    e = [float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(runs[0][1:], ref)]
+    print("   J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
+    assert eJ < 1e-4
+    assert e[2] < 1e-2
