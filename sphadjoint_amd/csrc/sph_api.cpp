@@ -424,6 +424,7 @@ struct sph_state {
   int *nl = nullptr, *nbcnt = nullptr;
   uint32_t *nm = nullptr;
   uint32_t *em = nullptr;  // f32c form: exact list of the last density pass
+  int *lane = nullptr;     // f32c form: thread -> slot order of the pair passes
   double2 *X = nullptr, *V = nullptr, *Xn = nullptr, *Vn = nullptr;
   double2 *Xref = nullptr, *Xref2 = nullptr;
   double2 *PI = nullptr;
@@ -495,6 +496,7 @@ struct sph_workspace {
   int *nl = nullptr, *nbcnt = nullptr, *flags = nullptr;  // neighbour list of the epoch
   uint32_t *nm = nullptr;                                  // f32c form: row bitmasks
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
+  int *lane = nullptr;                                     // f32c form: pair-pass thread order
   int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
   double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
@@ -512,7 +514,7 @@ namespace {
 void state_free(sph_state *st) {
   int **ints[] = {&st->ids, &st->scell, &st->cstart, &st->cellof, &st->counts, &st->cl_cstart,
                   &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
-                  &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt};
+                  &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt, &st->lane};
   for (auto pp : ints) dfree(*pp);
   double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
                     &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
@@ -559,11 +561,11 @@ void materialise_aux(sph_state *st) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   if (st->precision == SPH_F32C) {
-    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->bs.at(TK_DENSITY), st->PI32,
-                        st->rho_slot, st->em);
+    launch_density_f32c(s, st->xrn, st->scell, st->nm, st->cstart, st->lane,
+                        st->bs.at(TK_DENSITY), st->PI32, st->rho_slot, st->em);
     launch_force_f32c(s, st->Xn, st->xrn, st->V32n, st->PI32, st->theta_last, st->scell, st->em,
-                      st->cstart, st->bs.at(TK_FORCE), nullptr, nullptr, nullptr, nullptr, nullptr,
-                      st->a_slot, st->alpha_slot);
+                      st->cstart, st->lane, st->bs.at(TK_FORCE), nullptr, nullptr, nullptr,
+                      nullptr, nullptr, st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
     launch_density_f64(s, st->Xn, st->nl, st->nbcnt, st->g, st->P, st->PI, st->rho_slot, nullptr);
@@ -602,10 +604,11 @@ void do_build_neighbors(sph_state *st) {
   std::swap(st->cstart, st->cl_cstart);
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
-    if (st->precision == SPH_F32C)
+    if (st->precision == SPH_F32C) {
       launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
                            st->flags);
-    else
+      launch_lane_order_f32c(s, st->xr, st->bs.B, st->lane);
+    } else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
   });
@@ -676,13 +679,13 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->bs.at(TK_DENSITY), PI32,
-                          nullptr,
-                          st->em);
+      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->lane,
+                          st->bs.at(TK_DENSITY), PI32, nullptr, st->em);
     });
     timed(st->ctx, K_FORCE, [&] {
       launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->em, st->cstart,
-                        st->bs.at(TK_FORCE), st->Xn, st->xrn, st->V32n, tX, tV32, nullptr, nullptr);
+                        st->lane, st->bs.at(TK_FORCE), st->Xn, st->xrn, st->V32n, tX, tV32,
+                        nullptr, nullptr);
     });
     std::swap(st->X, st->Xn);
     std::swap(st->xr, st->xrn);
@@ -1005,6 +1008,7 @@ void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const in
       if (f32c) {
         st->nm = dalloc<uint32_t>(n * 3 * kMaskWords);
         st->em = dalloc<uint32_t>(n * 3 * kMaskWords);
+        st->lane = dalloc<int>(n);
       } else {
         st->nl = dalloc<int>(n * (size_t)maxnb);
         st->nbcnt = dalloc<int>(n);
@@ -1468,6 +1472,7 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->nl); dfree(ws->nbcnt); dfree(ws->flags); dfree(ws->ids_cur); dfree(ws->iota);
   dfree(ws->nm);
   dfree(ws->em);
+  dfree(ws->lane);
   dfree(ws->tkey);
   dfree(ws->tfxy);
 }
@@ -1635,6 +1640,7 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
   if (ws->precision == SPH_F32C) {
     if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
+    if (!ws->lane) ws->lane = dalloc<int>((size_t)ws->N);
     if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);
     if (!ws->tfxy) ws->tfxy = dalloc<double2>((size_t)ws->N);
     return;
@@ -1651,7 +1657,7 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
 // tape's first entry, whose ids are copied to ws->ids_cur.  When `lnl` is
 // given it is the already built neighbour list of the tape's last epoch.
 void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int *ids_in,
-            int maxnb, const int *lnl, const int *lcnt, const uint32_t *lnm) {
+            int maxnb, const int *lnl, const int *lcnt, const uint32_t *lnm, const int *llane) {
   hipStream_t s = ws->ctx->stream;
   sph_ctx *c = ws->ctx;
   const int N = (int)ws->N;
@@ -1683,6 +1689,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const int *nl = use_given ? lnl : ws->nl;
   const int *nbcnt = use_given ? lcnt : ws->nbcnt;
   const uint32_t *nm = use_given ? lnm : ws->nm;
+  const int *lane = use_given ? llane : ws->lane;
   int listed = use_given ? (int)tape->epochs.size() - 1 : -1;  // epoch whose list `nl` holds
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
@@ -1697,6 +1704,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
           launch_xrel_f32c(s, X0, E.scell, B, ws->xr32);
           launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
                                ws->flags);
+          launch_lane_order_f32c(s, ws->xr32, B, ws->lane);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);
@@ -1706,6 +1714,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       nl = ws->nl;
       nbcnt = ws->nbcnt;
       nm = ws->nm;
+      lane = ws->lane;
     }
     if (f32c) {
       const float2 *Vk = tape->V32 + off;
@@ -1732,9 +1741,9 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
                               ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
-          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, BS.at(TK_ADJ_R),
-                            ws->PI32, ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->em,
-                            ws->tkey, ws->tfxy);
+          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
+                            BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
+                            ws->galpha32, ws->em, ws->tkey, ws->tfxy);
         }
       });
       timed(c, K_THETA, [&] {
@@ -1746,18 +1755,16 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         // stored (pterm, invrho) P1 tests the pairs and writes it for P2
         if (tape->keep_pi)
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             nm, false, E.cstart, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
+                             nm, false, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
                              ws->grho32, ws->em);
         else
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             ws->em, true, E.cstart, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
-                             ws->grho32,
-                             nullptr);
+                             ws->em, true, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
+                             ws->grho32, nullptr);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart,
-                           BS.at(TK_ADJ_P2),
-                           ws->gX32);
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, lane,
+                           BS.at(TK_ADJ_P2), ws->gX32);
       });
     } else {
       const double2 *Vk = tape->V + off;
@@ -1862,7 +1869,8 @@ int backward_impl(sph_workspace *ws, sph_tape *tape, const sph_field *theta, con
     hipStream_t s = ws->ctx->stream;
     ensure_ws_list(ws, maxnb);
     bw_seed(ws, seedX, seedV, dev);
-    if (tape->n > 0) bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr, nullptr);
+    if (tape->n > 0)
+      bw_run(ws, tape, theta, ws->ids_cur, (int)maxnb, nullptr, nullptr, nullptr, nullptr);
     bw_finish(ws);
     HIPC(hipStreamSynchronize(s));
     prof_collect(ws->ctx);
@@ -2226,7 +2234,7 @@ void ck_bind(sph_ckpt *ck) {
     // the work state has not stepped since the epoch was taped: its list is
     // the epoch's list
     bw_run(ck->ws, ck->mini, ck->theta, ck->ws->ids_cur, (int)ck->maxnb, ck->work->nl,
-           ck->work->nbcnt, ck->work->nm);
+           ck->work->nbcnt, ck->work->nm, ck->work->lane);
   };
 }
 

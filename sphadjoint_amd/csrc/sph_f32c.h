@@ -58,16 +58,20 @@ void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *ce
                         int *counts);
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags);
+// thread -> slot order of the pair passes for the positions xr of a rebuild
+// (B: the 256-slot geometry, Bv[2]); every pair launcher takes it as `lane`
+void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane);
 // density + EOS over the mask list nm; when em is given, the exact list of
 // these positions (pairs with r2 < (2h)^2) is written there for the force pass
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, const Batch &B, float2 *PI, double *rho,
-                         uint32_t *em);
+                         const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                         double *rho, uint32_t *em);
 // force + integrator over the exact list nm of the density pass
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, const Batch &B, double2 *Xn, float2 *xrn, float2 *Vn,
-                       double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out);
+                       const int *cstart, const int *lane, const Batch &B, double2 *Xn,
+                       float2 *xrn, float2 *Vn, double2 *tapeX, float2 *tapeV, double2 *a_out,
+                       double *alpha_out);
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
                       float2 *xr);
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
@@ -78,8 +82,9 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 // keyed gtheta gather
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, const Batch &B, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy);
+                       const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                       float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
+                       int *tkey, double2 *tfxy);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy);
@@ -88,12 +93,12 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
                         const double *theta, const int *bcell, const uint32_t *nm, bool exact,
-                        const int *cstart, const Batch &B, float2 *gXs, float2 *gV,
-                        float *grho, uint32_t *em);
+                        const int *cstart, const int *lane, const Batch &B, float2 *gXs,
+                        float2 *gV, float *grho, uint32_t *em);
 // P2 over the exact list nm
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *bcell, const uint32_t *nm, const int *cstart, const Batch &B,
-                        float2 *gX);
+                        const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
+                        const Batch &B, float2 *gX);
 // deterministic gtheta gather over the keys of the R passes (sph_theta.hip)
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
                                    const float *galpha, const int *cstart, const Batch &B,

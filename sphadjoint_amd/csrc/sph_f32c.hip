@@ -24,6 +24,8 @@
 // follows src/forward.jl:25-147 and src/adjoint.jl:30-228.
 #include "sph_f32c.h"
 
+#include <stdexcept>
+
 // No implicit contraction: every FMA below is written out, so the rounding of
 // each expression is fixed by the source and identical in every kernel that
 // evaluates it (the checkpointed and full-tape reverse sweeps must agree bit
@@ -173,6 +175,26 @@ __device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
   const Grid &g = S.g;                                                  \
   const PhysF32 &P = S.P
 
+// Pair passes: thread sl_ of the scene takes the slot lane[off + sl_] (see
+// k_lane_order), a permutation of each aligned window of kLaneWin slots of the
+// scene, so that the lanes of a wavefront sweep rows of similar length; each
+// slot's arithmetic is unchanged by this assignment.  w0_..w1_ is the window
+// of this workgroup's slots.  Threads past the scene's end (valid_ false) take
+// part in the workgroup's barriers and then leave.
+constexpr int kLaneWin = 256;
+#define SCENE_WIN(B, lane)                                                             \
+  const int blk_ = xcd_block();                                                        \
+  const int scn_ = (B).b2s[blk_];                                                      \
+  const Scene32 S = (B).sc[scn_];                                                      \
+  const int bsl_ = (blk_ - (B).boff[scn_]) * (B).bs;                                   \
+  const int sl_ = bsl_ + (int)threadIdx.x;                                             \
+  const bool valid_ = sl_ < S.g.N;                                                     \
+  const int w0_ = S.off + (bsl_ / kLaneWin) * kLaneWin;                                \
+  const int w1_ = min(w0_ + kLaneWin, S.off + S.g.N);                                  \
+  const int s = valid_ ? (lane)[S.off + sl_] : w0_;                                    \
+  const Grid &g = S.g;                                                                 \
+  const PhysF32 &P = S.P
+
 // descriptors of the list and the per-slot arrays of one state
 #define NB_RSRC                                                                \
   const Rsrc nmr = rsrc(nm, (uint32_t)B.N * (uint32_t)(3 * kMaskWords) * 4u);  \
@@ -212,16 +234,23 @@ __device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &
 // Runs longer than kMaskRun slots are cut there and raise the overflow flag.
 constexpr int kMaskRun = 32 * kMaskWords;
 
+// Loader index of a neighbour: slot j of row r is passed to the loader as
+// j + ro.o[r].  Global loads use ro = 0 (the index is the slot); staged
+// passes use the LDS position of the row's staged segment minus its first
+// slot (stage_rowoff).
+struct RowOff {
+  int o[3];
+};
+
 // Calls fn(d, dx, dy, r2) for every neighbour flagged in the mask list m_in
 // of slot s (own-frame position xi, binning cell bc), in the reference's sweep
-// order, where d = ld(o8) holds the neighbour's per-slot data, o8 = 8*j being
-// its byte offset into the 8-byte per-slot arrays (d.x must be its xr).  fn
-// returns whether the pair is a true neighbour (r2 < (2h)^2); when em is given,
-// those pairs form a mask list of the same layout, written to em, that later
-// passes over the same positions iterate without testing r2.  The loads of the
-// next neighbour are issued before the current pair is evaluated (two
-// registers sets in turn), so a word exposes one memory round trip, not one per
-// pair.
+// order, where d = ld(i) holds the neighbour's per-slot data (d.x must be its
+// xr), i being its loader index (RowOff).  fn returns whether the pair is a
+// true neighbour (r2 < (2h)^2); when em is given, those pairs form a mask list
+// of the same layout, written to em, that later passes over the same
+// positions iterate without testing r2.  The loads of the next neighbour are
+// issued before the current pair is evaluated (two registers sets in turn),
+// so a word exposes one load latency, not one per pair.
 // one pair of for_each_nb: bit b of a word whose cells cx-1 / cx / cx+1 start
 // at bits <0 / s0 / s1; xm, x0, xp are the own x in those cells' frames
 template <class D, class Fn>
@@ -236,7 +265,8 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
 template <class Ld, class Fn>
 __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
                                             int bc, float2 xi, const Grid &g, int coff,
-                                            uint32_t Ntot, float cs, Ld &&ld, Fn &&fn) {
+                                            uint32_t Ntot, float cs, const RowOff &ro, Ld &&ld,
+                                            Fn &&fn) {
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
@@ -253,21 +283,21 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
       uint32_t hit = 0;
       if (m != 0) {
         const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
-        const uint32_t wb8 = (uint32_t)(run.rb + 32 * w) * 8u;
+        const int wb = run.rb + 32 * w + ro.o[r];
         int b0 = __builtin_ctz(m);
         m &= m - 1;
-        auto d0 = ld(wb8 + (uint32_t)b0 * 8u);
+        auto d0 = ld(wb + b0);
         for (;;) {
           const bool more1 = m != 0;
           const int b1 = more1 ? __builtin_ctz(m) : b0;
           m &= m - 1;
-          auto d1 = ld(wb8 + (uint32_t)b1 * 8u);
+          auto d1 = ld(wb + b1);
           nb_pair(d0, b0, s0, s1, xm, xi.x, xp, yi, hit, fn);
           if (!more1) break;
           const bool more0 = m != 0;
           b0 = more0 ? __builtin_ctz(m) : b1;
           m &= m - 1;
-          d0 = ld(wb8 + (uint32_t)b0 * 8u);
+          d0 = ld(wb + b0);
           nb_pair(d1, b1, s0, s1, xm, xi.x, xp, yi, hit, fn);
           if (!more0) break;
         }
@@ -275,6 +305,85 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
       if (em) em[widx] = hit;
     }
   }
+}
+
+// ---- LDS staging of the neighbour rows of a window --------------------------
+// A workgroup's slots form one window of consecutive slots, hence a
+// consecutive range of cells: cells (xa, ya) .. (xb, yb) in row-major order.
+// Every sweep of the window reads, for each own cell row y' (ya..yb) and each
+// r = 0..2, cells max(x0-1,0)..min(x1+1,nx-1) of row y'+r-1, x0..x1 being the
+// window's own cells in row y'; that cell range is one slot range (segment
+// (y'-ya)*3 + r).  The segments are copied into LDS with coalesced loads and
+// the pair loop then reads its neighbours there.  A window spanning more than
+// kStageOwnRows cell rows, or whose segments exceed kStageCap slots, reads
+// from global memory instead (same results: only the loader differs).
+constexpr int kStageCap = 1024;
+constexpr int kStageOwnRows = 4;
+constexpr int kStageSegs = 3 * kStageOwnRows;
+
+struct StageTab {
+  int lo[kStageSegs];         // first slot of each segment
+  int off[kStageSegs + 1];    // LDS position of each segment; off[kStageSegs] = total
+  int ya, ok;
+};
+
+// Plans the staging of the window of slots [w0, w1) (w1 > w0); every thread
+// of the workgroup calls it.  Wave 0 forms the segments and their prefix.
+__device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ bcell, Rsrc cst,
+                                           int w0, int w1, const Grid &g, int coff) {
+  if (threadIdx.x < 64) {
+    const int k = (int)threadIdx.x;
+    const int2 a = cell_xy(bcell[w0], g), b = cell_xy(bcell[w1 - 1], g);
+    const int nrow = b.y - a.y + 1;
+    int len = 0, lo = 0;
+    if (k < kStageSegs && k < 3 * nrow) {
+      const int yo = a.y + k / 3, y = yo + k % 3 - 1;
+      if (y >= 0 && y < g.ny) {
+        const int x0 = yo == a.y ? a.x : 0, x1 = yo == b.y ? b.x : g.nx - 1;
+        const int xlo = x0 > 0 ? x0 - 1 : 0, xhi = x1 < g.nx - 1 ? x1 + 1 : g.nx - 1;
+        const int base = coff + y * g.nx;
+        lo = (int)ldu(cst, (uint32_t)(base + xlo) * 4u);
+        len = (int)ldu(cst, (uint32_t)(base + xhi + 1) * 4u) - lo;
+      }
+    }
+    int inc = len;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int t = __shfl_up(inc, d, 64);
+      if (k >= d) inc += t;
+    }
+    const int total = __shfl(inc, kStageSegs - 1, 64);
+    if (k < kStageSegs) {
+      T.lo[k] = lo;
+      T.off[k] = inc - len;
+    }
+    if (k == 0) {
+      T.off[kStageSegs] = total;
+      T.ya = a.y;
+      T.ok = (nrow <= kStageOwnRows && total <= kStageCap) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+}
+
+// Copies the planned segments: cp(i, j) stores slot j's data at LDS index i.
+// Called by every thread of the workgroup when T.ok.
+template <class Cp>
+__device__ __forceinline__ void stage_copy(const StageTab &T, Cp &&cp) {
+  for (int k = 0; k < kStageSegs; ++k) {
+    const int o = T.off[k], n = T.off[k + 1] - o, lo = T.lo[k];
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) cp(o + i, lo + i);
+  }
+  __syncthreads();
+}
+
+// loader offsets of slot s (binning cell bc) when its window is staged
+__device__ __forceinline__ RowOff stage_rowoff(const StageTab &T, int bc, const Grid &g) {
+  const int k = (cell_xy(bc, g).y - T.ya) * 3;
+  RowOff r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) r.o[i] = T.off[k + i] - T.lo[k + i];
+  return r;
 }
 
 // neighbour data of the pair passes
@@ -334,14 +443,57 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   if (over) flags[0] = 1;
 }
 
-__device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, Rsrc xrr, int s,
-                                               int bc, float2 xi, const Grid &g, int coff,
-                                               uint32_t Ntot, const PhysF32 &P) {
+// Thread -> slot assignment of the pair passes.  A pair pass walks the three
+// cell rows of a slot's 3x3 block one after the other, so a wavefront spends
+// on each row the longest row count of its 64 lanes.  How a slot's neighbours
+// split over the rows depends on its height in its cell (near the bottom most
+// lie in the row below, near the top in the row above), and consecutive slots
+// of a cell climb through it (ids ascend with y on a lattice).  Each aligned
+// window of kLaneWin slots of a scene is therefore dealt to its threads in
+// order of that height (16 classes, then slot order), so that each wavefront
+// holds slots of similar height and similar row counts.  The order is fixed
+// (ballot ranks, no atomics) and changes which thread evaluates a slot, never
+// what is evaluated.
+constexpr int kLaneClasses = 16;
+
+__global__ void __launch_bounds__(kLaneWin)
+k_lane_order(const float2 *__restrict__ xr, Batch B, int *__restrict__ lane) {
+  __shared__ int wcnt[kLaneWin / 64][kLaneClasses];
+  const int blk = blockIdx.x;
+  const int scn = B.b2s[blk];
+  const Scene32 &S = B.sc[scn];
+  const int base = S.off + (blk - B.boff[scn]) * kLaneWin;
+  const int n = min(kLaneWin, S.off + S.g.N - base);
+  const int t = (int)threadIdx.x, w = t >> 6, l = t & 63;
+  int key = kLaneClasses;   // threads past the window's end take no class
+  if (t < n) {
+    const float y = xr[base + t].y * ((float)kLaneClasses / S.P.cs);
+    key = !(y > 0.f) ? 0 : (y >= (float)(kLaneClasses - 1) ? kLaneClasses - 1 : (int)y);
+  }
+  int below = 0;   // lanes of this wavefront below this one with the same class
+  for (int c = 0; c < kLaneClasses; ++c) {
+    const unsigned long long m = __ballot(key == c);
+    if (l == 0) wcnt[w][c] = __popcll(m);
+    if (key == c) below = __popcll(m & ((1ull << l) - 1ull));
+  }
+  __syncthreads();
+  if (t < n) {
+    int pos = below;
+    for (int c = 0; c < key; ++c)
+      for (int v = 0; v < kLaneWin / 64; ++v) pos += wcnt[v][c];
+    for (int v = 0; v < w; ++v) pos += wcnt[v][key];
+    lane[base + pos] = base + t;
+  }
+}
+
+template <class Ld>
+__device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, int s, int bc,
+                                               float2 xi, const Grid &g, int coff, uint32_t Ntot,
+                                               const PhysF32 &P, const RowOff &ro, Ld &&ld) {
   // Fast2Sum from the self term, the largest term of the sum: every later
   // term is positive and smaller, so each step's error term is exact
   FSum acc(P.mA);
-  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs,
-              [&](uint32_t o8) __attribute__((always_inline)) { return NbX{ld2(xrr, o8)}; },
+  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
@@ -357,6 +509,20 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
   return make_float2(P.rho0 + drho, drho);
 }
 
+// density over the staged rows when the window fits, else over global loads
+__device__ __forceinline__ float2 density_slot(Rsrc nmr, uint32_t *em, Rsrc cst, Rsrc xrr,
+                                               const StageTab &T, const float2 *sx, int s, int bc,
+                                               float2 xi, const Grid &g, int coff, uint32_t Ntot,
+                                               const PhysF32 &P) {
+  if (T.ok)
+    return density_pair(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, stage_rowoff(T, bc, g),
+                        [&](int i) __attribute__((always_inline)) { return NbX{sx[i]}; });
+  return density_pair(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, RowOff{},
+                      [&](int i) __attribute__((always_inline)) {
+                        return NbX{ld2(xrr, (uint32_t)i * 8u)};
+                      });
+}
+
 
 __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
   float inv = 1.f / rd.x;
@@ -365,11 +531,18 @@ __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
 
 __global__ void __launch_bounds__(kBlock)
 k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-            const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
-            float2 *__restrict__ PI, double *__restrict__ rho_out, uint32_t *__restrict__ em) {
-  SCENE_SLOT(B);
+            const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
+            const int *__restrict__ lane, Batch B, float2 *__restrict__ PI,
+            double *__restrict__ rho_out, uint32_t *__restrict__ em) {
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap];
+  SCENE_WIN(B, lane);
   NB_RSRC;
-  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, S.coff, B.N, P);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  if (T.ok) stage_copy(T, [&](int i, int j) { sx[i] = xr[j]; });
+  if (!valid_) return;
+  const int bc = bcell[s];
+  float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -378,23 +551,29 @@ __global__ void __launch_bounds__(kBlock)
 k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ V, const float2 *__restrict__ PI,
           const double *__restrict__ theta, const int *__restrict__ bcell,
-          const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
-          double2 *__restrict__ Xn, float2 *__restrict__ xrn, float2 *__restrict__ Vn,
-          double2 *__restrict__ tapeX, float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
+          const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
+          const int *__restrict__ lane, Batch B, double2 *__restrict__ Xn,
+          float2 *__restrict__ xrn, float2 *__restrict__ Vn, double2 *__restrict__ tapeX,
+          float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
           double *__restrict__ alpha_out) {
-  SCENE_SLOT(B);
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap], spi[kStageCap], sv[kStageCap];
+  SCENE_WIN(B, lane);
+  NB_RSRC;
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  if (T.ok)
+    stage_copy(T, [&](int i, int j) {
+      sx[i] = xr[j];
+      spi[i] = PI[j];
+      sv[i] = V[j];
+    });
+  if (!valid_) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
   const int bc = bcell[s];
   const float tmi = P.twomumass * pii.y;
-  NB_RSRC;
-  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
-  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   CSum ax, ay;
-  // nm is the exact list of the density pass over these positions
-  for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs,
-              [&](uint32_t o8) __attribute__((always_inline)) { return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)}; },
-              [&, P](const NbXPV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+  auto pair = [&, P](const NbXPV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
     float u = fmaf(-0.5f, q, 1.f);
     float F = P.Fc * u * u * u;
@@ -405,7 +584,22 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
     ax.add(fmaf(C, vi.x - vj.x, -(PF * dx)));
     ay.add(fmaf(C, vi.y - vj.y, -(PF * dy)));
     return true;
-  });
+  };
+  // nm is the exact list of the density pass over these positions
+  if (T.ok) {
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+                [&](int i) __attribute__((always_inline)) { return NbXPV{sx[i], spi[i], sv[i]}; },
+                pair);
+  } else {
+    const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+    const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+                [&](int i) __attribute__((always_inline)) {
+                  const uint32_t o8 = (uint32_t)i * 8u;
+                  return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)};
+                },
+                pair);
+  }
   InterpF ic = interp_cell(x64.x, x64.y, P);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float axv = ax.value() + (P.gx + wall_accel((float)x64.x, P.Lx, P.kw));
@@ -489,14 +683,20 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const float2 *__restrict__ Vnext, const double *__restrict__ theta,
           const int *__restrict__ bcell, const uint32_t *__restrict__ nm,
-          const int *__restrict__ cstart, Batch B, float2 *__restrict__ PI,
-          float2 *__restrict__ gX, float2 *__restrict__ gV, float2 *__restrict__ abar,
-          float *__restrict__ galpha, uint32_t *__restrict__ em, int *__restrict__ tkey,
-          double2 *__restrict__ tfxy) {
-  SCENE_SLOT(B);
-  const double2 x64 = X[s];
+          const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
+          float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
+          float2 *__restrict__ abar, float *__restrict__ galpha, uint32_t *__restrict__ em,
+          int *__restrict__ tkey, double2 *__restrict__ tfxy) {
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap];
+  SCENE_WIN(B, lane);
   NB_RSRC;
-  float2 rd = density_pair(nmr, em, cst, xrr, s, bcell[s], xr[s], g, S.coff, B.N, P);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  if (T.ok) stage_copy(T, [&](int i, int j) { sx[i] = xr[j]; });
+  if (!valid_) return;
+  const double2 x64 = X[s];
+  const int bc = bcell[s];
+  float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
@@ -546,23 +746,28 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const float2 *__restrict__ V, const float2 *__restrict__ PI,
            const float2 *__restrict__ abar, const float *__restrict__ galpha,
            const double *__restrict__ theta, const int *__restrict__ bcell,
-           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
-           float2 *__restrict__ gXs, float2 *__restrict__ gV, float *__restrict__ grho,
-           uint32_t *__restrict__ em) {
-  SCENE_SLOT(B);
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
+           const int *__restrict__ lane, Batch B, float2 *__restrict__ gXs,
+           float2 *__restrict__ gV, float *__restrict__ grho, uint32_t *__restrict__ em) {
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap], spi[kStageCap], sab[kStageCap], sv[kStageCap];
+  SCENE_WIN(B, lane);
+  NB_RSRC;
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  if (T.ok)
+    stage_copy(T, [&](int i, int j) {
+      sx[i] = xr[j];
+      spi[i] = PI[j];
+      sab[i] = abar[j];
+      sv[i] = V[j];
+    });
+  if (!valid_) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const float pti = pii.x, iri = pii.y;
-  NB_RSRC;
-  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
-  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
-  const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
+  const int bc = bcell[s];
   ASum gx, gy, gvx, gvy, grh, sumS;
-  for_each_nb(nmr, kExact ? nullptr : em, cst, s, bcell[s], xi, g, S.coff, B.N, P.cs,
-              [&](uint32_t o8) __attribute__((always_inline)) {
-                return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
-              },
-              [&, P](const NbXPAV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+  auto pair = [&, P](const NbXPAV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     if (kExact || r2 < P.r2max) {
       float rinv = __builtin_amdgcn_rsqf(r2);
       float q = (r2 * rinv) * P.invh;
@@ -593,7 +798,25 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
       return true;
     }
     return false;
-  });
+  };
+  uint32_t *emw = kExact ? nullptr : em;
+  if (T.ok) {
+    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+                [&](int i) __attribute__((always_inline)) {
+                  return NbXPAV{sx[i], spi[i], sab[i], sv[i]};
+                },
+                pair);
+  } else {
+    const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+    const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
+    const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
+    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+                [&](int i) __attribute__((always_inline)) {
+                  const uint32_t o8 = (uint32_t)i * 8u;
+                  return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
+                },
+                pair);
+  }
   float Ssum = sumS.value();
   float grv = grh.value() + (-2.f * P.m * pti * iri) * Ssum + (P.c2 * P.m * iri * iri) * Ssum;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
@@ -614,25 +837,44 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 __global__ void __launch_bounds__(kBlock)
 k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
            const float2 *__restrict__ gXs, const int *__restrict__ bcell,
-           const uint32_t *__restrict__ nm, const int *__restrict__ cstart, Batch B,
-           float2 *__restrict__ gX) {
-  SCENE_SLOT(B);
+           const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
+           const int *__restrict__ lane, Batch B, float2 *__restrict__ gX) {
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap];
+  __shared__ float sgr[kStageCap];
+  SCENE_WIN(B, lane);
+  NB_RSRC;
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  if (T.ok)
+    stage_copy(T, [&](int i, int j) {
+      sx[i] = xr[j];
+      sgr[i] = grho[j];
+    });
+  if (!valid_) return;
   const float2 xi = xr[s];
   const float gri = grho[s];
-  NB_RSRC;
-  const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
+  const int bc = bcell[s];
   ASum gx, gy;
-  // nm is the exact list of P1 over these positions
-  for_each_nb(nmr, nullptr, cst, s, bcell[s], xi, g, S.coff, B.N, P.cs,
-              [&](uint32_t o8) __attribute__((always_inline)) { return NbXG{ld2(xrr, o8), ld1(grr, o8 >> 1)}; },
-              [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
+  auto pair = [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
     float u = fmaf(-0.5f, q, 1.f);
     float wgt = P.m * (P.Fc * u * u * u) * (gri + d.gr);
     gx.add(wgt * dx);
     gy.add(wgt * dy);
     return true;
-  });
+  };
+  // nm is the exact list of P1 over these positions
+  if (T.ok) {
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+                [&](int i) __attribute__((always_inline)) { return NbXG{sx[i], sgr[i]}; }, pair);
+  } else {
+    const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+                [&](int i) __attribute__((always_inline)) {
+                  return NbXG{ld2(xrr, (uint32_t)i * 8u), ld1(grr, (uint32_t)i * 4u)};
+                },
+                pair);
+  }
   float2 gs = gXs[s];
   float2 go = gX[s];
   gX[s] = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
@@ -648,18 +890,23 @@ void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, con
                           const Batch &B, uint32_t *nm, int *flags) {
   if (B.N > 0) k_nm_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, B, nm, flags);
 }
+void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane) {
+  if (B.bs != kLaneWin) throw std::logic_error("lane order needs the 256-slot geometry");
+  if (B.N > 0) k_lane_order<<<B.nblk, kLaneWin, 0, s>>>(xr, B, lane);
+}
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
-                         const int *cstart, const Batch &B, float2 *PI, double *rho,
-                         uint32_t *em) {
-  if (B.N > 0) k_density32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, nm, cstart, B, PI, rho, em);
+                         const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                         double *rho, uint32_t *em) {
+  if (B.N > 0) k_density32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, nm, cstart, lane, B, PI, rho, em);
 }
 void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                        const float2 *PI, const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, const Batch &B, double2 *Xn, float2 *xrn, float2 *Vn,
-                       double2 *tapeX, float2 *tapeV, double2 *a_out, double *alpha_out) {
+                       const int *cstart, const int *lane, const Batch &B, double2 *Xn,
+                       float2 *xrn, float2 *Vn, double2 *tapeX, float2 *tapeV, double2 *a_out,
+                       double *alpha_out) {
   if (B.N > 0)
-    k_force32<<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, B, Xn, xrn, Vn,
-                                        tapeX, tapeV, a_out, alpha_out);
+    k_force32<<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, theta, bcell, nm, cstart, lane, B, Xn, xrn,
+                                        Vn, tapeX, tapeV, a_out, alpha_out);
 }
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
                       float2 *xr) {
@@ -675,11 +922,12 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
-                       const int *cstart, const Batch &B, float2 *PI, float2 *gX, float2 *gV,
-                       float2 *abar, float *galpha, uint32_t *em, int *tkey, double2 *tfxy) {
+                       const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                       float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
+                       int *tkey, double2 *tfxy) {
   if (B.N > 0)
-    k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, B, PI, gX, gV,
-                                        abar, galpha, em, tkey, tfxy);
+    k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, lane, B, PI, gX,
+                                        gV, abar, galpha, em, tkey, tfxy);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
@@ -691,20 +939,20 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
                         const double *theta, const int *bcell, const uint32_t *nm, bool exact,
-                        const int *cstart, const Batch &B, float2 *gXs, float2 *gV,
-                        float *grho, uint32_t *em) {
+                        const int *cstart, const int *lane, const Batch &B, float2 *gXs,
+                        float2 *gV, float *grho, uint32_t *em) {
   if (B.N <= 0) return;
   if (exact)
     k_adj_p132<true><<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
-                                               cstart, B, gXs, gV, grho, nullptr);
+                                               cstart, lane, B, gXs, gV, grho, nullptr);
   else
     k_adj_p132<false><<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
-                                                cstart, B, gXs, gV, grho, em);
+                                                cstart, lane, B, gXs, gV, grho, em);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
-                        const int *bcell, const uint32_t *nm, const int *cstart, const Batch &B,
-                        float2 *gX) {
-  if (B.N > 0) k_adj_p232<<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, B, gX);
+                        const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
+                        const Batch &B, float2 *gX) {
+  if (B.N > 0) k_adj_p232<<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, lane, B, gX);
 }
 
 }  // namespace sph
