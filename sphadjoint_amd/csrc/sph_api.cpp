@@ -171,6 +171,7 @@ struct BatchStore {
   std::vector<Scene32> host;
   Scene32 *sc = nullptr;
   int *b2s[3] = {}, *boff[3] = {}, *tb2s = nullptr, *nb2s = nullptr;
+  int2 *trange = nullptr;
   const Batch &at(int k) const { return Bv[cfg[k]]; }
 };
 
@@ -207,6 +208,7 @@ void batch_free(BatchStore &bs) {
   }
   dfree(bs.tb2s);
   dfree(bs.nb2s);
+  dfree(bs.trange);
 }
 
 // Lays the scenes out (slots, cells, design nodes and cells, workgroups in
@@ -214,7 +216,13 @@ void batch_free(BatchStore &bs) {
 // workgroup sizes.
 void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
   long long off = 0, coff = 0, toff = 0, tcoff = 0, tboff = 0, nboff = 0;
+  std::vector<int2> trange;
   for (auto &c : scenes) {
+    // SPH cell ranges of the theta columns, then of the theta rows
+    c.troff = (int)trange.size();
+    const int ncx = c.T.ngx - 1, ncy = c.T.ngy - 1;
+    for (int i = 0; i < ncx; ++i) trange.push_back(theta_cell_range(i, ncx, c.T.hx, c.T.cs, c.g.nx));
+    for (int j = 0; j < ncy; ++j) trange.push_back(theta_cell_range(j, ncy, c.T.hy, c.T.cs, c.g.ny));
     const long long nodes = (long long)c.T.ngx * c.T.ngy;
     const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
     c.off = (int)off;
@@ -227,7 +235,7 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     coff += c.g.ncell;
     toff += nodes;
     tcoff += tcells;
-    tboff += blocks_of(tcells * kThetaLanes);
+    tboff += theta_strip_blocks(c.T.ngx, c.T.ngy);
     nboff += blocks_of(nodes);
   }
   if (off >= (1LL << 27)) fail(SPH_E_ARG, "the f32c form packs slot offsets in 27 bits: N=%lld", off);
@@ -236,8 +244,7 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
   std::vector<int> tb2s((size_t)tboff), nb2s((size_t)nboff);
   for (int k = 0; k < nsc; ++k) {
     const Scene32 &c = scenes[k];
-    const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
-    for (int b = 0; b < blocks_of(tcells * kThetaLanes); ++b) tb2s[c.tboff + b] = k;
+    for (long long b = 0; b < theta_strip_blocks(c.T.ngx, c.T.ngy); ++b) tb2s[c.tboff + b] = k;
     for (int b = 0; b < blocks_of((long long)c.T.ngx * c.T.ngy); ++b) nb2s[c.nboff + b] = k;
   }
   int cfg[TK_COUNT];
@@ -249,6 +256,9 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
   HIPC(hipMemcpy(bs.sc, scenes.data(), sizeof(Scene32) * scenes.size(), hipMemcpyHostToDevice));
   bs.tb2s = dalloc<int>(tb2s.size() + 1);
   bs.nb2s = dalloc<int>(nb2s.size() + 1);
+  bs.trange = dalloc<int2>(trange.size() + 1);
+  if (!trange.empty())
+    HIPC(hipMemcpy(bs.trange, trange.data(), sizeof(int2) * trange.size(), hipMemcpyHostToDevice));
   if (!tb2s.empty())
     HIPC(hipMemcpy(bs.tb2s, tb2s.data(), sizeof(int) * tb2s.size(), hipMemcpyHostToDevice));
   if (!nb2s.empty())
@@ -272,6 +282,7 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     B.boff = bs.boff[v];
     B.tb2s = bs.tb2s;
     B.nb2s = bs.nb2s;
+    B.trange = bs.trange;
     B.bs = wg;
     B.nscenes = nsc;
     B.nblk = (int)b2s.size();

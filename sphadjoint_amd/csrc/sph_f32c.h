@@ -31,8 +31,9 @@ struct Scene32 {
   double dmax2;            // squared displacement allowed between rebuilds (skin*h)^2
   int off, coff;           // first slot, first cell
   int toff, tcoff, tboff;  // first theta node, first theta cell, first workgroup of
-                           // theta-cell launches (kThetaLanes lanes per cell)
+                           // the keyed gtheta gather (theta_strip_blocks per scene)
   int nboff;               // first workgroup of theta-node launches
+  int troff;               // first entry of the scene's SPH ranges in Batch::trange
 };
 
 // Launch geometry of an f32c state: device scene table, workgroup -> scene
@@ -42,6 +43,10 @@ struct Scene32 {
 struct Batch {
   const Scene32 *sc;
   const int *b2s, *boff, *tb2s, *nb2s;
+  // per scene (from Scene32::troff): for each theta column i the SPH cell
+  // columns (cx0, cx1) that can hold slots whose interp cell has column i,
+  // then for each theta row j the SPH cell rows (cy0, cy1); see theta_ranges
+  const int2 *trange;
   int bs;
   int nscenes, nblk, ntblk, nnblk;
   int N, ncell, ntnode, ntcell;
@@ -50,6 +55,23 @@ constexpr int kSlotBlocks[3] = {64, 128, 256};
 // kernels whose workgroup size is tuned (sph_autotune)
 enum TunedKernel { TK_NLBUILD, TK_DENSITY, TK_FORCE, TK_ADJ_R, TK_ADJ_P1, TK_ADJ_P2, TK_COUNT };
 constexpr int kThetaLanes = 8;   // lanes that share one theta cell in the gtheta gather
+constexpr int kThetaStrip = 32;  // theta cells of one theta row per gtheta-gather workgroup
+// SPH cell range [c0, c1] holding slots whose interp cell has index i along
+// an axis of nt theta cells of width ht: the cells of the theta cell's
+// interval (the division rule of cell_index) widened by one cell on each side
+// for the displacement since the last rebuild, the whole axis for the edge
+// cells (the interpolation clamps there)
+inline int2 theta_cell_range(int i, int nt, double ht, double cs, int n) {
+  int c0 = (i == 0) ? 0 : cell_coord_div((double)i * ht / cs, n) - 1;
+  int c1 = (i == nt - 1) ? n - 1 : cell_coord_div((double)(i + 1) * ht / cs, n) + 1;
+  c0 = c0 < 0 ? 0 : c0;
+  c1 = c1 > n - 1 ? n - 1 : c1;
+  return make_int2(c0, c1);
+}
+// workgroups of the keyed gtheta gather of one scene: one per strip
+inline long long theta_strip_blocks(int ngx, int ngy) {
+  return (long long)(ngy - 1) * ((ngx - 1 + kThetaStrip - 1) / kThetaStrip);
+}
 
 // bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)
 constexpr int kMaskWords = 4;

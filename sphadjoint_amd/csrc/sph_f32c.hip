@@ -315,9 +315,12 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
 // window's own cells in row y'; that cell range is one slot range (segment
 // (y'-ya)*3 + r).  The segments are copied into LDS with coalesced loads and
 // the pair loop then reads its neighbours there.  A window spanning more than
-// kStageOwnRows cell rows, or whose segments exceed kStageCap slots, reads
-// from global memory instead (same results: only the loader differs).
+// kStageOwnRows cell rows, or whose segments exceed the pass's capacity
+// (kStageCap slots; kStageCapP1 for adjoint P1, whose 32 bytes per slot would
+// otherwise leave room for only four workgroups per CU), reads from global
+// memory instead (same results: only the loader differs).
 constexpr int kStageCap = 1024;
+constexpr int kStageCapP1 = 960;
 constexpr int kStageOwnRows = 4;
 constexpr int kStageSegs = 3 * kStageOwnRows;
 
@@ -330,7 +333,7 @@ struct StageTab {
 // Plans the staging of the window of slots [w0, w1) (w1 > w0); every thread
 // of the workgroup calls it.  Wave 0 forms the segments and their prefix.
 __device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ bcell, Rsrc cst,
-                                           int w0, int w1, const Grid &g, int coff) {
+                                           int w0, int w1, const Grid &g, int coff, int cap) {
   if (threadIdx.x < 64) {
     const int k = (int)threadIdx.x;
     const int2 a = cell_xy(bcell[w0], g), b = cell_xy(bcell[w1 - 1], g);
@@ -360,19 +363,41 @@ __device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ 
     if (k == 0) {
       T.off[kStageSegs] = total;
       T.ya = a.y;
-      T.ok = (nrow <= kStageOwnRows && total <= kStageCap) ? 1 : 0;
+      T.ok = (nrow <= kStageOwnRows && total <= cap) ? 1 : 0;
     }
   }
   __syncthreads();
 }
 
-// Copies the planned segments: cp(i, j) stores slot j's data at LDS index i.
-// Called by every thread of the workgroup when T.ok.
-template <class Cp>
-__device__ __forceinline__ void stage_copy(const StageTab &T, Cp &&cp) {
-  for (int k = 0; k < kStageSegs; ++k) {
-    const int o = T.off[k], n = T.off[k + 1] - o, lo = T.lo[k];
-    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) cp(o + i, lo + i);
+// Copies the planned segments: st(i, ld(j)) stores slot j's data at LDS
+// index i.  Called by every thread of the workgroup when T.ok.  Each thread
+// issues the loads of kStageUnroll elements before storing any of them, so a
+// window's copy costs one load latency per kStageUnroll*blockDim elements.
+constexpr int kStageUnroll = 4;
+template <class Ld, class St>
+__device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) {
+  const int total = T.off[kStageSegs];
+  int off[kStageSegs];
+#pragma unroll
+  for (int q = 0; q < kStageSegs; ++q) off[q] = T.off[q];
+  for (int i0 = (int)threadIdx.x; i0 < total; i0 += kStageUnroll * (int)blockDim.x) {
+    decltype(ld(0)) d[kStageUnroll];
+#pragma unroll
+    for (int u = 0; u < kStageUnroll; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < total) {
+        // segment of i: the last one starting at or before i (off ascends)
+        int k = 0;
+#pragma unroll
+        for (int q = 1; q < kStageSegs; ++q) k += (i >= off[q]) ? 1 : 0;
+        d[u] = ld(T.lo[k] + (i - off[k]));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStageUnroll; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < total) st(i, d[u]);
+    }
   }
   __syncthreads();
 }
@@ -405,41 +430,63 @@ __global__ void __launch_bounds__(kBlock)
 k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
              const int *__restrict__ cstart, Batch B, uint32_t *__restrict__ nm,
              int *__restrict__ flags) {
-  SCENE_SLOT(B);
-  const float rc2 = (float)(S.cs64 * S.cs64);
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap];
+  const int blk = xcd_block();
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const Grid &g = S.g;
+  const PhysF32 &P = S.P;
+  const int bsl = (blk - B.boff[scn]) * B.bs;
+  const int sl = bsl + (int)threadIdx.x;
+  const int w0 = S.off + bsl, w1 = min(w0 + B.bs, S.off + S.g.N);
+  const int s = sl < S.g.N ? S.off + sl : w0;
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
+  stage_plan(T, bcell, cst, w0, w1, g, S.coff, kStageCap);
+  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (sl >= S.g.N) return;
+  const float rc2 = (float)(S.cs64 * S.cs64);
   const float2 xi = xr[s];
-  const int2 c = cell_xy(bcell[s], g);
+  const int bc = bcell[s];
+  const int2 c = cell_xy(bc, g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
   bool over = false;
+  auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int jy = c.y + r - 1;
-    if (jy < 0 || jy >= g.ny) continue;
-    const RowRun run = row_run(cst, c.x, jy, g, S.coff);
-    const float yi = xi.y - (float)(r - 1) * P.cs;
-    int len = run.len;
-    if (len > kMaskRun) {
-      over = true;
-      len = kMaskRun;
-    }
-    const uint32_t rb8 = (uint32_t)run.rb * 8u;
-    for (int w = 0; w * 32 < len; ++w) {
-      uint32_t m = 0;
-      const int e = len - 32 * w < 32 ? len - 32 * w : 32;
-      for (int k = 0; k < e; ++k) {
-        const int b = 32 * w + k;
-        const float2 xj = ld2(xrr, rb8 + (uint32_t)b * 8u);
-        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
-        const float dx = xs - xj.x;
-        const float dy = yi - xj.y;
-        const float r2 = fmaf(dx, dx, dy * dy);
-        if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
+    for (int r = 0; r < 3; ++r) {
+      const int jy = c.y + r - 1;
+      if (jy < 0 || jy >= g.ny) continue;
+      const RowRun run = row_run(cst, c.x, jy, g, S.coff);
+      const float yi = xi.y - (float)(r - 1) * P.cs;
+      int len = run.len;
+      if (len > kMaskRun) {
+        over = true;
+        len = kMaskRun;
       }
-      nm[(size_t)(w * 3 + r) * B.N + s] = m;
+      const int rb = run.rb + ro.o[r];
+      for (int w = 0; w * 32 < len; ++w) {
+        uint32_t m = 0;
+        const int e = len - 32 * w < 32 ? len - 32 * w : 32;
+        for (int k = 0; k < e; ++k) {
+          const int b = 32 * w + k;
+          const float2 xj = ld(rb + b);
+          const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+          const float dx = xs - xj.x;
+          const float dy = yi - xj.y;
+          const float r2 = fmaf(dx, dx, dy * dy);
+          if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
+        }
+        nm[(size_t)(w * 3 + r) * B.N + s] = m;
+      }
     }
-  }
+  };
+  if (T.ok)
+    rows(stage_rowoff(T, bc, g), [&](int i) __attribute__((always_inline)) { return sx[i]; });
+  else
+    rows(RowOff{}, [&](int i) __attribute__((always_inline)) {
+      return ld2(xrr, (uint32_t)i * 8u);
+    });
   if (over) flags[0] = 1;
 }
 
@@ -538,8 +585,8 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
-  if (T.ok) stage_copy(T, [&](int i, int j) { sx[i] = xr[j]; });
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const int bc = bcell[s];
   float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
@@ -560,13 +607,14 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sx[kStageCap], spi[kStageCap], sv[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
   if (T.ok)
-    stage_copy(T, [&](int i, int j) {
-      sx[i] = xr[j];
-      spi[i] = PI[j];
-      sv[i] = V[j];
-    });
+    stage_copy(T, [&](int j) { return NbXPV{xr[j], PI[j], V[j]}; },
+               [&](int i, const NbXPV &d) {
+                 sx[i] = d.x;
+                 spi[i] = d.pi;
+                 sv[i] = d.v;
+               });
   if (!valid_) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
@@ -691,8 +739,8 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
-  if (T.ok) stage_copy(T, [&](int i, int j) { sx[i] = xr[j]; });
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const double2 x64 = X[s];
   const int bc = bcell[s];
@@ -750,17 +798,18 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
            const int *__restrict__ lane, Batch B, float2 *__restrict__ gXs,
            float2 *__restrict__ gV, float *__restrict__ grho, uint32_t *__restrict__ em) {
   __shared__ StageTab T;
-  __shared__ float2 sx[kStageCap], spi[kStageCap], sab[kStageCap], sv[kStageCap];
+  __shared__ float2 sx[kStageCapP1], spi[kStageCapP1], sab[kStageCapP1], sv[kStageCapP1];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCapP1);
   if (T.ok)
-    stage_copy(T, [&](int i, int j) {
-      sx[i] = xr[j];
-      spi[i] = PI[j];
-      sab[i] = abar[j];
-      sv[i] = V[j];
-    });
+    stage_copy(T, [&](int j) { return NbXPAV{xr[j], PI[j], abar[j], V[j]}; },
+               [&](int i, const NbXPAV &d) {
+                 sx[i] = d.x;
+                 spi[i] = d.pi;
+                 sab[i] = d.ab;
+                 sv[i] = d.v;
+               });
   if (!valid_) return;
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
@@ -844,12 +893,13 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   __shared__ float sgr[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
   if (T.ok)
-    stage_copy(T, [&](int i, int j) {
-      sx[i] = xr[j];
-      sgr[i] = grho[j];
-    });
+    stage_copy(T, [&](int j) { return NbXG{xr[j], grho[j]}; },
+               [&](int i, const NbXG &d) {
+                 sx[i] = d.x;
+                 sgr[i] = d.gr;
+               });
   if (!valid_) return;
   const float2 xi = xr[s];
   const float gri = grho[s];

@@ -77,7 +77,7 @@ __device__ __forceinline__ unsigned long long wave_runs(int key, int &head_lane,
 // unsafe_trunc(Int, floor(v)) + 2 clamped to [1, n], then made 0-based.
 // NaN / out-of-range inputs give the lowest cell, as x86-64's cvttsd2si
 // (typemin) followed by the clamp does in the reference.
-__device__ __forceinline__ int cell_coord_div(double v, int n) {
+__host__ __device__ inline int cell_coord_div(double v, int n) {
   double f = floor(v);
   long long t = (f > -1e15 && f < 1e15) ? (long long)f : (long long)(-(1LL << 60));
   long long c = t + 2;

@@ -97,45 +97,173 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
 // The same gather over per-slot keys written by the f32c adjoint's pointwise
 // pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
 // interp cell and fxy[p] = (fx, fy) in FP64, so that a candidate costs one
-// 4-byte load and a compare; the contributions and their order are those of
-// k_theta_cells.  Workgroups are dealt to the scenes of a batched state in
-// order (Scene32::tboff), kThetaLanes lanes per theta cell.
+// key compare; the contributions and their order are those of k_theta_cells.
+// A workgroup takes a strip of kThetaStrip theta cells of one theta row
+// (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
+// strip sweeps, the keys of the slot run that covers the SPH cell ranges of
+// all its cells into LDS (coalesced); each cell then compares keys there and
+// reads fxy and galpha only for its own slots; the cell starts of the strip's
+// rows are staged too, so that a cell's sweep issues no dependent global loads
+// before its matches.  A strip whose runs exceed kThetaStage keys, that sweeps
+// more than kThetaRows rows or more than kThetaCols - 1 SPH cells per row,
+// reads keys and cell starts in global memory (same order, same sums).
+// Workgroups are dealt to the scenes of a batched state in order
+// (Scene32::tboff).
+constexpr int kThetaStage = 2048;
+constexpr int kThetaRows = 12;
+constexpr int kThetaCols = 64;
+constexpr int kMatchCap = 8;
+constexpr int kMatchBatch = 4;
+
 template <class GA>
 __global__ void __launch_bounds__(kBlock)
-k_theta_cells_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
+k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
                     const GA *__restrict__ galpha, const int *__restrict__ cstart, Batch B,
                     double4 *__restrict__ part) {
-  const int blk = xcd_block();
+  static_assert(kThetaStrip * kLanes == kBlock, "one strip per workgroup");
+  __shared__ int skey[kThetaStage];
+  __shared__ int slo[kThetaRows], soff[kThetaRows + 1];
+  __shared__ int scs[kThetaRows][kThetaCols];   // cstart of cells X0 .. X1+1 of each row
+  __shared__ int sok, sx0;
+  __shared__ int smatch[kMatchCap][kBlock];      // each lane's matched slots, in order
+  const int blk = blockIdx.x;
   const Scene32 &Sc = B.sc[B.tb2s[blk]];
   const Grid g = Sc.g;
-  const ThetaGrid T = Sc.T;
+  const ThetaGrid &T = Sc.T;
   const int coff = Sc.coff, tcoff = Sc.tcoff;
   const int ncx = T.ngx - 1, ncy = T.ngy - 1;
+  const int nstrip = (ncx + kThetaStrip - 1) / kThetaStrip;
+  const int lb = blk - Sc.tboff;
+  const int j0 = lb / nstrip;
+  const int ifirst = (lb % nstrip) * kThetaStrip;
+  const int ilast = min(ifirst + kThetaStrip, ncx) - 1;
   const int lane = threadIdx.x & (kLanes - 1);
-  const int t = ((blk - Sc.tboff) * kBlock + (int)threadIdx.x) / kLanes;
-  if (t >= ncx * ncy) return;      // the kLanes lanes of a group leave together
-  const int i0 = t % ncx;
-  const int j0 = t / ncx;
+  const int i0 = ifirst + (int)threadIdx.x / kLanes;
+  // SPH cell ranges holding particles with interp cell (i, j0), +1 margin
+  // (theta_cell_range, tabulated per scene)
+  const int2 *tr = B.trange + Sc.troff;
+  auto xrange = [&](int i, int &cx0, int &cx1) {
+    const int2 r = tr[i];
+    cx0 = r.x;
+    cx1 = r.y;
+  };
+  const int2 ry = tr[ncx + j0];
+  const int cy0 = ry.x, cy1 = ry.y;
+  const int nrow = cy1 - cy0 + 1;
+  // plan: row k of the strip stages slots [lo_k, lo_k + len_k) at soff[k]
+  if (threadIdx.x < 64) {
+    const int k = (int)threadIdx.x;
+    int X0, X1, dummy;
+    xrange(ifirst, X0, dummy);
+    xrange(ilast, dummy, X1);
+    int lo = 0, len = 0;
+    if (k < nrow && k < kThetaRows) {
+      const int base = coff + (cy0 + k) * g.nx;
+      lo = cstart[base + X0];
+      len = cstart[base + X1 + 1] - lo;
+    }
+    int inc = len;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int t = __shfl_up(inc, d, 64);
+      if (k >= d) inc += t;
+    }
+    if (k < kThetaRows) {
+      slo[k] = lo;
+      soff[k] = inc - len;
+    }
+    const int total = __shfl(inc, kThetaRows - 1, 64);
+    if (k == 0) {
+      soff[kThetaRows] = total;
+      sx0 = X0;
+      sok = (nrow <= kThetaRows && total <= kThetaStage && X1 - X0 + 2 <= kThetaCols) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const bool staged = sok != 0;
+  if (staged) {
+    // cell starts of the rows, then the keys; each thread issues the loads of
+    // four elements before storing them
+    const int X0 = sx0;
+    const int ncol = g.nx - X0 + 1 < kThetaCols ? g.nx - X0 + 1 : kThetaCols;
+    const int ncs = nrow * kThetaCols, total = soff[kThetaRows];
+    int off[kThetaRows];
+#pragma unroll
+    for (int q = 0; q < kThetaRows; ++q) off[q] = soff[q];
+    for (int i0 = (int)threadIdx.x; i0 < ncs + total; i0 += 4 * kBlock) {
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        v[u] = 0;
+        if (i < ncs) {
+          const int k = i / kThetaCols, c = i % kThetaCols;
+          if (c < ncol) v[u] = cstart[coff + (cy0 + k) * g.nx + X0 + c];
+        } else if (i < ncs + total) {
+          const int t = i - ncs;
+          int k = 0;
+#pragma unroll
+          for (int q = 1; q < kThetaRows; ++q) k += (t >= off[q]) ? 1 : 0;
+          v[u] = key[slo[k] + (t - off[k])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        if (i < ncs) (&scs[0][0])[i] = v[u];
+        else if (i < ncs + total) skey[i - ncs] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (i0 > ilast) return;          // the kLanes lanes of a group leave together
   const int mykey = tcoff + j0 + i0 * ncy;
-  int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - 1;
-  int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + 1;
-  int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - 1;
-  int cy1 = (j0 == ncy - 1) ? g.ny - 1 : cell_of((double)(j0 + 1) * T.hy, T.cs, g.ny) + 1;
-  cx0 = cx0 < 0 ? 0 : cx0;
-  cy0 = cy0 < 0 ? 0 : cy0;
-  cx1 = cx1 > g.nx - 1 ? g.nx - 1 : cx1;
-  cy1 = cy1 > g.ny - 1 ? g.ny - 1 : cy1;
+  int cx0, cx1;
+  xrange(i0, cx0, cx1);
   double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
+  auto add = [&](double2 f, double ga) {
+    s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
+    s10 += (ga * f.x) * (1.0 - f.y);
+    s01 += (ga * (1.0 - f.x)) * f.y;
+    s11 += (ga * f.x) * f.y;
+  };
+  // the lane's matches, in sweep order: their positions first (LDS), then
+  // their data in batches of kMatchBatch independent loads
+  const int tid = (int)threadIdx.x;
+  int nmatch = 0;
   for (int cy = cy0; cy <= cy1; ++cy) {
-    const int b = cstart[coff + cy * g.nx + cx0], e = cstart[coff + cy * g.nx + cx1 + 1];
+    const int k = cy - cy0;
+    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + cy * g.nx + cx0];
+    const int e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + cy * g.nx + cx1 + 1];
+    const int ro = staged ? soff[k] - slo[k] : 0;
     for (int p = b + lane; p < e; p += kLanes) {
-      if (key[p] != mykey) continue;
-      const double2 f = fxy[p];
-      const double ga = (double)galpha[p];
-      s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
-      s10 += (ga * f.x) * (1.0 - f.y);
-      s01 += (ga * (1.0 - f.x)) * f.y;
-      s11 += (ga * f.x) * f.y;
+      if ((staged ? skey[p + ro] : key[p]) != mykey) continue;
+      if (nmatch < kMatchCap) smatch[nmatch][tid] = p;
+      ++nmatch;
+    }
+  }
+  if (nmatch <= kMatchCap) {
+    for (int m0 = 0; m0 < nmatch; m0 += kMatchBatch) {
+      double2 f[kMatchBatch];
+      double ga[kMatchBatch];
+#pragma unroll
+      for (int q = 0; q < kMatchBatch; ++q) {
+        if (m0 + q < nmatch) {
+          const int p = smatch[m0 + q][tid];
+          f[q] = fxy[p];
+          ga[q] = (double)galpha[p];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kMatchBatch; ++q)
+        if (m0 + q < nmatch) add(f[q], ga[q]);
+    }
+  } else {
+    // more matches than the position list holds: the same sweep, summing as it goes
+    for (int cy = cy0; cy <= cy1; ++cy) {
+      const int b = cstart[coff + cy * g.nx + cx0], e = cstart[coff + cy * g.nx + cx1 + 1];
+      for (int p = b + lane; p < e; p += kLanes)
+        if (key[p] == mykey) add(fxy[p], (double)galpha[p]);
     }
   }
 #pragma unroll
@@ -203,7 +331,7 @@ void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 
                                    const float *galpha, const int *cstart, const Batch &B,
                                    double4 *part, double *gtheta) {
   static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
-  k_theta_cells_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, B, part);
+  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, B, part);
   k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta);
 }
 
