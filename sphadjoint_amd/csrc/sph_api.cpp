@@ -139,6 +139,12 @@ PhysF32 make_phys32(const sph_params &p, double rc) {
   P.hy = p.Ly / (double)(p.ngy - 1);
   P.inv_hx = (float)(1.0 / P.hx);
   P.inv_hy = (float)(1.0 / P.hy);
+  P.inv_hx64 = 1.0 / P.hx;
+  P.inv_hy64 = 1.0 / P.hy;
+  P.mhh = (float)(-0.5 / h);
+  P.twoinvh = (float)(2.0 / h);
+  P.mFc = (float)(p.m * (-5.0 * A / (h * h)));
+  P.Gch = (float)(7.5 * A / (h * h * h));
   P.gxmax = (double)(p.ngx - 1);
   P.gymax = (double)(p.ngy - 1);
   P.gxclamp = (double)(p.ngx - 1) - 1e-6;

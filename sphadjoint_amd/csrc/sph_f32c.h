@@ -10,7 +10,11 @@ struct PhysF32 {
   float m, rho0, c2, mu, gx, gy, dt, Lx, Ly, kw;
   float mA, invh, r2max, r2min, Fc, Gc, twomumass, cs;
   float inv_hx, inv_hy;
+  // products folded once on the host: u = 1 - q/2 = fma(r, mhh, 1) with
+  // mhh = -1/(2h); 2q + 1 = fma(r, twoinvh, 1); mFc = m*Fc; Gch = Gc/h
+  float mhh, twoinvh, mFc, Gch;
   double dt64, hx, hy, gxmax, gymax, gxclamp, gyclamp;
+  double inv_hx64, inv_hy64;   // 1/hx, 1/hy: interp_cell's grid coordinates
   int ngx, ngy;
 };
 

@@ -99,8 +99,8 @@ struct InterpF {
 // interp_cell (src/params.jl:53-68) from the FP64 position
 __device__ __forceinline__ InterpF interp_cell(double x, double y, const PhysF32 &P) {
   InterpF r;
-  double gx = x / P.hx;
-  double gy = y / P.hy;
+  double gx = x * P.inv_hx64;
+  double gy = y * P.inv_hy64;
   r.ax = (gx > 0.0 && gx < P.gxmax) ? 1.f : 0.f;
   r.ay = (gy > 0.0 && gy < P.gymax) ? 1.f : 0.f;
   double gxc = (gx > P.gxclamp) ? P.gxclamp : ((gx < 0.0) ? 0.0 : gx);
@@ -537,22 +537,26 @@ template <class Ld>
 __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, int s, int bc,
                                                float2 xi, const Grid &g, int coff, uint32_t Ntot,
                                                const PhysF32 &P, const RowOff &ro, Ld &&ld) {
-  // Fast2Sum from the self term, the largest term of the sum: every later
-  // term is positive and smaller, so each step's error term is exact
-  FSum acc(P.mA);
+  // sum of W/(m A) = u^4 (2q + 1) by Fast2Sum from the self term 1, the
+  // largest term of the sum: every later term is positive and no larger, so
+  // each step's error term is exact
+  FSum acc(1.f);
   for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
-      float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-      float u = fmaf(-0.5f, q, 1.f);
-      float u2 = u * u;
-      acc.add(P.mA * (u2 * u2 * fmaf(2.f, q, 1.f)));
+      const float r = __builtin_amdgcn_sqrtf(r2);
+      const float u = fmaf(r, P.mhh, 1.f);
+      const float u2 = u * u;
+      acc.add(u2 * u2 * fmaf(r, P.twoinvh, 1.f));
       return true;
     }
     return false;
   });
-  // rho - rho0 from the compensated pair (exact subtraction, then the tail)
-  float drho = (acc.s - P.rho0) + acc.c;
+  // rho = mA (s + c): the product's rounding error by an FMA, then rho - rho0
+  // with the exact subtraction first and the tails after
+  const float p = P.mA * acc.s;
+  const float pe = fmaf(P.mA, acc.s, -p);
+  const float drho = (p - P.rho0) + fmaf(P.mA, acc.c, pe);
   return make_float2(P.rho0 + drho, drho);
 }
 
@@ -572,7 +576,7 @@ __device__ __forceinline__ float2 density_slot(Rsrc nmr, uint32_t *em, Rsrc cst,
 
 
 __device__ __forceinline__ float2 eos(float2 rd, const PhysF32 &P) {
-  float inv = 1.f / rd.x;
+  float inv = __builtin_amdgcn_rcpf(rd.x);
   return make_float2(P.c2 * rd.y * inv * inv, inv);    // (pterm, invrho)
 }
 
@@ -619,15 +623,17 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
   const int bc = bcell[s];
-  const float tmi = P.twomumass * pii.y;
+  // pair term F (C (v_i - v_j) - m (pt_i + pt_j) d), F = Fc u^3, with Fc and
+  // m folded into the own-particle factors
+  const float piF = P.mFc * pii.x;
+  const float tmF = (P.Fc * P.twomumass) * pii.y;
   CSum ax, ay;
   auto pair = [&, P](const NbXPV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
-    float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-    float u = fmaf(-0.5f, q, 1.f);
-    float F = P.Fc * u * u * u;
+    const float u = fmaf(__builtin_amdgcn_sqrtf(r2), P.mhh, 1.f);
+    const float u3 = u * u * u;
     const float2 pij = d.pi;
-    float PF = P.m * (pii.x + pij.x) * F;
-    float C = tmi * pij.y * F;
+    const float PF = fmaf(P.mFc, pij.x, piF) * u3;
+    const float C = (tmF * pij.y) * u3;
     const float2 vj = d.v;
     ax.add(fmaf(C, vi.x - vj.x, -(PF * dx)));
     ay.add(fmaf(C, vi.y - vj.y, -(PF * dy)));
@@ -657,7 +663,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
     alpha_out[s] = al;
     return;
   }
-  float D = 1.f / (1.f + P.dt * al);
+  float D = __builtin_amdgcn_rcpf(fmaf(P.dt, al, 1.f));
   float v1 = (vi.x + P.dt * axv) * D;
   float v2 = (vi.y + P.dt * ayv) * D;
   Vn[s] = make_float2(v1, v2);
@@ -753,7 +759,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
-  float D = 1.f / (1.f + P.dt * al);
+  float D = __builtin_amdgcn_rcpf(fmaf(P.dt, al, 1.f));
   float dtD = P.dt * D;
   abar[s] = make_float2(dtD * gvx, dtD * gvy);
   galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
@@ -778,7 +784,7 @@ k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
-  float D = 1.f / (1.f + P.dt * al);
+  float D = __builtin_amdgcn_rcpf(fmaf(P.dt, al, 1.f));
   float dtD = P.dt * D;
   abar[s] = make_float2(dtD * gvx, dtD * gvy);
   galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
@@ -814,19 +820,20 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   const double2 x64 = X[s];
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const float pti = pii.x, iri = pii.y;
+  const float mpti = P.m * pti, tmi = P.twomumass * iri;
   const int bc = bcell[s];
   ASum gx, gy, gvx, gvy, grh, sumS;
   auto pair = [&, P](const NbXPAV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     if (kExact || r2 < P.r2max) {
-      float rinv = __builtin_amdgcn_rsqf(r2);
-      float q = (r2 * rinv) * P.invh;
-      float u = fmaf(-0.5f, q, 1.f);
-      float F = P.Fc * u * u * u;
-      float G = P.Gc * u * u;
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float u = fmaf(r2 * rinv, P.mhh, 1.f);
+      const float u2 = u * u;
+      const float F = (P.Fc * u2) * u;
+      const float G = P.Gch * u2;       // G/h
       const float2 pij = d.pi;
-      float Pij = P.m * (pti + pij.x);
-      float Cbase = P.twomumass * iri * pij.y;
-      float C = Cbase * F;
+      const float Pij = fmaf(P.m, pij.x, mpti);
+      const float Cbase = tmi * pij.y;
+      const float C = Cbase * F;
       const float2 abj = d.ab;
       const float2 vj = d.v;
       float dax = ab.x - abj.x;
@@ -836,7 +843,7 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
       float ad_d = -fmaf(dax, dx, day * dy);
       float ad_v = fmaf(dax, dvx, day * dvy);
       float Fbar = fmaf(Pij, ad_d, Cbase * ad_v);
-      float wgt = Fbar * G * P.invh * rinv;
+      float wgt = (Fbar * G) * rinv;
       float PF = Pij * F;
       gx.add(fmaf(wgt, dx, -(PF * dax)));
       gy.add(fmaf(wgt, dy, -(PF * day)));
@@ -906,9 +913,8 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   const int bc = bcell[s];
   ASum gx, gy;
   auto pair = [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
-    float q = __builtin_amdgcn_sqrtf(r2) * P.invh;
-    float u = fmaf(-0.5f, q, 1.f);
-    float wgt = P.m * (P.Fc * u * u * u) * (gri + d.gr);
+    const float u = fmaf(__builtin_amdgcn_sqrtf(r2), P.mhh, 1.f);
+    const float wgt = (P.mFc * (u * u * u)) * (gri + d.gr);
     gx.add(wgt * dx);
     gy.add(wgt * dy);
     return true;
