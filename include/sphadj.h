@@ -228,6 +228,37 @@ int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5);
  * most snapshots alive at once} for `epochs` segments and `nsnap` snapshots */
 int sph_ckpt_plan(int64_t epochs, int64_t nsnap, int64_t *out3);
 
+/* --- design-field chain of the topology optimiser, on the device in FP64.
+ * The reference keeps it in its optimisation script (scripts/03_optimize.jl:142-244):
+ *   rd -> apply_filter (linear weights (r+1) - dist, radius r grid points,
+ *         normalised per node, clamped to [0,1]) -> rdf -> alpha_of(rdf) =
+ *         alpha_max*qp*(1-rdf)/(qp+rdf) -> theta
+ *   gtheta -> .* dalpha_of(rdf) -> filter_adjoint -> grd
+ *   project_volume(x, volfrac) (bisection, scripts/03_optimize.jl:205-219) and
+ *   the line-search candidate project_volume(rd - step*g/max|g|, volfrac)
+ *   (scripts/03_optimize.jl:311).
+ * Fields are sph_field handles of (ngy, ngx) column-major doubles. */
+typedef struct sph_design sph_design;
+/* build_filter (scripts/03_optimize.jl:156-164) for an (ngy, ngx) grid */
+int sph_design_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, int64_t radius, double alpha_max,
+                      double qp, sph_design **out);
+int sph_design_destroy(sph_design *d);
+/* rdf = apply_filter(rd); theta[offset + node] = alpha_of(rdf[node]).  rdf is
+ * (ngy, ngx); theta holds at least offset + ngy*ngx nodes (offset > 0 writes
+ * one scene's field of a batched design field). */
+int sph_design_theta(sph_design *d, const sph_field *rd, sph_field *rdf, sph_field *theta,
+                     int64_t offset);
+/* grd = filter_adjoint(gtheta .* dalpha_of(rdf)) with gtheta the nodes
+ * [offset, offset + ngy*ngx) of the workspace's dJ/dtheta (after sph_backward) */
+int sph_design_grad(sph_design *d, const sph_field *rdf, const sph_workspace *ws, int64_t offset,
+                    sph_field *grd);
+/* out = project_volume(x, volfrac); out and x are distinct fields */
+int sph_design_project(sph_design *d, const sph_field *x, double volfrac, sph_field *out);
+/* out = project_volume(rd - step * g / max|g|, volfrac); *gmax (host, may be
+ * NULL) receives max|g|; out is distinct from rd and g */
+int sph_design_candidate(sph_design *d, const sph_field *rd, const sph_field *g, double step,
+                         double volfrac, sph_field *out, double *gmax);
+
 /* --- measurement: per-kernel GPU time with events on the context's stream.
  * No counterpart in the reference (its stage timing lives in
  * scripts/06_metal_profile.jl:158-290); used by bench.py for the roofline. */

@@ -92,6 +92,12 @@ SIGNATURES = {
     "sph_ckpt_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "sph_ckpt_stats": (C.c_int, [_vp, _vp]),
     "sph_ckpt_plan": (C.c_int, [_i64, _i64, _vp]),
+    "sph_design_create": (C.c_int, [_vp, _i64, _i64, _i64, _d, _d, _pp]),
+    "sph_design_destroy": (C.c_int, [_vp]),
+    "sph_design_theta": (C.c_int, [_vp, _vp, _vp, _vp, _i64]),
+    "sph_design_grad": (C.c_int, [_vp, _vp, _vp, _i64, _vp]),
+    "sph_design_project": (C.c_int, [_vp, _vp, _d, _vp]),
+    "sph_design_candidate": (C.c_int, [_vp, _vp, _vp, _d, _d, _vp, C.POINTER(_d)]),
     "sph_profile_enable": (C.c_int, [_vp, C.c_int]),
     "sph_profile_get": (C.c_int, [_vp, C.c_int, C.POINTER(_i64), C.POINTER(_d)]),
     "sph_profile_name": (C.c_char_p, [C.c_int]),

@@ -859,6 +859,151 @@ int sph_field_destroy(sph_field *f) {
   });
 }
 
+// ---- design-field chain (scripts/03_optimize.jl:142-244) -------------------
+
+struct sph_design {
+  sph_ctx *ctx = nullptr;
+  DesignGrid D{};
+  double *wtab = nullptr;   // (r+1) - sqrt(k) for squared grid distance k
+  double *snorm = nullptr;  // normalisation of each node's weight list
+  double *grdf = nullptr;   // scratch: gtheta .* dalpha(rdf)
+  double *gmax = nullptr;   // scratch: max|g| of the last candidate
+};
+
+namespace {
+void need_grid(const sph_design *d, const sph_field *f, const char *what) {
+  if (!f) fail(SPH_E_ARG, "%s is NULL", what);
+  if (f->ngy != d->D.ngy || f->ngx != d->D.ngx)
+    fail(SPH_E_ARG, "%s is (%lld, %lld), the design grid is (%d, %d)", what, (long long)f->ngy,
+         (long long)f->ngx, d->D.ngy, d->D.ngx);
+}
+}  // namespace
+
+int sph_design_create(sph_ctx *ctx, int64_t ngy, int64_t ngx, int64_t radius, double alpha_max,
+                      double qp, sph_design **out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
+    if (ngy < 1 || ngx < 1 || ngy * ngx >= (1LL << 30)) fail(SPH_E_ARG, "bad design grid");
+    if (radius < 0 || radius > 64) fail(SPH_E_ARG, "filter radius must be in [0, 64]");
+    if (!(qp > 0.0)) fail(SPH_E_ARG, "qp must be positive");
+    HIPC(hipSetDevice(ctx->device));
+    auto *d = new sph_design;
+    d->ctx = ctx;
+    d->D = DesignGrid{(int)ngy, (int)ngx, (int)radius, alpha_max, qp};
+    // build_filter (scripts/03_optimize.jl:156-164): w = (r+1) - sqrt(di^2 + dj^2),
+    // kept when w > 0; s = sum of the kept w over the box, jj outer, ii inner
+    const int r = (int)radius;
+    std::vector<double> wtab((size_t)(2 * r * r + 1));
+    for (int k = 0; k <= 2 * r * r; ++k) wtab[k] = (double)(r + 1) - std::sqrt((double)k);
+    std::vector<double> snorm((size_t)(ngy * ngx));
+    for (int64_t j = 0; j < ngy; ++j)
+      for (int64_t i = 0; i < ngx; ++i) {
+        double sum = 0.0;
+        for (int64_t jj = std::max<int64_t>(0, j - r); jj <= std::min<int64_t>(ngy - 1, j + r); ++jj)
+          for (int64_t ii = std::max<int64_t>(0, i - r); ii <= std::min<int64_t>(ngx - 1, i + r);
+               ++ii) {
+            const double w = wtab[(size_t)((ii - i) * (ii - i) + (jj - j) * (jj - j))];
+            if (w > 0.0) sum += w;
+          }
+        snorm[(size_t)(j + i * ngy)] = sum;
+      }
+    d->wtab = dalloc<double>(wtab.size());
+    d->snorm = dalloc<double>(snorm.size());
+    d->grdf = dalloc<double>(snorm.size());
+    d->gmax = dalloc<double>(1);
+    HIPC(hipMemcpy(d->wtab, wtab.data(), sizeof(double) * wtab.size(), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d->snorm, snorm.data(), sizeof(double) * snorm.size(), hipMemcpyHostToDevice));
+    ctx_retain(ctx);
+    *out = d;
+  });
+}
+
+int sph_design_destroy(sph_design *d) {
+  return guarded([&] {
+    if (!d) return;
+    (void)hipStreamSynchronize(d->ctx->stream);
+    dfree(d->wtab);
+    dfree(d->snorm);
+    dfree(d->grdf);
+    dfree(d->gmax);
+    sph_ctx *c = d->ctx;
+    delete d;
+    ctx_release(c);
+  });
+}
+
+int sph_design_theta(sph_design *d, const sph_field *rd, sph_field *rdf, sph_field *theta,
+                     int64_t offset) {
+  return guarded([&] {
+    if (!d) fail(SPH_E_ARG, "design is NULL");
+    need_grid(d, rd, "rd");
+    need_grid(d, rdf, "rdf");
+    if (rd == rdf) fail(SPH_E_ARG, "rd and rdf must be distinct fields");
+    if (!theta) fail(SPH_E_ARG, "theta is NULL");
+    const int64_t n = (int64_t)d->D.ngy * d->D.ngx;
+    if (offset < 0 || offset + n > theta->ngy * theta->ngx)
+      fail(SPH_E_ARG, "theta holds %lld nodes, need %lld", (long long)(theta->ngy * theta->ngx),
+           (long long)(offset + n));
+    HIPC(hipSetDevice(d->ctx->device));
+    hipStream_t s = d->ctx->stream;
+    launch_design_filter(s, rd->d, d->wtab, d->snorm, d->D, rdf->d);
+    launch_design_alpha(s, rdf->d, d->D, theta->d + offset);
+    HIPC(hipGetLastError());
+  });
+}
+
+int sph_design_grad(sph_design *d, const sph_field *rdf, const sph_workspace *ws, int64_t offset,
+                    sph_field *grd) {
+  return guarded([&] {
+    if (!d || !ws) fail(SPH_E_ARG, "NULL argument");
+    need_grid(d, rdf, "rdf");
+    need_grid(d, grd, "grd");
+    if (rdf == grd) fail(SPH_E_ARG, "rdf and grd must be distinct fields");
+    const int64_t n = (int64_t)d->D.ngy * d->D.ngx;
+    const int64_t wn = ws->ntnode > 0 ? ws->ntnode : (int64_t)ws->p.ngy * ws->p.ngx;
+    if (offset < 0 || offset + n > wn)
+      fail(SPH_E_ARG, "the workspace's gtheta holds %lld nodes, need %lld", (long long)wn,
+           (long long)(offset + n));
+    HIPC(hipSetDevice(d->ctx->device));
+    hipStream_t s = d->ctx->stream;
+    launch_design_chain(s, ws->gtheta + offset, rdf->d, d->D, d->grdf);
+    launch_design_filter_adjoint(s, d->grdf, d->wtab, d->snorm, d->D, grd->d);
+    HIPC(hipGetLastError());
+  });
+}
+
+int sph_design_project(sph_design *d, const sph_field *x, double volfrac, sph_field *out) {
+  return guarded([&] {
+    if (!d) fail(SPH_E_ARG, "design is NULL");
+    need_grid(d, x, "x");
+    need_grid(d, out, "out");
+    if (x == out) fail(SPH_E_ARG, "x and out must be distinct fields");
+    HIPC(hipSetDevice(d->ctx->device));
+    launch_design_project(d->ctx->stream, x->d, nullptr, 0.0, d->D.ngy * d->D.ngx, volfrac,
+                          out->d, nullptr);
+    HIPC(hipGetLastError());
+  });
+}
+
+int sph_design_candidate(sph_design *d, const sph_field *rd, const sph_field *g, double step,
+                         double volfrac, sph_field *out, double *gmax) {
+  return guarded([&] {
+    if (!d) fail(SPH_E_ARG, "design is NULL");
+    need_grid(d, rd, "rd");
+    need_grid(d, g, "g");
+    need_grid(d, out, "out");
+    if (out == rd || out == g) fail(SPH_E_ARG, "out must be distinct from rd and g");
+    HIPC(hipSetDevice(d->ctx->device));
+    hipStream_t s = d->ctx->stream;
+    launch_design_project(s, rd->d, g->d, step, d->D.ngy * d->D.ngx, volfrac, out->d, d->gmax);
+    HIPC(hipGetLastError());
+    if (gmax) {
+      HIPC(hipMemcpyAsync(gmax, d->gmax, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+    }
+  });
+}
+
 int sph_buffer_create(sph_ctx *ctx, int64_t nbytes, const void *host, sph_buffer **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");

@@ -196,5 +196,24 @@ void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galph
                              const int *cstart, Grid g, ThetaGrid T, double4 *part,
                              double *gtheta);
 
+// design-field chain of the topology optimiser (sph_design.hip): an
+// (ngy, ngx) grid, the density filter radius r (grid points) and the
+// Borrvall-Petersson constants
+struct DesignGrid {
+  int ngy, ngx, r;
+  double alpha_max, qp;
+};
+void launch_design_filter(hipStream_t s, const double *x, const double *wtab, const double *snorm,
+                          const DesignGrid &D, double *y);
+void launch_design_filter_adjoint(hipStream_t s, const double *g, const double *wtab,
+                                  const double *snorm, const DesignGrid &D, double *out);
+void launch_design_alpha(hipStream_t s, const double *rdf, const DesignGrid &D, double *theta);
+void launch_design_chain(hipStream_t s, const double *gtheta, const double *rdf,
+                         const DesignGrid &D, double *grdf);
+// out = project_volume(in - step * g / max|g|) (g may be NULL: project_volume(in));
+// gmax (device, may be NULL) receives max|g|
+void launch_design_project(hipStream_t s, const double *in, const double *g, double step, int n,
+                           double volfrac, double *out, double *gmax);
+
 
 }  // namespace sph
