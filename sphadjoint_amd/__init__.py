@@ -9,11 +9,11 @@ from .api import (AdjointWorkspace, BatchField, BatchState, BatchWorkspace, Cell
                   State, Tape, autotune_, backward_, batch_objective, build_, build_neighbors_,
                   capacity, check_neighbor_flags, ckpt_plan, reset_, simulate_, simulate_graph_,
                   step_, target_objective, tuning)
-from .design import DesignChain, Optimizer
+from .design import DesignChain, Optimizer, Sensitivity
 
 __all__ = ["SPHParams", "State", "CellList", "NeighborList", "AdjointWorkspace", "Tape",
            "simulate_", "step_", "build_", "build_neighbors_", "backward_", "target_objective",
            "reset_", "capacity", "check_neighbor_flags", "MI355XBackend", "Field", "Checkpointer",
            "ckpt_plan", "DeviceArray", "BatchState", "BatchWorkspace", "BatchField",
            "batch_objective", "simulate_graph_", "autotune_",
-           "tuning", "DesignChain", "Optimizer"]
+           "tuning", "DesignChain", "Optimizer", "Sensitivity"]

@@ -14,6 +14,10 @@ through the sph_design_* entry points of include/sphadj.h:
                                  halve the step up to 6 times, grow it by 1.3
                                  up to 0.3 after an acceptance)
 
+Sensitivity is the look-ahead overlay of scripts/04_interactive.jl:169-215
+(forward + adjoint over a short horizon from the current state, mapped to the
+improvement of painting each cell solid) without the GUI.
+
 Optimizer.optimize(batched=True) evaluates the up to six candidates of one
 backtracking round together, as the scenes of one batched f32c state, and
 accepts the first improving one in the order the sequential search would try
@@ -183,3 +187,43 @@ class Optimizer:
             _, g = self.objective_and_grad(rd)
         return np.asarray(rd), hist, it
 
+
+
+class Sensitivity:
+    """The look-ahead sensitivity overlay of the interactive script
+    (scripts/04_interactive.jl:169-215) without its GUI: from a state (X, V),
+    run ``nsens`` steps on a tape, take J = target_objective at (xc, yc) with
+    width sigma, run the adjoint, and map dJ/dtheta to the improvement of
+    painting a cell solid.  In that script theta = alpha_max * (1 - canvas^T),
+    so painting (canvas 1 -> 0) raises theta by alpha_max and the improvement
+    is -alpha_max * gtheta^T, shown on an (ngx, ngy) canvas, normalised by the
+    95th percentile of its nonzero magnitudes and clamped to [-1, 1].  A
+    look-ahead whose |J| is below ``min_signal`` (0.5 in the script) carries no
+    signal and is reported as invalid."""
+
+    def __init__(self, backend: MI355XBackend, p: SPHParams, N, nsens, *, interval=2,
+                 precision="f64", alpha_max=1500.0, min_signal=0.5):
+        self.backend, self.p, self.N, self.nsens = backend, p, int(N), int(nsens)
+        self.interval, self.precision = int(interval), precision
+        self.alpha_max, self.min_signal = float(alpha_max), float(min_signal)
+        # allocated once and reused at every request, as the script does
+        self.tape = Tape(backend, self.N, self.nsens, p, precision=precision)
+        self.ws = AdjointWorkspace(backend, self.N, p, precision=precision)
+
+    def compute(self, X, V, theta, target, sigma):
+        """(J, overlay (ngx, ngy) in [-1, 1], raw improvement (ngx, ngy), valid)."""
+        st = State(self.backend, X, V, self.p, interval=self.interval,
+                   precision=self.precision)      # teleport!: a fresh list, no displacement check
+        reset_(self.tape)
+        simulate_(st, theta, self.p, self.backend, self.nsens, tape=self.tape)
+        J, seed = target_objective(st, target[0], target[1], sigma, device=True)
+        backward_(self.ws, self.tape, theta, self.p, self.backend, seedX=seed,
+                  seedV=DeviceArray(self.backend, N=self.N), interval=self.interval)
+        improve = -self.alpha_max * self.ws.gtheta.T
+        nz = np.abs(improve)[np.abs(improve) > 0.0]
+        scale = float(np.quantile(nz, 0.95)) if nz.size else 1.0
+        overlay = np.clip(improve / max(scale, np.finfo(np.float64).eps), -1.0, 1.0)
+        valid = abs(J) >= self.min_signal
+        if not valid:
+            overlay = np.zeros_like(overlay)
+        return J, overlay, improve, valid

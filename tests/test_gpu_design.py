@@ -150,3 +150,39 @@ def test_line_search_f64_improves(be):
     rd_out, hist, it = opt.optimize(rd, maxit=4)
     assert len(hist) >= 2 and all(b < a for a, b in zip(hist, hist[1:]))
     assert rd_out.mean() >= 0.75 - 1e-15 and rd_out.min() >= 0 and rd_out.max() <= 1
+
+
+def test_sensitivity_overlay(be):
+    """scripts/04_interactive.jl:196-215 without the GUI: J and the improvement
+    map against the oracle, and the predicted improvement of painting the most
+    sensitive node against central differences in theta."""
+    say("sensitivity overlay")
+    sc = scenes.gradcheck_5x5(seed=0, nsteps=12)
+    p = S.SPHParams(**sc.params)
+    amax = 1500.0
+    sens = S.Sensitivity(be, p, sc.N, 12, interval=1, alpha_max=amax)
+    tgt, sig = (0.32, 0.32), 0.1
+    J, overlay, improve, valid = sens.compute(sc.X0, sc.V0, sc.theta, tgt, sig)
+    assert valid and overlay.shape == (7, 7) and np.abs(overlay).max() == 1.0
+    po = O.SPHParams(**sc.params)
+    st = O.State(sc.X0, sc.V0, po, interval=1, layout="slot")
+    tape = O.Tape(st.N, 12, po)
+    O.simulate_(st, sc.theta, po, 12, tape)
+    Jo, seed = O.target_objective(st.X, *tgt, sig)
+    ws = O.AdjointWorkspace(st.N, po)
+    O.backward_(ws, tape, sc.theta, po, seedX=seed, seedV=np.zeros_like(seed), interval=1,
+                layout="slot")
+    assert J == pytest.approx(Jo, rel=1e-13)
+    assert np.allclose(improve, -amax * ws.gtheta.T, rtol=1e-10,
+                       atol=1e-12 * np.abs(improve).max())
+    # painting node (j, i) by delta raises theta there by amax*delta
+    i, j = np.unravel_index(np.argmax(np.abs(improve)), improve.shape)
+    d = 1e-6
+
+    def J_of(th):
+        s2 = S.State(be, sc.X0, sc.V0, p, interval=1)
+        S.simulate_(s2, th, p, be, 12)
+        return S.target_objective(s2, *tgt, sig)[0]
+    tp = sc.theta.copy(); tp[j, i] += amax * d
+    tm = sc.theta.copy(); tm[j, i] -= amax * d
+    assert -(J_of(tp) - J_of(tm)) / (2 * d) == pytest.approx(improve[i, j], rel=1e-5)
