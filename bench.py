@@ -62,6 +62,18 @@ ALG_STEP_BYTES = {"f64": {"fwd": 96, "fwd_adj": 304}, "f32c": {"fwd": 72, "fwd_a
 # measured HBM traffic per launch of a kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # scaled by a plain copy of known size), keyed "<form>:<kernel>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_block_1000x1000.json")
+# VALU instructions per wave of each kernel (rocprofv3 SQ_INSTS_VALU / SQ_WAVES);
+# with the live launch time they give the fraction of SIMD issue cycles spent
+# on VALU work (a wave64 instruction holds a 16-lane SIMD for 4 cycles)
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_busy_block_1000x1000.json")
+SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint_r": "k_adj_r",
+                  "adjoint_p1": "k_adj_p1", "adjoint_p2": "k_adj_p2"},
+          "f32c": {"density": "k_density32", "force_integrate": "k_force32",
+                   "adjoint_r": "k_adj_r32", "adjoint_p1": "k_adj_p132<true>",
+                   "adjoint_p2": "k_adj_p232"},
+          "f32c_ckpt": {"density": "k_density32", "force_integrate": "k_force32",
+                        "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<false>",
+                        "adjoint_p2": "k_adj_p232"}}
 
 
 def log(*a):
@@ -257,10 +269,22 @@ def roofline_of(form, kern, N):
             source = os.path.relpath(TRAFFIC_FILE, ROOT)
     except (OSError, ValueError):
         pass
+    valu = None
+    try:
+        with open(VALU_FILE) as f:
+            vk = json.load(f)
+        rec = vk["kernels"].get(SYMBOL.get(form, {}).get(dom, ""))
+        if rec is not None and vk.get("N") == N:
+            cyc = rec["valu_per_wave"] * 4 * rec["waves"] / vk["simds"]
+            valu = {"valu_per_wave": rec["valu_per_wave"],
+                    "busy": round(cyc / (kern[dom]["avg_us"] * 1e-6 * vk["clock_GHz"] * 1e9), 3),
+                    "source": os.path.relpath(VALU_FILE, ROOT)}
+    except (OSError, ValueError, KeyError):
+        pass
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": dom, "alg_bytes_per_launch": alg, "avg_launch_us": round(kern[dom]["avg_us"], 3),
-            "traffic_source": source}
+            "traffic_source": source, "valu": valu}
 
 
 def main():
