@@ -488,6 +488,10 @@ struct sph_tape {
   bool keep_pi = false;
   double2 *PI = nullptr;
   float2 *PI32 = nullptr;
+  // f32c form with keep_pi: each step's exact neighbour masks (the density
+  // pass's em list, 3*kMaskWords words per slot), so that the reverse sweep's
+  // P1 and P2 walk them without testing r^2
+  uint32_t *EM = nullptr;
   std::vector<int> entry_epoch;         // epoch record of each entry (cap+1)
   std::vector<TapeEpoch> epochs;        // records in use
   std::vector<TapeEpoch> pool;          // retained allocations
@@ -681,6 +685,7 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   float2 *tV32 = nullptr;
   double2 *PI = st->PI;
   float2 *PI32 = st->PI32;
+  uint32_t *em = st->em;
   if (tp) {
     tape_epoch_for(tp, st);
     tp->n += 1;
@@ -693,14 +698,15 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
       if (st->precision == SPH_F32C) PI32 = tp->PI32 + off;
       else PI = tp->PI + off;
     }
+    if (tp->EM) em = tp->EM + off * (3 * kMaskWords);
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
       launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->lane,
-                          st->bs.at(TK_DENSITY), PI32, nullptr, st->em);
+                          st->bs.at(TK_DENSITY), PI32, nullptr, em);
     });
     timed(st->ctx, K_FORCE, [&] {
-      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, st->em, st->cstart,
+      launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, em, st->cstart,
                         st->lane, st->bs.at(TK_FORCE), st->Xn, st->xrn, st->V32n, tX, tV32,
                         nullptr, nullptr);
     });
@@ -1563,6 +1569,7 @@ int sph_tape_destroy(sph_tape *tp) {
     dfree(tp->V32);
     dfree(tp->PI);
     dfree(tp->PI32);
+    dfree(tp->EM);
     batch_free(tp->bs);
     sph_ctx *c = tp->ctx;
     delete tp;
@@ -1864,8 +1871,11 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
           launch_xrel_f32c(s, X0, E.scell, B, ws->xr32);
-          launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
-                               ws->flags);
+          // a tape with each step's exact masks needs no mask list, only the
+          // thread order of the epoch
+          if (!tape->EM)
+            launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
+                                 ws->flags);
           launch_lane_order_f32c(s, ws->xr32, B, ws->lane);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
@@ -1912,20 +1922,22 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, B,
                                       ws->tpart, ws->gtheta);
       });
+      // the exact list of these positions: the forward's density pass kept it
+      // on the tape, or the density recompute just wrote it; with stored
+      // (pterm, invrho) and no stored list, P1 tests the pairs and writes it
+      const uint32_t *exact = tape->EM ? tape->EM + off * (3 * kMaskWords) : ws->em;
       timed(c, K_ADJ_P1, [&] {
-        // the density recompute wrote the exact list of these positions; with
-        // stored (pterm, invrho) P1 tests the pairs and writes it for P2
-        if (tape->keep_pi)
+        if (tape->keep_pi && !tape->EM)
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
                              nm, false, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
                              ws->grho32, ws->em);
         else
           launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             ws->em, true, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
+                             exact, true, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
                              ws->grho32, nullptr);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, ws->em, E.cstart, lane,
+        launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, exact, E.cstart, lane,
                            BS.at(TK_ADJ_P2), ws->gX32);
       });
     } else {
@@ -2557,8 +2569,12 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
       if (rc != SPH_OK) throw Error(rc, g_err);
       ck->mini->keep_pi = true;
       const size_t tot = (size_t)(cap + 1) * (size_t)N;
-      if (precision == SPH_F32C) ck->mini->PI32 = dalloc<float2>(tot);
-      else ck->mini->PI = dalloc<double2>(tot);
+      if (precision == SPH_F32C) {
+        ck->mini->PI32 = dalloc<float2>(tot);
+        ck->mini->EM = dalloc<uint32_t>((size_t)cap * (size_t)N * (3 * kMaskWords));
+      } else {
+        ck->mini->PI = dalloc<double2>(tot);
+      }
       ck->snaps.resize((size_t)ck->nsnap + 1);
       for (auto &q : ck->snaps) {
         q.X = dalloc<double2>((size_t)N);
