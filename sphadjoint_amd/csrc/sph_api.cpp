@@ -601,8 +601,9 @@ void materialise_aux(sph_state *st) {
 }
 
 // build_neighbors! (src/neighbors.jl:252-263): build, then move every slot to
-// its (cell, id) position and start a new layout epoch.
-void do_build_neighbors(sph_state *st) {
+// its (cell, id) position and start a new layout epoch.  With defer_list (f32c
+// form) the mask list is left to the step's fused density pass.
+void do_build_neighbors(sph_state *st, bool defer_list = false) {
   hipStream_t s = st->ctx->stream;
   do_build(st);
   int check = (st->age != kAgeNever) ? 1 : 0;
@@ -626,8 +627,9 @@ void do_build_neighbors(sph_state *st) {
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C) {
-      launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
-                           st->flags);
+      if (!defer_list)
+        launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
+                             st->flags);
       launch_lane_order_f32c(s, st->xr, st->bs.B, st->lane);
     } else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
@@ -679,7 +681,10 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
 // state before the step (written by the kernels themselves).
 void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   hipStream_t s = st->ctx->stream;
-  if (st->age >= st->interval) do_build_neighbors(st);   // maybe_rebuild!
+  // maybe_rebuild!; the f32c form builds the new mask list in the step's
+  // density pass (the same list, one sweep fewer)
+  const bool fused = st->age >= st->interval && st->precision == SPH_F32C;
+  if (st->age >= st->interval) do_build_neighbors(st, fused);
   st->age += 1;
   double2 *tX = nullptr, *tV = nullptr;
   float2 *tV32 = nullptr;
@@ -702,8 +707,12 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   }
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_DENSITY, [&] {
-      launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->lane,
-                          st->bs.at(TK_DENSITY), PI32, nullptr, em);
+      if (fused)
+        launch_density_build_f32c(s, st->xr, st->scell, st->cstart, st->lane,
+                                  st->bs.at(TK_DENSITY), PI32, st->nm, em, st->flags);
+      else
+        launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->lane,
+                            st->bs.at(TK_DENSITY), PI32, nullptr, em);
     });
     timed(st->ctx, K_FORCE, [&] {
       launch_force_f32c(s, st->X, st->xr, st->V32, PI32, theta, st->scell, em, st->cstart,

@@ -87,6 +87,11 @@ void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, con
 // thread -> slot order of the pair passes for the positions xr of a rebuild
 // (B: the 256-slot geometry, Bv[2]); every pair launcher takes it as `lane`
 void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane);
+// first density pass after a rebuild, fused with the mask-list build: writes
+// the mask list nm, the exact list em, PI and the overflow flag (flags[0])
+void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
+                               const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                               uint32_t *nm, uint32_t *em, int *flags);
 // density + EOS over the mask list nm; when em is given, the exact list of
 // these positions (pairs with r2 < (2h)^2) is written there for the force pass
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,

@@ -533,6 +533,22 @@ k_lane_order(const float2 *__restrict__ xr, Batch B, int *__restrict__ lane) {
   }
 }
 
+// W/(m A) = u^4 (2q + 1) of a pair at squared distance r2 < (2h)^2
+__device__ __forceinline__ float density_term(float r2, const PhysF32 &P) {
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  const float u = fmaf(r, P.mhh, 1.f);
+  const float u2 = u * u;
+  return u2 * u2 * fmaf(r, P.twoinvh, 1.f);
+}
+// rho = mA (s + c): the product's rounding error by an FMA, then rho - rho0
+// with the exact subtraction first and the tails after; returns (rho, rho - rho0)
+__device__ __forceinline__ float2 density_finish(const FSum &acc, const PhysF32 &P) {
+  const float p = P.mA * acc.s;
+  const float pe = fmaf(P.mA, acc.s, -p);
+  const float drho = (p - P.rho0) + fmaf(P.mA, acc.c, pe);
+  return make_float2(P.rho0 + drho, drho);
+}
+
 template <class Ld>
 __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, int s, int bc,
                                                float2 xi, const Grid &g, int coff, uint32_t Ntot,
@@ -544,20 +560,12 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
   for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
-      const float r = __builtin_amdgcn_sqrtf(r2);
-      const float u = fmaf(r, P.mhh, 1.f);
-      const float u2 = u * u;
-      acc.add(u2 * u2 * fmaf(r, P.twoinvh, 1.f));
+      acc.add(density_term(r2, P));
       return true;
     }
     return false;
   });
-  // rho = mA (s + c): the product's rounding error by an FMA, then rho - rho0
-  // with the exact subtraction first and the tails after
-  const float p = P.mA * acc.s;
-  const float pe = fmaf(P.mA, acc.s, -p);
-  const float drho = (p - P.rho0) + fmaf(P.mA, acc.c, pe);
-  return make_float2(P.rho0 + drho, drho);
+  return density_finish(acc, P);
 }
 
 // density over the staged rows when the window fits, else over global loads
@@ -596,6 +604,79 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
+}
+
+// The first density pass after a rebuild, fused with the mask-list build:
+// one sweep over every slot of the three row runs sets the mask bit of each
+// list neighbour (r2min < r2 < rc2, as k_nm_build32), and for those within
+// (2h)^2 adds the density term and sets the exact-list bit, in the order in
+// which k_density32 would visit them -- the same pairs, the same arithmetic,
+// the same sums.  Writes nm, em, PI and the overflow flag.
+__global__ void __launch_bounds__(kBlock)
+k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
+                  const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
+                  float2 *__restrict__ PI, uint32_t *__restrict__ nm,
+                  uint32_t *__restrict__ em, int *__restrict__ flags) {
+  __shared__ StageTab T;
+  __shared__ float2 sx[kStageCap];
+  SCENE_WIN(B, lane);
+  const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
+  const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (!valid_) return;
+  const float rc2 = (float)(S.cs64 * S.cs64);
+  const float2 xi = xr[s];
+  const int bc = bcell[s];
+  const int2 c = cell_xy(bc, g);
+  const float xm = xi.x + P.cs, xp = xi.x - P.cs;
+  bool over = false;
+  FSum acc(1.f);
+  auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int jy = c.y + r - 1;
+      if (jy < 0 || jy >= g.ny) continue;
+      const RowRun run = row_run(cst, c.x, jy, g, S.coff);
+      const float yi = xi.y - (float)(r - 1) * P.cs;
+      int len = run.len;
+      if (len > kMaskRun) {
+        over = true;
+        len = kMaskRun;
+      }
+      const int rb = run.rb + ro.o[r];
+      for (int w = 0; w * 32 < len; ++w) {
+        uint32_t m = 0, e = 0;
+        const int n = len - 32 * w < 32 ? len - 32 * w : 32;
+        for (int k = 0; k < n; ++k) {
+          const int b = 32 * w + k;
+          const float2 xj = ld(rb + b);
+          const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+          const float dx = xs - xj.x;
+          const float dy = yi - xj.y;
+          const float r2 = fmaf(dx, dx, dy * dy);
+          if (r2 < rc2 && r2 > P.r2min) {
+            m |= 1u << k;
+            if (r2 < P.r2max) {
+              acc.add(density_term(r2, P));
+              e |= 1u << k;
+            }
+          }
+        }
+        const size_t widx = (size_t)(w * 3 + r) * B.N + s;
+        nm[widx] = m;
+        em[widx] = e;
+      }
+    }
+  };
+  if (T.ok)
+    rows(stage_rowoff(T, bc, g), [&](int i) __attribute__((always_inline)) { return sx[i]; });
+  else
+    rows(RowOff{}, [&](int i) __attribute__((always_inline)) {
+      return ld2(xrr, (uint32_t)i * 8u);
+    });
+  if (over) flags[0] = 1;
+  PI[s] = eos(density_finish(acc, P), P);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -949,6 +1030,12 @@ void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, con
 void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane) {
   if (B.bs != kLaneWin) throw std::logic_error("lane order needs the 256-slot geometry");
   if (B.N > 0) k_lane_order<<<B.nblk, kLaneWin, 0, s>>>(xr, B, lane);
+}
+void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
+                               const int *cstart, const int *lane, const Batch &B, float2 *PI,
+                               uint32_t *nm, uint32_t *em, int *flags) {
+  if (B.N > 0)
+    k_density_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, lane, B, PI, nm, em, flags);
 }
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
                          const int *cstart, const int *lane, const Batch &B, float2 *PI,
