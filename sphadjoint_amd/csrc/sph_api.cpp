@@ -177,7 +177,7 @@ struct BatchStore {
   std::vector<Scene32> host;
   Scene32 *sc = nullptr;
   int *b2s[3] = {}, *boff[3] = {}, *tb2s = nullptr, *nb2s = nullptr;
-  int2 *trange = nullptr;
+  int2 *trange = nullptr, *trange_tight = nullptr;
   const Batch &at(int k) const { return Bv[cfg[k]]; }
 };
 
@@ -215,6 +215,7 @@ void batch_free(BatchStore &bs) {
   dfree(bs.tb2s);
   dfree(bs.nb2s);
   dfree(bs.trange);
+  dfree(bs.trange_tight);
 }
 
 // Lays the scenes out (slots, cells, design nodes and cells, workgroups in
@@ -222,13 +223,20 @@ void batch_free(BatchStore &bs) {
 // workgroup sizes.
 void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
   long long off = 0, coff = 0, toff = 0, tcoff = 0, tboff = 0, nboff = 0;
-  std::vector<int2> trange;
+  std::vector<int2> trange, trange_tight;
   for (auto &c : scenes) {
     // SPH cell ranges of the theta columns, then of the theta rows
     c.troff = (int)trange.size();
     const int ncx = c.T.ngx - 1, ncy = c.T.ngy - 1;
-    for (int i = 0; i < ncx; ++i) trange.push_back(theta_cell_range(i, ncx, c.T.hx, c.T.cs, c.g.nx));
-    for (int j = 0; j < ncy; ++j) trange.push_back(theta_cell_range(j, ncy, c.T.hy, c.T.cs, c.g.ny));
+    const double ex = 0.25 * c.T.cs;
+    for (int i = 0; i < ncx; ++i) {
+      trange.push_back(theta_cell_range(i, ncx, c.T.hx, c.T.cs, c.g.nx));
+      trange_tight.push_back(theta_cell_range_tight(i, ncx, c.T.hx, c.T.cs, c.g.nx, ex));
+    }
+    for (int j = 0; j < ncy; ++j) {
+      trange.push_back(theta_cell_range(j, ncy, c.T.hy, c.T.cs, c.g.ny));
+      trange_tight.push_back(theta_cell_range_tight(j, ncy, c.T.hy, c.T.cs, c.g.ny, ex));
+    }
     const long long nodes = (long long)c.T.ngx * c.T.ngy;
     const long long tcells = (long long)(c.T.ngx - 1) * (c.T.ngy - 1);
     c.off = (int)off;
@@ -263,8 +271,12 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
   bs.tb2s = dalloc<int>(tb2s.size() + 1);
   bs.nb2s = dalloc<int>(nb2s.size() + 1);
   bs.trange = dalloc<int2>(trange.size() + 1);
-  if (!trange.empty())
+  bs.trange_tight = dalloc<int2>(trange_tight.size() + 1);
+  if (!trange.empty()) {
     HIPC(hipMemcpy(bs.trange, trange.data(), sizeof(int2) * trange.size(), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(bs.trange_tight, trange_tight.data(), sizeof(int2) * trange_tight.size(),
+                   hipMemcpyHostToDevice));
+  }
   if (!tb2s.empty())
     HIPC(hipMemcpy(bs.tb2s, tb2s.data(), sizeof(int) * tb2s.size(), hipMemcpyHostToDevice));
   if (!nb2s.empty())
@@ -289,6 +301,7 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     B.tb2s = bs.tb2s;
     B.nb2s = bs.nb2s;
     B.trange = bs.trange;
+    B.trange_tight = bs.trange_tight;
     B.bs = wg;
     B.nscenes = nsc;
     B.nblk = (int)b2s.size();
@@ -519,6 +532,7 @@ struct sph_workspace {
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
   int *lane = nullptr;                                     // f32c form: pair-pass thread order
   int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
+  int *tfar = nullptr;        // f32c form: a slot lies beyond its key's tight SPH range
   double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int64_t ntnode = 0, ntcell = 0;  // design-field nodes and cells (all scenes)
@@ -1652,6 +1666,7 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->em);
   dfree(ws->lane);
   dfree(ws->tkey);
+  dfree(ws->tfar);
   dfree(ws->tfxy);
 }
 
@@ -1820,6 +1835,10 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
     if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->lane) ws->lane = dalloc<int>((size_t)ws->N);
     if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);
+    if (!ws->tfar) {
+      ws->tfar = dalloc<int>(1);
+      HIPC(hipMemsetAsync(ws->tfar, 0, sizeof(int), ws->ctx->stream));
+    }
     if (!ws->tfxy) ws->tfxy = dalloc<double2>((size_t)ws->N);
     return;
   }
@@ -1919,16 +1938,16 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
           launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32, ws->gX32,
-                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy);
+                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy, ws->tfar);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
           launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
                             BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
-                            ws->galpha32, ws->em, ws->tkey, ws->tfxy);
+                            ws->galpha32, ws->em, ws->tkey, ws->tfxy, ws->tfar);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, B,
+        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, ws->tfar, B,
                                       ws->tpart, ws->gtheta);
       });
       // the exact list of these positions: the forward's density pass kept it

@@ -49,8 +49,10 @@ struct Batch {
   const int *b2s, *boff, *tb2s, *nb2s;
   // per scene (from Scene32::troff): for each theta column i the SPH cell
   // columns (cx0, cx1) that can hold slots whose interp cell has column i,
-  // then for each theta row j the SPH cell rows (cy0, cy1); see theta_ranges
-  const int2 *trange;
+  // then for each theta row j the SPH cell rows (cy0, cy1); see
+  // theta_cell_range.  trange allows a slot one cell beyond its binning cell,
+  // trange_tight a quarter cell (used for a step in which no slot is further out)
+  const int2 *trange, *trange_tight;
   int bs;
   int nscenes, nblk, ntblk, nnblk;
   int N, ncell, ntnode, ntcell;
@@ -70,6 +72,16 @@ inline int2 theta_cell_range(int i, int nt, double ht, double cs, int n) {
   int c1 = (i == nt - 1) ? n - 1 : cell_coord_div((double)(i + 1) * ht / cs, n) + 1;
   c0 = c0 < 0 ? 0 : c0;
   c1 = c1 > n - 1 ? n - 1 : c1;
+  return make_int2(c0, c1);
+}
+// The same range when a slot may lie at most ex beyond its binning cell: a slot
+// in binning cell c at x has (c-1) cs - ex <= x < c cs + ex, so c lies in
+// [cell_of(x - ex), cell_of(x + ex)], and x in [i ht, (i+1) ht) bounds both
+// (slack covers the rounding of the interpolation coordinate)
+inline int2 theta_cell_range_tight(int i, int nt, double ht, double cs, int n, double ex) {
+  const double slack = 1e-9 * cs;
+  int c0 = (i == 0) ? 0 : cell_coord_div(((double)i * ht - ex - slack) / cs, n);
+  int c1 = (i == nt - 1) ? n - 1 : cell_coord_div(((double)(i + 1) * ht + ex + slack) / cs, n);
   return make_int2(c0, c1);
 }
 // workgroups of the keyed gtheta gather of one scene: one per strip
@@ -111,14 +123,17 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          int *flags);
 // R passes also write each slot's interp-cell key and FP64 fractions for the
 // keyed gtheta gather
+// (both R passes also raise *tfar when a slot lies beyond its interp cell's
+// tight SPH range, Batch::trange_tight)
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, double2 *tfxy);
+                       int *tkey, double2 *tfxy, int *tfar);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy);
+                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy,
+                         int *tfar);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
@@ -131,9 +146,10 @@ void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, cons
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
                         const Batch &B, float2 *gX);
 // deterministic gtheta gather over the keys of the R passes (sph_theta.hip)
+// sweeps the tight SPH ranges unless *tfar is set; clears *tfar for the next step
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, const Batch &B,
-                                   double4 *part, double *gtheta);
+                                   const float *galpha, const int *cstart, int *tfar,
+                                   const Batch &B, double4 *part, double *gtheta);
 
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);

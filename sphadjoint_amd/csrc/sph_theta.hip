@@ -118,8 +118,8 @@ constexpr int kMatchBatch = 4;
 template <class GA>
 __global__ void __launch_bounds__(kBlock)
 k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
-                    const GA *__restrict__ galpha, const int *__restrict__ cstart, Batch B,
-                    double4 *__restrict__ part) {
+                    const GA *__restrict__ galpha, const int *__restrict__ cstart,
+                    const int *__restrict__ tfar, Batch B, double4 *__restrict__ part) {
   static_assert(kThetaStrip * kLanes == kBlock, "one strip per workgroup");
   __shared__ int skey[kThetaStage];
   __shared__ int slo[kThetaRows], soff[kThetaRows + 1];
@@ -139,9 +139,10 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
   const int ilast = min(ifirst + kThetaStrip, ncx) - 1;
   const int lane = threadIdx.x & (kLanes - 1);
   const int i0 = ifirst + (int)threadIdx.x / kLanes;
-  // SPH cell ranges holding particles with interp cell (i, j0), +1 margin
-  // (theta_cell_range, tabulated per scene)
-  const int2 *tr = B.trange + Sc.troff;
+  // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
+  // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
+  // found a slot further out this step, else one cell (theta_cell_range)
+  const int2 *tr = (*tfar ? B.trange : B.trange_tight) + Sc.troff;
   auto xrange = [&](int i, int &cx0, int &cx1) {
     const int2 r = tr[i];
     cx0 = r.x;
@@ -279,7 +280,8 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
 // node sums of the theta-cell partials, per scene (workgroups dealt in order
 // from Scene32::nboff)
 __global__ void k_theta_nodes_b(const double4 *__restrict__ part, Batch B,
-                                double *__restrict__ gtheta) {
+                                double *__restrict__ gtheta, int *__restrict__ tfar) {
+  if (tfar && blockIdx.x == 0 && threadIdx.x == 0) *tfar = 0;   // for the next step's R pass
   const Scene32 &Sc = B.sc[B.nb2s[blockIdx.x]];
   const int l = (blockIdx.x - Sc.nboff) * blockDim.x + threadIdx.x;
   const int ngx = Sc.T.ngx, ngy = Sc.T.ngy;
@@ -328,11 +330,11 @@ void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galph
 }
 
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, const Batch &B,
-                                   double4 *part, double *gtheta) {
+                                   const float *galpha, const int *cstart, int *tfar,
+                                   const Batch &B, double4 *part, double *gtheta) {
   static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
-  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, B, part);
-  k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta);
+  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, tfar, B, part);
+  k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta, tfar);
 }
 
 }  // namespace sph

@@ -7,6 +7,7 @@ same interaction set as FP64 (positions stay within ~1e-9 of FP64 over tens of
 steps).  Measured errors are printed for the record.
 """
 import sys
+import warnings
 
 import numpy as np
 import pytest
@@ -119,3 +120,39 @@ def test_f32c_repeat_runs_bit_identical(be):
         S.simulate_(st, sc.theta, pg, be, 100)
         Js.append(S.target_objective(st, 0.3, 0.2, 0.1)[0])
     assert Js[0] == Js[1] == Js[2]
+
+
+@pytest.mark.parametrize("interval", [1, 20])
+def test_f32c_gtheta_with_fast_particles(be, interval):
+    """The keyed gtheta gather sweeps, for each theta cell, the SPH cells whose
+    slots can carry its key: a quarter cell around it, or one cell when the R
+    pass finds a slot further from its binning cell.  Fast particles with a
+    long rebuild interval (interval 20, displacements of several quarter cells
+    between rebuilds) take the wider sweep; every contribution must still be
+    counted: gtheta of the f32c form against the FP64 path (whose gather is the
+    oracle's to 1e-12) on every node carrying more than 1e-3 of the largest."""
+    say("f32c gtheta, fast particles, interval", interval)
+    sc = scenes.gradcheck_5x5(seed=4, nsteps=40)
+    rng = np.random.default_rng(4)
+    V0 = 5.0 * rng.standard_normal(sc.V0.shape)
+    p = S.SPHParams(**sc.params)
+    out = {}
+    for prec in ("f64", "f32c"):
+        st = S.State(be, sc.X0, V0, p, interval=interval, precision=prec)
+        tape = S.Tape(be, sc.N, sc.nsteps, p, precision=prec)
+        with warnings.catch_warnings():
+            # the long interval raises the reference's displacement warning
+            warnings.simplefilter("ignore")
+            S.simulate_(st, sc.theta, p, be, sc.nsteps, tape=tape)
+        J, seed = S.target_objective(st, 0.5, 0.5, 0.3)
+        ws = S.AdjointWorkspace(be, sc.N, p, precision=prec)
+        S.backward_(ws, tape, sc.theta, p, be, seedX=seed, seedV=np.zeros_like(seed),
+                    interval=interval)
+        out[prec] = (J, ws.gtheta)
+    g64, g32 = out["f64"][1], out["f32c"][1]
+    big = np.abs(g64) > 1e-3 * np.abs(g64).max()
+    err = float((np.abs(g32 - g64)[big] / np.abs(g64)[big]).max())
+    print(f"   J {out['f64'][0]:.6e} vs {out['f32c'][0]:.6e}; gtheta rel err {err:.2e}")
+    assert big.sum() >= 10
+    assert err < 1e-3
+
