@@ -572,7 +572,7 @@ void do_build(sph_state *st) {
   const int N = (int)st->N;
   sph_ctx *c = st->ctx;
   timed(c, K_ASSIGN, [&] {
-    launch_fill_i32(s, st->counts, st->g.ncell, 0);
+    // counts are zero here: cleared at creation, then by each build's scan
     if (st->precision == SPH_F32C) launch_assign_f32c(s, st->X, st->bs.B, st->cellof, st->counts);
     else launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts);
   });
@@ -1387,10 +1387,10 @@ int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *
     std::vector<int> cnt(nc), stt(nc + 1), ord(n);
     const int *dstart = st->cl_view == 1 ? st->cl_cstart : st->cstart;
     const int *dorder = st->cl_view == 1 ? st->new_ids : st->ids;
-    HIPC(hipMemcpyAsync(cnt.data(), st->counts, sizeof(int) * nc, hipMemcpyDeviceToHost, s));
     HIPC(hipMemcpyAsync(stt.data(), dstart, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, s));
     HIPC(hipMemcpyAsync(ord.data(), dorder, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    for (size_t c = 0; c < nc; ++c) cnt[c] = stt[c + 1] - stt[c];
     if (counts) std::memcpy(counts, cnt.data(), sizeof(int) * nc);
     if (starts) std::memcpy(starts, stt.data(), sizeof(int) * nc);
     if (order)

@@ -88,32 +88,28 @@ __global__ void k_scan_partial(const int *__restrict__ in, int n, int *__restric
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// one block scans the block sums in place (exclusive), chunk by chunk
-__global__ void k_scan_bsum(int *__restrict__ bsum, int nb) {
-  int carry = 0;
-  for (int base = 0; base < nb; base += kBlock) {
-    int i = base + threadIdx.x;
-    int v = (i < nb) ? bsum[i] : 0;
-    int total;
-    int ex = block_excl_scan(v, total);
-    if (i < nb) bsum[i] = carry + ex;
-    carry += total;
-  }
-}
-
-__global__ void k_scan_final(const int *__restrict__ in, int n, const int *__restrict__ bsum,
+// Exclusive scan of one tile; the tile's offset is the sum of the partial sums
+// of the tiles before it (each block adds them itself: integers, so any order
+// gives the same sum).  The counts are cleared once read, ready for the next
+// build's atomics.
+__global__ void k_scan_final(int *__restrict__ in, int n, const int *__restrict__ bsum,
                              int *__restrict__ out, int *__restrict__ cursor) {
   const int base = blockIdx.x * kScanTile;
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) pre += bsum[b];
+  int tile_off;
+  block_excl_scan(pre, tile_off);
   int vals[kScanItems];
   int v = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     int i = base + threadIdx.x * kScanItems + k;
     vals[k] = (i < n) ? in[i] : 0;
+    if (i < n) in[i] = 0;
     v += vals[k];
   }
   int total;
-  int ex = block_excl_scan(v, total) + bsum[blockIdx.x];
+  int ex = block_excl_scan(v, total) + tile_off;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     int i = base + threadIdx.x * kScanItems + k;
@@ -255,10 +251,9 @@ void launch_fill_i32(hipStream_t s, int *p, int n, int v) {
 void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts) {
   if (g.N > 0) k_assign<<<nblocks(g.N), kBlock, 0, s>>>(X, g, cs, cellof, counts);
 }
-void launch_scan_excl(hipStream_t s, const int *in, int n, int *out, int *cursor, int *bsum) {
+void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum) {
   int nb = (n + kScanTile - 1) / kScanTile;
   k_scan_partial<<<nb, kBlock, 0, s>>>(in, n, bsum);
-  k_scan_bsum<<<1, kBlock, 0, s>>>(bsum, nb);
   k_scan_final<<<nb, kBlock, 0, s>>>(in, n, bsum, out, cursor);
 }
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,

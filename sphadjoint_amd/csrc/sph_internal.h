@@ -142,7 +142,8 @@ __device__ __forceinline__ NbRow nb_row(const int *__restrict__ nl,
 // cell list (sph_cells.hip)
 void launch_fill_i32(hipStream_t s, int *p, int n, int v);
 void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts);
-void launch_scan_excl(hipStream_t s, const int *in, int n, int *out, int *cursor, int *bsum);
+// exclusive prefix of in[0..n) into out[0..n] (and cursor); clears in
+void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum);
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
                  int *tmp_id, int *tmp_slot);
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
