@@ -72,7 +72,7 @@ SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint
                    "adjoint_r": "k_adj_r32", "adjoint_p1": "k_adj_p132<true>",
                    "adjoint_p2": "k_adj_p232"},
           "f32c_ckpt": {"density": "k_density32", "force_integrate": "k_force32",
-                        "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<false>",
+                        "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
                         "adjoint_p2": "k_adj_p232"}}
 
 
