@@ -97,3 +97,34 @@ def test_autotune_keeps_results(be):
     assert outs[0][0] == outs[1][0]
     for a, b in zip(outs[0][1:], outs[1][1:]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("size", [64, 128])
+def test_small_workgroups_stage_the_whole_window(be, size):
+    """With 64- or 128-thread workgroups the pair passes still deal each
+    256-slot window by height, so every workgroup stages its whole window's
+    rows: forward + adjoint equal the 256-thread launches bit for bit, on a
+    batch (scene-local windows)."""
+    say("workgroup size", size)
+    scs = scenes.batch_line_search(seed=5, nscenes=3, nmin=800, nmax=6000, nsteps=16)
+    args = [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs]
+    outs = []
+    for sz in (256, size):
+        bst = S.BatchState(be, args)
+        names = ("nl_build", "density", "force_integrate", "adjoint_r", "adjoint_p1",
+                 "adjoint_p2")
+        S.tuning(bst, {n: sz for n in names})
+        th = bst.field([sc.theta for sc in scs])
+        tape = S.Tape(be, bst.N, 16, bst.p, precision="f32c")
+        S.simulate_(bst, th, bst.p, be, 16, tape=tape)
+        J, seed = S.batch_objective(bst, 0.3, 0.2, 0.1)
+        ws = S.BatchWorkspace(be, bst)
+        S.backward_(ws, tape, th, bst.p, be, seedX=seed, seedV=np.zeros_like(seed))
+        outs.append((bst.X, J, ws.gX, ws.gV, ws.gtheta))
+    a, b = outs
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+    np.testing.assert_array_equal(a[3], b[3])
+    for x, y in zip(a[4], b[4]):
+        np.testing.assert_array_equal(x, y)
