@@ -624,7 +624,7 @@ void do_build_neighbors(sph_state *st, bool defer_list = false) {
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_PERMUTE, [&] {
       launch_permute_f32c(s, st->perm, st->cellof, st->X, st->V32, st->Xref, check, st->bs.B,
-                          st->Xn, st->V32n, st->xr, st->Xref2, st->scell, st->flags);
+                          st->Xn, st->V32n, st->xr, st->Xref2, st->scell, st->flags, st->lane);
     });
     std::swap(st->V32, st->V32n);
   } else {
@@ -641,10 +641,10 @@ void do_build_neighbors(sph_state *st, bool defer_list = false) {
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C) {
+      // (the permute above has dealt the new thread order)
       if (!defer_list)
         launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
                              st->flags);
-      launch_lane_order_f32c(s, st->xr, st->bs.B, st->lane);
     } else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
@@ -1898,13 +1898,12 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       timed(c, K_NLBUILD, [&] {
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
-          launch_xrel_f32c(s, X0, E.scell, B, ws->xr32);
-          // a tape with each step's exact masks needs no mask list, only the
-          // thread order of the epoch
+          // the thread order of the epoch with its relative coordinates; a
+          // tape with each step's exact masks needs no mask list
+          launch_xrel_f32c(s, X0, E.scell, B, ws->xr32, ws->lane);
           if (!tape->EM)
             launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
                                  ws->flags);
-          launch_lane_order_f32c(s, ws->xr32, B, ws->lane);
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);

@@ -96,9 +96,6 @@ void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *ce
                         int *counts);
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags);
-// thread -> slot order of the pair passes for the positions xr of a rebuild
-// (B: the 256-slot geometry, Bv[2]); every pair launcher takes it as `lane`
-void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane);
 // first density pass after a rebuild, fused with the mask-list build: writes
 // the mask list nm, the exact list em, PI and the overflow flag (flags[0])
 void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
@@ -115,12 +112,16 @@ void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                        const int *cstart, const int *lane, const Batch &B, double2 *Xn,
                        float2 *xrn, float2 *Vn, double2 *tapeX, float2 *tapeV, double2 *a_out,
                        double *alpha_out);
+// relative coordinates xr of the slots; with `lane` (B: the 256-slot
+// geometry, Bv[2]) also the thread -> slot order of the pair passes for these
+// positions, which every pair launcher takes as `lane`
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
-                      float2 *xr);
+                      float2 *xr, int *lane = nullptr);
+// the slot reorder of a rebuild, and with `lane` the new thread order as above
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                          const float2 *V, const double2 *Xref, int check, const Batch &B,
                          double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
-                         int *flags);
+                         int *flags, int *lane);
 // R passes also write each slot's interp-cell key and FP64 fractions for the
 // keyed gtheta gather
 // (both R passes also raise *tfar when a slot lies beyond its interp cell's

@@ -176,7 +176,7 @@ __device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
   const PhysF32 &P = S.P
 
 // Pair passes: thread sl_ of the scene takes the slot lane[off + sl_] (see
-// k_lane_order), a permutation of each aligned window of kLaneWin slots of the
+// lane_order_window), a permutation of each aligned window of kLaneWin slots of the
 // scene, so that the lanes of a wavefront sweep rows of similar length; each
 // slot's arithmetic is unchanged by this assignment.  w0_..w1_ is the window
 // of this workgroup's slots.  Threads past the scene's end (valid_ false) take
@@ -500,21 +500,21 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
 // order of that height (16 classes, then slot order), so that each wavefront
 // holds slots of similar height and similar row counts.  The order is fixed
 // (ballot ranks, no atomics) and changes which thread evaluates a slot, never
-// what is evaluated.
+// what is evaluated.  The kernels that form a window's relative coordinates
+// (k_permute32 at a rebuild, k_xrel at an epoch of the reverse sweep) deal it
+// on the way: their workgroups of kLaneWin threads are the windows.
 constexpr int kLaneClasses = 16;
 
-__global__ void __launch_bounds__(kLaneWin)
-k_lane_order(const float2 *__restrict__ xr, Batch B, int *__restrict__ lane) {
+// The window [base, base + n) (0 < n <= kLaneWin) of a workgroup of kLaneWin
+// threads: thread t < n gives the relative height yr of slot base + t.  Every
+// thread of the workgroup calls it (it holds a barrier).
+__device__ __forceinline__ void lane_order_window(float yr, float cs, int base, int n,
+                                                  int *__restrict__ lane) {
   __shared__ int wcnt[kLaneWin / 64][kLaneClasses];
-  const int blk = blockIdx.x;
-  const int scn = B.b2s[blk];
-  const Scene32 &S = B.sc[scn];
-  const int base = S.off + (blk - B.boff[scn]) * kLaneWin;
-  const int n = min(kLaneWin, S.off + S.g.N - base);
   const int t = (int)threadIdx.x, w = t >> 6, l = t & 63;
   int key = kLaneClasses;   // threads past the window's end take no class
   if (t < n) {
-    const float y = xr[base + t].y * ((float)kLaneClasses / S.P.cs);
+    const float y = yr * ((float)kLaneClasses / cs);
     key = !(y > 0.f) ? 0 : (y >= (float)(kLaneClasses - 1) ? kLaneClasses - 1 : (int)y);
   }
   int below = 0;   // lanes of this wavefront below this one with the same class
@@ -757,35 +757,60 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   }
 }
 
-__global__ void k_xrel(const double2 *__restrict__ X, const int *__restrict__ bcell, Batch B,
-                       float2 *__restrict__ xr) {
-  SCENE_SLOT(B);
-  xr[s] = rel_coord(X[s], bcell[s], g, P);
+// relative coordinates of the slots; with `lane`, also their thread order
+// (the workgroups are then the kLaneWin-slot windows)
+__global__ void __launch_bounds__(kBlock)
+k_xrel(const double2 *__restrict__ X, const int *__restrict__ bcell, Batch B,
+       float2 *__restrict__ xr, int *__restrict__ lane) {
+  const int blk = xcd_block();
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const int bsl = (blk - B.boff[scn]) * B.bs;
+  const int sl = bsl + (int)threadIdx.x;
+  float yr = 0.f;
+  if (sl < S.g.N) {
+    const int s = S.off + sl;
+    const float2 r = rel_coord(X[s], bcell[s], S.g, S.P);
+    xr[s] = r;
+    yr = r.y;
+  }
+  if (lane) lane_order_window(yr, S.P.cs, S.off + bsl, min(B.bs, S.g.N - bsl), lane);
 }
 
-__global__ void k_permute32(const int *__restrict__ perm, const int *__restrict__ cellof,
-                            const double2 *__restrict__ X, const float2 *__restrict__ V,
-                            const double2 *__restrict__ Xref, int check, Batch B,
-                            double2 *__restrict__ Xn, float2 *__restrict__ Vn,
-                            float2 *__restrict__ xr, double2 *__restrict__ Xrefn,
-                            int *__restrict__ bcell, int *__restrict__ flags) {
+// the slot reorder of a rebuild; with `lane`, also the new thread order
+__global__ void __launch_bounds__(kBlock)
+k_permute32(const int *__restrict__ perm, const int *__restrict__ cellof,
+            const double2 *__restrict__ X, const float2 *__restrict__ V,
+            const double2 *__restrict__ Xref, int check, Batch B, double2 *__restrict__ Xn,
+            float2 *__restrict__ Vn, float2 *__restrict__ xr, double2 *__restrict__ Xrefn,
+            int *__restrict__ bcell, int *__restrict__ flags, int *__restrict__ lane) {
   // slots keep their scene through the reorder: scenes own contiguous cell
   // ranges, and cells own contiguous slot ranges
-  SCENE_SLOT(B);
-  const int t = s;
-  int o = perm[t];
-  double2 x = X[o];
-  if (check) {
-    double2 r = Xref[o];
-    double ddx = x.x - r.x, ddy = x.y - r.y;
-    if (ddx * ddx + ddy * ddy > S.dmax2) flags[1] = 1;
+  const int blk = xcd_block();
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const int bsl = (blk - B.boff[scn]) * B.bs;
+  const int sl = bsl + (int)threadIdx.x;
+  float yr = 0.f;
+  if (sl < S.g.N) {
+    const int t = S.off + sl;
+    int o = perm[t];
+    double2 x = X[o];
+    if (check) {
+      double2 r = Xref[o];
+      double ddx = x.x - r.x, ddy = x.y - r.y;
+      if (ddx * ddx + ddy * ddy > S.dmax2) flags[1] = 1;
+    }
+    int c = cellof[o] - S.coff;     // the scene's own cell numbering
+    Xn[t] = x;
+    Vn[t] = V[o];
+    Xrefn[t] = x;
+    bcell[t] = c;
+    const float2 r = rel_coord(x, c, S.g, S.P);
+    xr[t] = r;
+    yr = r.y;
   }
-  int c = cellof[o] - S.coff;     // the scene's own cell numbering
-  Xn[t] = x;
-  Vn[t] = V[o];
-  Xrefn[t] = x;
-  bcell[t] = c;
-  xr[t] = rel_coord(x, c, g, P);
+  if (lane) lane_order_window(yr, S.P.cs, S.off + bsl, min(B.bs, S.g.N - bsl), lane);
 }
 
 // cell of every slot (the scene's grid, the division rule of cell_index,
@@ -1041,10 +1066,6 @@ void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, con
                           const Batch &B, uint32_t *nm, int *flags) {
   if (B.N > 0) k_nm_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, B, nm, flags);
 }
-void launch_lane_order_f32c(hipStream_t s, const float2 *xr, const Batch &B, int *lane) {
-  if (B.bs != kLaneWin) throw std::logic_error("lane order needs the 256-slot geometry");
-  if (B.N > 0) k_lane_order<<<B.nblk, kLaneWin, 0, s>>>(xr, B, lane);
-}
 void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
                                const int *cstart, const int *lane, const Batch &B, float2 *PI,
                                uint32_t *nm, uint32_t *em, int *flags) {
@@ -1066,16 +1087,18 @@ void launch_force_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                                         Vn, tapeX, tapeV, a_out, alpha_out);
 }
 void launch_xrel_f32c(hipStream_t s, const double2 *X, const int *bcell, const Batch &B,
-                      float2 *xr) {
-  if (B.N > 0) k_xrel<<<B.nblk, B.bs, 0, s>>>(X, bcell, B, xr);
+                      float2 *xr, int *lane) {
+  if (lane && B.bs != kLaneWin) throw std::logic_error("the thread order needs window-sized workgroups");
+  if (B.N > 0) k_xrel<<<B.nblk, B.bs, 0, s>>>(X, bcell, B, xr, lane);
 }
 void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                          const float2 *V, const double2 *Xref, int check, const Batch &B,
                          double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
-                         int *flags) {
+                         int *flags, int *lane) {
+  if (lane && B.bs != kLaneWin) throw std::logic_error("the thread order needs window-sized workgroups");
   if (B.N > 0)
     k_permute32<<<B.nblk, B.bs, 0, s>>>(perm, cellof, X, V, Xref, check, B, Xn, Vn, xr, Xrefn,
-                                          bcell, flags);
+                                          bcell, flags, lane);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
