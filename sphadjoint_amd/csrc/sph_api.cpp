@@ -532,7 +532,9 @@ struct sph_workspace {
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
   int *lane = nullptr;                                     // f32c form: pair-pass thread order
   int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
-  int *tfar = nullptr;        // f32c form: a slot lies beyond its key's tight SPH range
+  int *tfar = nullptr;        // f32c form: stamp of the last step in which a slot lay
+                              // beyond its key's tight SPH range
+  int tstamp = 0;             // f32c form: stamp of the current step (1, 2, ...)
   double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int64_t ntnode = 0, ntcell = 0;  // design-field nodes and cells (all scenes)
@@ -1866,6 +1868,15 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const BatchStore &BS = tape->bs;
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
   int cur = tape->entry_epoch[tape->n];
+  // gtheta: the theta-cell partials of an epoch's steps add up in tpart; its
+  // node sums go to gtheta when the sweep leaves the epoch
+  int tepoch = -1;
+  auto theta_nodes = [&] {
+    timed(c, K_THETA, [&] {
+      if (f32c) launch_theta_nodes_keyed_f32(s, ws->tpart, B, ws->gtheta);
+      else launch_theta_nodes_f64(s, ws->tpart, T, ws->gtheta);
+    });
+  };
   // the adjoint state into the slot order of the final entry
   timed(c, K_REMAP, [&] {
     launch_inverse(s, ids_in, N, ws->inv);
@@ -1893,6 +1904,9 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
     const TapeEpoch &E = tape->epochs[e];
     const size_t off = (size_t)(k - 1) * n;
     const double2 *Xk = tape->X + off;
+    const bool tfirst = e != tepoch;
+    if (tfirst && tepoch >= 0) theta_nodes();
+    tepoch = e;
     if (e != listed) {
       // the epoch's list, rebuilt from the positions of its first entry
       timed(c, K_NLBUILD, [&] {
@@ -1933,21 +1947,23 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         cur = e;
       }
       const float2 *PIk = ws->PI32;
+      ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
           launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32, ws->gX32,
-                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy, ws->tfar);
+                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy, ws->tfar,
+                              ws->tstamp);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
           launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
                             BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
-                            ws->galpha32, ws->em, ws->tkey, ws->tfxy, ws->tfar);
+                            ws->galpha32, ws->em, ws->tkey, ws->tfxy, ws->tfar, ws->tstamp);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, ws->tfar, B,
-                                      ws->tpart, ws->gtheta);
+        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, ws->tfar,
+                                      ws->tstamp, B, tfirst, ws->tpart);
       });
       // the exact list of these positions: the forward's density pass kept it
       // on the tape, or the density recompute just wrote it; with stored
@@ -1993,7 +2009,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_f64(s, Xk, ws->galpha, E.cstart, g, T, ws->tpart, ws->gtheta);
+        launch_theta_cells_f64(s, Xk, ws->galpha, E.cstart, g, T, tfirst, ws->tpart);
       });
       timed(c, K_ADJ_P1, [&] {
         launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, ws->gXs,
@@ -2004,6 +2020,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       });
     }
   }
+  if (tepoch >= 0) theta_nodes();
   HIPC(hipMemcpyAsync(ws->ids_cur, tape->epochs[cur].ids, sizeof(int) * n,
                       hipMemcpyDeviceToDevice, s));
 }

@@ -124,17 +124,17 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          int *flags, int *lane);
 // R passes also write each slot's interp-cell key and FP64 fractions for the
 // keyed gtheta gather
-// (both R passes also raise *tfar when a slot lies beyond its interp cell's
-// tight SPH range, Batch::trange_tight)
+// (both R passes also store tstamp in *tfar when a slot lies beyond its interp
+// cell's tight SPH range, Batch::trange_tight; tstamp differs from step to step)
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, double2 *tfxy, int *tfar);
+                       int *tkey, double2 *tfxy, int *tfar, int tstamp);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy,
-                         int *tfar);
+                         int *tfar, int tstamp);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
@@ -146,11 +146,16 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
                         const Batch &B, float2 *gX);
-// deterministic gtheta gather over the keys of the R passes (sph_theta.hip)
-// sweeps the tight SPH ranges unless *tfar is set; clears *tfar for the next step
+// deterministic gtheta gather over the keys of the R passes (sph_theta.hip):
+// the theta-cell partials of one step, written (first) or added to part; the
+// gather sweeps the tight SPH ranges unless this step's R pass stored its
+// stamp in *tfar.  launch_theta_nodes_keyed_f32 adds the node sums of part to
+// gtheta (once per neighbour-list epoch).
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, int *tfar,
-                                   const Batch &B, double4 *part, double *gtheta);
+                                   const float *galpha, const int *cstart, const int *tfar,
+                                   int tstamp, const Batch &B, bool first, double4 *part);
+void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batch &B,
+                                  double *gtheta);
 
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);

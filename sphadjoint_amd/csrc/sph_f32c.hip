@@ -838,16 +838,17 @@ k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
   if (c >= 0 && head_lane == lane) atomicAdd(&counts[c], run_len);
 }
 
-// raises *tfar when slot s (binning cell bc) lies beyond the tight SPH range
-// of its interp cell ic, so that the gtheta gather sweeps the wider ranges
+// stores this step's stamp in *tfar when slot s (binning cell bc) lies beyond
+// the tight SPH range of its interp cell ic, so that the step's gtheta gather
+// sweeps the wider ranges (a stamp of an earlier step selects nothing)
 __device__ __forceinline__ void check_tight(const InterpF &ic, int bc, const Grid &g,
                                             const PhysF32 &P, const Scene32 &S, const Batch &B,
-                                            int *tfar) {
+                                            int *tfar, int tstamp) {
   const int i0 = ic.l00 / P.ngy, j0 = ic.l00 - i0 * P.ngy;
   const int2 c = cell_xy(bc, g);
   const int2 rx = B.trange_tight[S.troff + i0];
   const int2 ry = B.trange_tight[S.troff + (P.ngx - 1) + j0];
-  if (c.x < rx.x || c.x > rx.y || c.y < ry.x || c.y > ry.y) *tfar = 1;
+  if (c.x < rx.x || c.x > rx.y || c.y < ry.x || c.y > ry.y) *tfar = tstamp;
 }
 
 // adjoint R: density recompute, _adj_integrate!, _adj_drag! (src/adjoint.jl:193-228)
@@ -858,7 +859,8 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
           float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
           float2 *__restrict__ abar, float *__restrict__ galpha, uint32_t *__restrict__ em,
-          int *__restrict__ tkey, double2 *__restrict__ tfxy, int *__restrict__ tfar) {
+          int *__restrict__ tkey, double2 *__restrict__ tfxy, int *__restrict__ tfar,
+          int tstamp) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
@@ -872,7 +874,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
-  check_tight(ic, bcell[s], g, P, S, B, tfar);
+  check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
   tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
@@ -892,13 +894,13 @@ k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
             const float2 *__restrict__ Vnext, const double *__restrict__ theta, Batch B,
             float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
             float2 *__restrict__ abar, float *__restrict__ galpha, int *__restrict__ tkey,
-            double2 *__restrict__ tfxy, int *__restrict__ tfar) {
+            double2 *__restrict__ tfxy, int *__restrict__ tfar, int tstamp) {
   SCENE_SLOT(B);
   const double2 x64 = X[s];
   xr[s] = rel_coord(x64, bcell[s], g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
-  check_tight(ic, bcell[s], g, P, S, B, tfar);
+  check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
   tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
@@ -1104,18 +1106,18 @@ void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, double2 *tfxy, int *tfar) {
+                       int *tkey, double2 *tfxy, int *tfar, int tstamp) {
   if (B.N > 0)
     k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, lane, B, PI, gX,
-                                        gV, abar, galpha, em, tkey, tfxy, tfar);
+                                        gV, abar, galpha, em, tkey, tfxy, tfar, tstamp);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy,
-                         int *tfar) {
+                         int *tfar, int tstamp) {
   if (B.N > 0)
     k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
-                                          tkey, tfxy, tfar);
+                                          tkey, tfxy, tfar, tstamp);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,

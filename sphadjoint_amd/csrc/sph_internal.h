@@ -189,13 +189,12 @@ void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, dou
                           double2 *seed);
 int objective_blocks(int N);
 
-// deterministic gtheta gather (sph_theta.hip)
-void launch_theta_gather_f64(hipStream_t s, const double2 *X, const double *galpha,
-                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
-                             double *gtheta);
-void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galpha,
-                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
-                             double *gtheta);
+// deterministic gtheta gather (sph_theta.hip): the theta-cell partials of one
+// step, written (first step of a neighbour-list epoch) or added to part; the
+// node sums of part are added to gtheta once per epoch (launch_theta_nodes_f64)
+void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
+                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part);
+void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta);
 
 // design-field chain of the topology optimiser (sph_design.hip): an
 // (ngy, ngx) grid, the density filter radius r (grid points) and the

@@ -10,8 +10,10 @@
 //      as the reference does, and sum the four weighted contributions
 //      (ga*(1-fx))*(1-fy), (ga*fx)*(1-fy), (ga*(1-fx))*fy, (ga*fx)*fy;
 //      the lanes take candidates round-robin and combine by a fixed butterfly;
-//   B  one thread per theta node: add the partials of the (up to) four cells
-//      that touch the node, in a fixed order.
+//      The partials of the steps of one neighbour-list epoch add up in
+//      the partial array (the epoch's first step writes it);
+//   B  once per epoch, one thread per theta node: add the partials of the
+//      (up to) four cells that touch the node, in a fixed order, to gtheta.
 // Every contribution is the reference's product, bit for bit; only the order
 // of the sum is fixed.  FP64 throughout; compiled with -ffp-contract=off.
 #pragma clang fp contract(off)
@@ -33,7 +35,8 @@ constexpr int kLanes = 8;   // lanes that share one theta cell
 template <class GA>
 __global__ void __launch_bounds__(kBlock)
 k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
-              const int *__restrict__ cstart, Grid g, ThetaGrid T, double4 *__restrict__ part) {
+              const int *__restrict__ cstart, Grid g, ThetaGrid T, int first,
+              double4 *__restrict__ part) {
   const int ncx = T.ngx - 1, ncy = T.ngy - 1;
   const int lane = threadIdx.x & (kLanes - 1);
   // theta cells in x-fastest order, so that neighbouring groups read
@@ -91,7 +94,15 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
     s01 += __shfl_xor(s01, m, kLanes);
     s11 += __shfl_xor(s11, m, kLanes);
   }
-  if (lane == 0) part[j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
+  if (lane == 0) {
+    double4 &pt = part[j0 + i0 * ncy];
+    if (first) {
+      pt = make_double4(s00, s10, s01, s11);
+    } else {
+      const double4 o = pt;
+      pt = make_double4(o.x + s00, o.y + s10, o.z + s01, o.w + s11);
+    }
+  }
 }
 
 // The same gather over per-slot keys written by the f32c adjoint's pointwise
@@ -119,7 +130,8 @@ template <class GA>
 __global__ void __launch_bounds__(kBlock)
 k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
                     const GA *__restrict__ galpha, const int *__restrict__ cstart,
-                    const int *__restrict__ tfar, Batch B, double4 *__restrict__ part) {
+                    const int *__restrict__ tfar, int tstamp, Batch B, int first,
+                    double4 *__restrict__ part) {
   static_assert(kThetaStrip * kLanes == kBlock, "one strip per workgroup");
   __shared__ int skey[kThetaStage];
   __shared__ int slo[kThetaRows], soff[kThetaRows + 1];
@@ -142,7 +154,7 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
   // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
   // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
   // found a slot further out this step, else one cell (theta_cell_range)
-  const int2 *tr = (*tfar ? B.trange : B.trange_tight) + Sc.troff;
+  const int2 *tr = (*tfar == tstamp ? B.trange : B.trange_tight) + Sc.troff;
   auto xrange = [&](int i, int &cx0, int &cx1) {
     const int2 r = tr[i];
     cx0 = r.x;
@@ -274,14 +286,21 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
     s01 += __shfl_xor(s01, m, kLanes);
     s11 += __shfl_xor(s11, m, kLanes);
   }
-  if (lane == 0) part[tcoff + j0 + i0 * ncy] = make_double4(s00, s10, s01, s11);
+  if (lane == 0) {
+    double4 &pt = part[tcoff + j0 + i0 * ncy];
+    if (first) {
+      pt = make_double4(s00, s10, s01, s11);
+    } else {
+      const double4 o = pt;
+      pt = make_double4(o.x + s00, o.y + s10, o.z + s01, o.w + s11);
+    }
+  }
 }
 
 // node sums of the theta-cell partials, per scene (workgroups dealt in order
 // from Scene32::nboff)
 __global__ void k_theta_nodes_b(const double4 *__restrict__ part, Batch B,
-                                double *__restrict__ gtheta, int *__restrict__ tfar) {
-  if (tfar && blockIdx.x == 0 && threadIdx.x == 0) *tfar = 0;   // for the next step's R pass
+                                double *__restrict__ gtheta) {
   const Scene32 &Sc = B.sc[B.nb2s[blockIdx.x]];
   const int l = (blockIdx.x - Sc.nboff) * blockDim.x + threadIdx.x;
   const int ngx = Sc.T.ngx, ngy = Sc.T.ngy;
@@ -313,28 +332,28 @@ __global__ void k_theta_nodes(const double4 *__restrict__ part, ThetaGrid T,
 
 }  // namespace
 
-void launch_theta_gather_f64(hipStream_t s, const double2 *X, const double *galpha,
-                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
-                             double *gtheta) {
+void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
+                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part) {
   const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
-  k_theta_cells<double><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T, part);
-  k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
+  k_theta_cells<double><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T,
+                                                                    first ? 1 : 0, part);
 }
 
-void launch_theta_gather_f32(hipStream_t s, const double2 *X, const float *galpha,
-                             const int *cstart, Grid g, ThetaGrid T, double4 *part,
-                             double *gtheta) {
-  const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
-  k_theta_cells<float><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T, part);
+void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta) {
   k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
 }
 
 void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
-                                   const float *galpha, const int *cstart, int *tfar,
-                                   const Batch &B, double4 *part, double *gtheta) {
+                                   const float *galpha, const int *cstart, const int *tfar,
+                                   int tstamp, const Batch &B, bool first, double4 *part) {
   static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
-  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, tfar, B, part);
-  k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta, tfar);
+  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, tfar, tstamp, B,
+                                                        first ? 1 : 0, part);
+}
+
+void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batch &B,
+                                  double *gtheta) {
+  k_theta_nodes_b<<<B.nnblk, kBlock, 0, s>>>(part, B, gtheta);
 }
 
 }  // namespace sph
