@@ -535,7 +535,6 @@ struct sph_workspace {
   int *tfar = nullptr;        // f32c form: stamp of the last step in which a slot lay
                               // beyond its key's tight SPH range
   int tstamp = 0;             // f32c form: stamp of the current step (1, 2, ...)
-  double2 *tfxy = nullptr;    // f32c form: FP64 interp fractions of each slot
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int64_t ntnode = 0, ntcell = 0;  // design-field nodes and cells (all scenes)
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
@@ -1669,7 +1668,6 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->lane);
   dfree(ws->tkey);
   dfree(ws->tfar);
-  dfree(ws->tfxy);
 }
 
 }  // extern "C"
@@ -1841,7 +1839,6 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
       ws->tfar = dalloc<int>(1);
       HIPC(hipMemsetAsync(ws->tfar, 0, sizeof(int), ws->ctx->stream));
     }
-    if (!ws->tfxy) ws->tfxy = dalloc<double2>((size_t)ws->N);
     return;
   }
   if (ws->nl_cap < maxnb) {
@@ -1952,17 +1949,17 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         if (tape->keep_pi) {
           PIk = tape->PI32 + off;
           launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32, ws->gX32,
-                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfxy, ws->tfar,
+                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfar,
                               ws->tstamp);
         } else {
           launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
           launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
                             BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
-                            ws->galpha32, ws->em, ws->tkey, ws->tfxy, ws->tfar, ws->tstamp);
+                            ws->galpha32, ws->em, ws->tkey, ws->tfar, ws->tstamp);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_gather_keyed_f32(s, ws->tkey, ws->tfxy, ws->galpha32, E.cstart, ws->tfar,
+        launch_theta_gather_keyed_f32(s, ws->tkey, Xk, ws->galpha32, E.cstart, ws->tfar,
                                       ws->tstamp, B, tfirst, ws->tpart);
       });
       // the exact list of these positions: the forward's density pass kept it

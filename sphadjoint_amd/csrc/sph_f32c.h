@@ -122,19 +122,18 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          const float2 *V, const double2 *Xref, int check, const Batch &B,
                          double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
                          int *flags, int *lane);
-// R passes also write each slot's interp-cell key and FP64 fractions for the
-// keyed gtheta gather
+// R passes also write each slot's interp-cell key for the keyed gtheta gather
 // (both R passes also store tstamp in *tfar when a slot lies beyond its interp
 // cell's tight SPH range, Batch::trange_tight; tstamp differs from step to step)
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, double2 *tfxy, int *tfar, int tstamp);
+                       int *tkey, int *tfar, int tstamp);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy,
-                         int *tfar, int tstamp);
+                         float2 *gV, float2 *abar, float *galpha, int *tkey, int *tfar,
+                         int tstamp);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
@@ -151,7 +150,7 @@ void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, cons
 // gather sweeps the tight SPH ranges unless this step's R pass stored its
 // stamp in *tfar.  launch_theta_nodes_keyed_f32 adds the node sums of part to
 // gtheta (once per neighbour-list epoch).
-void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
+void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *X,
                                    const float *galpha, const int *cstart, const int *tfar,
                                    int tstamp, const Batch &B, bool first, double4 *part);
 void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batch &B,

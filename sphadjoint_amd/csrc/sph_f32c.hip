@@ -859,8 +859,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
           float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
           float2 *__restrict__ abar, float *__restrict__ galpha, uint32_t *__restrict__ em,
-          int *__restrict__ tkey, double2 *__restrict__ tfxy, int *__restrict__ tfar,
-          int tstamp) {
+          int *__restrict__ tkey, int *__restrict__ tfar, int tstamp) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
@@ -875,7 +874,6 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
   check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
-  tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
@@ -894,14 +892,13 @@ k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
             const float2 *__restrict__ Vnext, const double *__restrict__ theta, Batch B,
             float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
             float2 *__restrict__ abar, float *__restrict__ galpha, int *__restrict__ tkey,
-            double2 *__restrict__ tfxy, int *__restrict__ tfar, int tstamp) {
+            int *__restrict__ tfar, int tstamp) {
   SCENE_SLOT(B);
   const double2 x64 = X[s];
   xr[s] = rel_coord(x64, bcell[s], g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
   check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
-  tfxy[s] = make_double2(ic.fx64, ic.fy64);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
@@ -1106,18 +1103,18 @@ void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, double2 *tfxy, int *tfar, int tstamp) {
+                       int *tkey, int *tfar, int tstamp) {
   if (B.N > 0)
     k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, lane, B, PI, gX,
-                                        gV, abar, galpha, em, tkey, tfxy, tfar, tstamp);
+                                        gV, abar, galpha, em, tkey, tfar, tstamp);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha, int *tkey, double2 *tfxy,
-                         int *tfar, int tstamp) {
+                         float2 *gV, float2 *abar, float *galpha, int *tkey, int *tfar,
+                         int tstamp) {
   if (B.N > 0)
     k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
-                                          tkey, tfxy, tfar, tstamp);
+                                          tkey, tfar, tstamp);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,

@@ -107,8 +107,10 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
 
 // The same gather over per-slot keys written by the f32c adjoint's pointwise
 // pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
-// interp cell and fxy[p] = (fx, fy) in FP64, so that a candidate costs one
-// key compare; the contributions and their order are those of k_theta_cells.
+// interp cell, so that a candidate costs one key compare; a match's FP64
+// fractions (fx, fy) are formed from its position exactly as the f32c
+// interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
+// contributions and their order are those of k_theta_cells.
 // A workgroup takes a strip of kThetaStrip theta cells of one theta row
 // (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
 // strip sweeps, the keys of the slot run that covers the SPH cell ranges of
@@ -128,7 +130,7 @@ constexpr int kMatchBatch = 4;
 
 template <class GA>
 __global__ void __launch_bounds__(kBlock)
-k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy,
+k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ X,
                     const GA *__restrict__ galpha, const int *__restrict__ cstart,
                     const int *__restrict__ tfar, int tstamp, Batch B, int first,
                     double4 *__restrict__ part) {
@@ -231,6 +233,13 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
   }
   if (i0 > ilast) return;          // the kLanes lanes of a group leave together
   const int mykey = tcoff + j0 + i0 * ncy;
+  const PhysF32 &P = Sc.P;
+  auto frac = [&](double2 x) {
+    double gx = x.x * P.inv_hx64, gy = x.y * P.inv_hy64;
+    gx = (gx > P.gxclamp) ? P.gxclamp : ((gx < 0.0) ? 0.0 : gx);
+    gy = (gy > P.gyclamp) ? P.gyclamp : ((gy < 0.0) ? 0.0 : gy);
+    return make_double2(gx - (double)i0, gy - (double)j0);
+  };
   int cx0, cx1;
   xrange(i0, cx0, cx1);
   double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
@@ -263,20 +272,20 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ fxy
       for (int q = 0; q < kMatchBatch; ++q) {
         if (m0 + q < nmatch) {
           const int p = smatch[m0 + q][tid];
-          f[q] = fxy[p];
+          f[q] = X[p];
           ga[q] = (double)galpha[p];
         }
       }
 #pragma unroll
       for (int q = 0; q < kMatchBatch; ++q)
-        if (m0 + q < nmatch) add(f[q], ga[q]);
+        if (m0 + q < nmatch) add(frac(f[q]), ga[q]);
     }
   } else {
     // more matches than the position list holds: the same sweep, summing as it goes
     for (int cy = cy0; cy <= cy1; ++cy) {
       const int b = cstart[coff + cy * g.nx + cx0], e = cstart[coff + cy * g.nx + cx1 + 1];
       for (int p = b + lane; p < e; p += kLanes)
-        if (key[p] == mykey) add(fxy[p], (double)galpha[p]);
+        if (key[p] == mykey) add(frac(X[p]), (double)galpha[p]);
     }
   }
 #pragma unroll
@@ -343,11 +352,11 @@ void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, dou
   k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
 }
 
-void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *fxy,
+void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *X,
                                    const float *galpha, const int *cstart, const int *tfar,
                                    int tstamp, const Batch &B, bool first, double4 *part) {
   static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
-  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, fxy, galpha, cstart, tfar, tstamp, B,
+  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, X, galpha, cstart, tfar, tstamp, B,
                                                         first ? 1 : 0, part);
 }
 
