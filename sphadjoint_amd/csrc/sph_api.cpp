@@ -1933,12 +1933,12 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         // entry k+1 was stored in the next epoch's slot order: move the
         // adjoint state and V_{k+1} into this epoch's order
         timed(c, K_REMAP, [&] {
+          // (gXs32 is free between steps: P1 writes it before P2 reads it)
           launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
-          launch_gather_remap2f(s, ws->gX32, ws->inv, E.ids, N, ws->t2f);
+          launch_gather_remap3f(s, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids, N, ws->t2f,
+                                ws->gXs32, ws->Vtmp32);
           std::swap(ws->gX32, ws->t2f);
-          launch_gather_remap2f(s, ws->gV32, ws->inv, E.ids, N, ws->t2f);
-          std::swap(ws->gV32, ws->t2f);
-          launch_gather_remap2f(s, Vnext, ws->inv, E.ids, N, ws->Vtmp32);
+          std::swap(ws->gV32, ws->gXs32);
         });
         Vnext = ws->Vtmp32;
         cur = e;

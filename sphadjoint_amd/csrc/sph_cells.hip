@@ -321,6 +321,19 @@ __global__ void k_gather_remap2f(const float2 *__restrict__ src, const int *__re
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < N) dst[s] = src[inv_src[ids_dst[s]]];
 }
+// three float2 arrays into a new slot order at once (one index lookup per slot)
+__global__ void k_gather_remap3f(const float2 *__restrict__ a, const float2 *__restrict__ b,
+                                 const float2 *__restrict__ c, const int *__restrict__ inv_src,
+                                 const int *__restrict__ ids_dst, int N, float2 *__restrict__ da,
+                                 float2 *__restrict__ db, float2 *__restrict__ dc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) {
+    const int j = inv_src[ids_dst[s]];
+    da[s] = a[j];
+    db[s] = b[j];
+    dc[s] = c[j];
+  }
+}
 // dst[s] = (float2) src[ids[s]]: id-order FP64 host data into the slot order
 __global__ void k_gather2f(const double2 *__restrict__ src, const int *__restrict__ ids, int N,
                            float2 *__restrict__ dst) {
@@ -342,6 +355,12 @@ void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst) {
 }
 void launch_d2_to_f2(hipStream_t s, const double2 *src, int N, float2 *dst) {
   if (N > 0) k_d2_to_f2<<<nblocks(N), kBlock, 0, s>>>(src, N, dst);
+}
+void launch_gather_remap3f(hipStream_t s, const float2 *a, const float2 *b, const float2 *c,
+                           const int *inv_src, const int *ids_dst, int N, float2 *da, float2 *db,
+                           float2 *dc) {
+  if (N > 0)
+    k_gather_remap3f<<<nblocks(N), kBlock, 0, s>>>(a, b, c, inv_src, ids_dst, N, da, db, dc);
 }
 void launch_gather_remap2f(hipStream_t s, const float2 *src, const int *inv_src,
                            const int *ids_dst, int N, float2 *dst) {

@@ -159,6 +159,10 @@ void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batc
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);
 void launch_d2_to_f2(hipStream_t s, const double2 *src, int N, float2 *dst);
+// (da, db, dc)[s] = (a, b, c)[inv_src[ids_dst[s]]]
+void launch_gather_remap3f(hipStream_t s, const float2 *a, const float2 *b, const float2 *c,
+                           const int *inv_src, const int *ids_dst, int N, float2 *da, float2 *db,
+                           float2 *dc);
 void launch_gather_remap2f(hipStream_t s, const float2 *src, const int *inv_src,
                            const int *ids_dst, int N, float2 *dst);
 void launch_gather2f(hipStream_t s, const double2 *src, const int *ids, int N, float2 *dst);

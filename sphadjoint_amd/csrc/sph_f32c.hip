@@ -968,7 +968,7 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
       gy.add(fmaf(wgt, dy, -(PF * day)));
       gvx.add(C * dax);
       gvy.add(C * day);
-      grh.add(-(C * ad_v * iri));
+      grh.add(C * ad_v);
       sumS.add(F * ad_d);
       return true;
     }
@@ -993,7 +993,9 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
                 pair);
   }
   float Ssum = sumS.value();
-  float grv = grh.value() + (-2.f * P.m * pti * iri) * Ssum + (P.c2 * P.m * iri * iri) * Ssum;
+  // the pair sum of -C ad_v ir_i, with ir_i taken out of the sum
+  float grv = -(grh.value() * iri) + (-2.f * P.m * pti * iri) * Ssum +
+              (P.c2 * P.m * iri * iri) * Ssum;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
   float gyv = gy.value() + ab.y * wall_daccel((float)x64.y, P.Ly, P.kw);
   float ga = galpha[s];
