@@ -446,6 +446,10 @@ struct sph_state {
   int layout = SPH_LAYOUT_SLOT;
   int64_t age = kAgeNever;
   uint64_t epoch = 0;  // serial number of the current slot layout
+  // f32c form: the layout of this epoch is built but its mask list is not;
+  // the next step's density pass builds it (set by the checkpointer, which
+  // rebuilds before a snapshot and restores the rebuilt layout)
+  bool pending_list = false;
   // current layout; for the f32c form scell holds the binning cell of each slot
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
   float2 *V32 = nullptr, *V32n = nullptr, *xr = nullptr, *xrn = nullptr, *PI32 = nullptr;
@@ -698,8 +702,10 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
   hipStream_t s = st->ctx->stream;
   // maybe_rebuild!; the f32c form builds the new mask list in the step's
   // density pass (the same list, one sweep fewer)
-  const bool fused = st->age >= st->interval && st->precision == SPH_F32C;
-  if (st->age >= st->interval) do_build_neighbors(st, fused);
+  const bool rebuild = st->age >= st->interval;
+  const bool fused = st->precision == SPH_F32C && (rebuild || st->pending_list);
+  if (rebuild) do_build_neighbors(st, fused);
+  st->pending_list = false;
   st->age += 1;
   double2 *tX = nullptr, *tV = nullptr;
   float2 *tV32 = nullptr;
@@ -2343,6 +2349,7 @@ struct sph_ckpt {
     double2 *X = nullptr, *V = nullptr;
     float2 *V32 = nullptr;
     int *ids = nullptr;
+    int *scell = nullptr, *cstart = nullptr, *lane = nullptr;   // f32c form: the layout
   };
   std::vector<Snap> snaps;              // slot 0: the initial state; 1..nsnap: schedule
   std::vector<int> slot_of;             // epoch -> slot (-1: none)
@@ -2365,27 +2372,55 @@ int64_t ck_steps(const sph_ckpt *ck, int64_t e) {
   return rest < ck->seg ? rest : ck->seg;
 }
 
+// Snapshots sit at the start of schedule units, where the next step rebuilds.
+// FP64 form: (X, V, ids) before that rebuild; a restored state rebuilds as
+// the original run did.  f32c form: the rebuild is done before the snapshot
+// (do_build_neighbors leaves the mask list to the next step's density pass),
+// and the snapshot holds the rebuilt layout (X, V, ids, binning cells, cell
+// starts, thread order), so that a restored state skips the cell-list build;
+// the layout is a pure function of the positions and ids, so either way the
+// restored run repeats the original bit for bit.
 void ck_copy(sph_ckpt *ck, bool to_snap, int slot) {
   hipStream_t s = ck->ctx->stream;
   sph_state *st = ck->work;
   sph_ckpt::Snap &q = ck->snaps[slot];
   const size_t n = (size_t)ck->N;
+  const bool f32c = ck->precision == SPH_F32C;
   auto cp = [&](void *dst, const void *src, size_t bytes) {
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
   };
   if (to_snap) {
-    cp(q.X, st->X, sizeof(double2) * n);
-    if (ck->precision == SPH_F32C) cp(q.V32, st->V32, sizeof(float2) * n);
-    else cp(q.V, st->V, sizeof(double2) * n);
-    cp(q.ids, st->ids, sizeof(int) * n);
+    if (f32c && !st->pending_list) {
+      do_build_neighbors(st, true);
+      st->pending_list = true;
+    }
+    if (f32c) {
+      launch_snapshot_f32c(s, true, st->bs.B, st->X, st->V32, st->ids, st->scell, st->lane,
+                           st->cstart, st->Xref, st->xr, q.X, q.V32, q.ids, q.scell, q.lane,
+                           q.cstart);
+    } else {
+      cp(q.X, st->X, sizeof(double2) * n);
+      cp(q.V, st->V, sizeof(double2) * n);
+      cp(q.ids, st->ids, sizeof(int) * n);
+    }
   } else {
-    cp(st->X, q.X, sizeof(double2) * n);
-    if (ck->precision == SPH_F32C) cp(st->V32, q.V32, sizeof(float2) * n);
-    else cp(st->V, q.V, sizeof(double2) * n);
-    cp(st->ids, q.ids, sizeof(int) * n);
-    // the next step rebuilds the cell list and neighbour list from these
-    // positions, exactly as the original run did at this epoch start
-    st->age = kAgeNever;
+    if (f32c) {
+      // the rebuilt layout of the snapshot: reference positions and relative
+      // coordinates follow from X; the next step builds the mask list
+      launch_snapshot_f32c(s, false, st->bs.B, st->X, st->V32, st->ids, st->scell, st->lane,
+                           st->cstart, st->Xref, st->xr, q.X, q.V32, q.ids, q.scell, q.lane,
+                           q.cstart);
+      st->age = 0;
+      st->pending_list = true;
+      st->cl_view = 2;
+    } else {
+      cp(st->X, q.X, sizeof(double2) * n);
+      cp(st->V, q.V, sizeof(double2) * n);
+      cp(st->ids, q.ids, sizeof(int) * n);
+      // the next step rebuilds the cell list and neighbour list from these
+      // positions, exactly as the original run did at this epoch start
+      st->age = kAgeNever;
+    }
     st->aux_state = 0;
     st->epoch += 1;
   }
@@ -2459,6 +2494,9 @@ void ck_free(sph_ckpt *ck) {
     dfree(q.V);
     dfree(q.V32);
     dfree(q.ids);
+    dfree(q.scell);
+    dfree(q.cstart);
+    dfree(q.lane);
   }
   dfree(ck->iota);
   if (ck->work) sph_state_destroy(ck->work);
@@ -2622,6 +2660,11 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
         if (precision == SPH_F32C) q.V32 = dalloc<float2>((size_t)N);
         else q.V = dalloc<double2>((size_t)N);
         q.ids = dalloc<int>((size_t)N);
+        if (precision == SPH_F32C) {
+          q.scell = dalloc<int>((size_t)N);
+          q.cstart = dalloc<int>((size_t)ck->work->g.ncell + 1);
+          q.lane = dalloc<int>((size_t)N);
+        }
       }
       ck->sched.E = ck->E;
       ck->sched.nsnap = ck->nsnap;
@@ -2684,6 +2727,7 @@ int ckpt_forward_impl(sph_ckpt *ck, const double *X0, const double *V0, const sp
     }
     if (!dev) HIPC(hipStreamSynchronize(s));  // the host buffers are released on return
     st->age = kAgeNever;
+    st->pending_list = false;
     st->aux_state = 0;
     st->epoch += 1;
     copy_theta(st, theta);

@@ -122,6 +122,13 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
                          const float2 *V, const double2 *Xref, int check, const Batch &B,
                          double2 *Xn, float2 *Vn, float2 *xr, double2 *Xrefn, int *bcell,
                          int *flags, int *lane);
+// a checkpoint snapshot of a rebuilt single-scene layout: to_snap copies
+// (X, V, ids, scell, lane, cstart) into (qX, ...); otherwise copies them back,
+// X also into Xref, and forms xr
+void launch_snapshot_f32c(hipStream_t s, bool to_snap, const Batch &B, double2 *X, float2 *V,
+                          int *ids, int *scell, int *lane, int *cstart, double2 *Xref,
+                          float2 *xr, double2 *qX, float2 *qV, int *qids, int *qscell,
+                          int *qlane, int *qcstart);
 // R passes also write each slot's interp-cell key for the keyed gtheta gather
 // (both R passes also store tstamp in *tfar when a slot lies beyond its interp
 // cell's tight SPH range, Batch::trange_tight; tstamp differs from step to step)

@@ -813,6 +813,44 @@ k_permute32(const int *__restrict__ perm, const int *__restrict__ cellof,
   if (lane) lane_order_window(yr, S.P.cs, S.off + bsl, min(B.bs, S.g.N - bsl), lane);
 }
 
+// A checkpoint snapshot of a rebuilt single-scene layout, in one pass:
+// to_snap copies (X, V, ids, binning cells, thread order, cell starts) into
+// the snapshot; otherwise they are copied back, X also into the reference
+// positions, and the relative coordinates are formed.  Thread i covers slot i
+// and cell-start entry i.
+__global__ void __launch_bounds__(kBlock)
+k_snapshot32(int to_snap, Batch B, double2 *__restrict__ X, float2 *__restrict__ V,
+             int *__restrict__ ids, int *__restrict__ scell, int *__restrict__ lane,
+             int *__restrict__ cstart, double2 *__restrict__ Xref, float2 *__restrict__ xr,
+             double2 *__restrict__ qX, float2 *__restrict__ qV, int *__restrict__ qids,
+             int *__restrict__ qscell, int *__restrict__ qlane, int *__restrict__ qcstart) {
+  const int i = blockIdx.x * kBlock + (int)threadIdx.x;
+  const Scene32 &S = B.sc[0];
+  if (i < B.N) {
+    if (to_snap) {
+      qX[i] = X[i];
+      qV[i] = V[i];
+      qids[i] = ids[i];
+      qscell[i] = scell[i];
+      qlane[i] = lane[i];
+    } else {
+      const double2 x = qX[i];
+      const int c = qscell[i];
+      X[i] = x;
+      Xref[i] = x;
+      V[i] = qV[i];
+      ids[i] = qids[i];
+      scell[i] = c;
+      lane[i] = qlane[i];
+      xr[i] = rel_coord(x, c, S.g, S.P);
+    }
+  }
+  if (i <= B.ncell) {
+    if (to_snap) qcstart[i] = cstart[i];
+    else cstart[i] = qcstart[i];
+  }
+}
+
 // cell of every slot (the scene's grid, the division rule of cell_index,
 // src/neighbors.jl:38-42) in the global cell numbering, counts by
 // wave-aggregated atomics
@@ -1100,6 +1138,15 @@ void launch_permute_f32c(hipStream_t s, const int *perm, const int *cellof, cons
   if (B.N > 0)
     k_permute32<<<B.nblk, B.bs, 0, s>>>(perm, cellof, X, V, Xref, check, B, Xn, Vn, xr, Xrefn,
                                           bcell, flags, lane);
+}
+void launch_snapshot_f32c(hipStream_t s, bool to_snap, const Batch &B, double2 *X, float2 *V,
+                          int *ids, int *scell, int *lane, int *cstart, double2 *Xref,
+                          float2 *xr, double2 *qX, float2 *qV, int *qids, int *qscell,
+                          int *qlane, int *qcstart) {
+  if (B.nscenes != 1) throw std::logic_error("snapshots hold single-scene layouts");
+  const long long n = B.N > B.ncell + 1 ? B.N : B.ncell + 1;
+  k_snapshot32<<<nblocks(n), kBlock, 0, s>>>(to_snap ? 1 : 0, B, X, V, ids, scell, lane, cstart,
+                                             Xref, xr, qX, qV, qids, qscell, qlane, qcstart);
 }
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
