@@ -546,6 +546,7 @@ struct sph_workspace {
   // f32c form
   float2 *gX32 = nullptr, *gV32 = nullptr, *gXs32 = nullptr, *abar32 = nullptr, *PI32 = nullptr;
   float2 *Vtmp32 = nullptr, *t2f = nullptr, *xr32 = nullptr;
+  float2 *xr32b = nullptr;   // relative coordinates of the next step (R pass fused into P2)
   float *galpha32 = nullptr, *grho32 = nullptr;
 };
 
@@ -1664,7 +1665,7 @@ void ws_free(sph_workspace *ws) {
                     &ws->gX_id, &ws->gV_id};
   for (auto pp : d2) dfree(*pp);
   float2 **f2[] = {&ws->gX32, &ws->gV32, &ws->gXs32, &ws->abar32, &ws->PI32, &ws->Vtmp32,
-                   &ws->t2f, &ws->xr32};
+                   &ws->t2f, &ws->xr32, &ws->xr32b};
   for (auto pp : f2) dfree(*pp);
   dfree(ws->galpha); dfree(ws->grho); dfree(ws->gtheta); dfree(ws->inv);
   dfree(ws->galpha32); dfree(ws->grho32); dfree(ws->tpart);
@@ -1728,6 +1729,7 @@ void workspace_create_impl(sph_ctx *ctx, int64_t N, const sph_params *p, int pre
         ws->Vtmp32 = dalloc<float2>(n);
         ws->t2f = dalloc<float2>(n);
         ws->xr32 = dalloc<float2>(n);
+        ws->xr32b = dalloc<float2>(n);
         ws->galpha32 = dalloc<float>(n);
         ws->grho32 = dalloc<float>(n);
       } else {
@@ -1902,6 +1904,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const uint32_t *nm = use_given ? lnm : ws->nm;
   const int *lane = use_given ? llane : ws->lane;
   int listed = use_given ? (int)tape->epochs.size() - 1 : -1;  // epoch whose list `nl` holds
+  bool r_done = false;   // f32c form: the R pass of step k already ran (in P2 of step k+1)
+  int r_stamp = 0;
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
@@ -1950,20 +1954,26 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         cur = e;
       }
       const float2 *PIk = ws->PI32;
-      ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
-      timed(c, K_ADJ_R, [&] {
-        if (tape->keep_pi) {
-          PIk = tape->PI32 + off;
-          launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32, ws->gX32,
-                              ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfar,
-                              ws->tstamp);
-        } else {
-          launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
-          launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
-                            BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
-                            ws->galpha32, ws->em, ws->tkey, ws->tfar, ws->tstamp);
-        }
-      });
+      if (r_done) {
+        // this step's R pass ran in the previous step's P2 launch
+        ws->tstamp = r_stamp;
+        PIk = tape->PI32 + off;
+      } else {
+        ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
+        timed(c, K_ADJ_R, [&] {
+          if (tape->keep_pi) {
+            PIk = tape->PI32 + off;
+            launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32,
+                                ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfar,
+                                ws->tstamp);
+          } else {
+            launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
+            launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
+                              BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
+                              ws->galpha32, ws->em, ws->tkey, ws->tfar, ws->tstamp);
+          }
+        });
+      }
       timed(c, K_THETA, [&] {
         launch_theta_gather_keyed_f32(s, ws->tkey, Xk, ws->galpha32, E.cstart, ws->tfar,
                                       ws->tstamp, B, tfirst, ws->tpart);
@@ -1982,10 +1992,22 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
                              exact, true, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
                              ws->grho32, nullptr);
       });
+      // P2; when the next step of the sweep (k-1) is in this epoch and its
+      // R pass needs no density recompute, each slot runs that R pass on its
+      // final gX in the same launch (its xr goes to the spare buffer)
+      const bool fuse = tape->keep_pi && k >= 2 && tape->entry_epoch[k - 1] == e;
+      RpwArgs prev{};
+      if (fuse) {
+        r_stamp = ws->tstamp % 2000000000 + 1;
+        prev = RpwArgs{Xk - n, Vk, theta->d, ws->xr32b, ws->gV32, ws->abar32,
+                       ws->galpha32, ws->tkey, ws->tfar, r_stamp};
+      }
       timed(c, K_ADJ_P2, [&] {
         launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, exact, E.cstart, lane,
-                           BS.at(TK_ADJ_P2), ws->gX32);
+                           BS.at(TK_ADJ_P2), ws->gX32, fuse ? &prev : nullptr);
       });
+      if (fuse) std::swap(ws->xr32, ws->xr32b);
+      r_done = fuse;
     } else {
       const double2 *Vk = tape->V + off;
       const double2 *Vnext = tape->V + (size_t)k * n;

@@ -89,6 +89,20 @@ inline long long theta_strip_blocks(int ngx, int ngy) {
   return (long long)(ngy - 1) * ((ngx - 1 + kThetaStrip - 1) / kThetaStrip);
 }
 
+// Arguments of the adjoint R pass without the density recompute (tapes that
+// kept each step's pterm/invrho), which a P2 launch can also run for the
+// previous step (same slot order) as each slot's epilogue
+struct RpwArgs {
+  const double2 *X;      // positions of the step
+  const float2 *Vnext;   // velocities after the step
+  const double *theta;
+  float2 *xr;            // out: relative coordinates of the step
+  float2 *gV, *abar;
+  float *galpha;
+  int *tkey, *tfar;
+  int tstamp;
+};
+
 // bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)
 constexpr int kMaskWords = 4;
 
@@ -148,10 +162,11 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
                         const double *theta, const int *bcell, const uint32_t *nm, bool exact,
                         const int *cstart, const int *lane, const Batch &B, float2 *gXs,
                         float2 *gV, float *grho, uint32_t *em);
-// P2 over the exact list nm
+// P2 over the exact list nm; with `prev`, each slot then runs the R pass of
+// the previous step (rpw) on its final gX (prev->xr must not alias xr)
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
-                        const Batch &B, float2 *gX);
+                        const Batch &B, float2 *gX, const RpwArgs *prev = nullptr);
 // deterministic gtheta gather over the keys of the R passes (sph_theta.hip):
 // the theta-cell partials of one step, written (first) or added to part; the
 // gather sweeps the tight SPH ranges unless this step's R pass stored its

@@ -924,28 +924,37 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 }
 
 // adjoint R without the density recompute (tapes that kept each step's
-// pterm/invrho); also forms this step's relative coordinates xr
-__global__ void __launch_bounds__(kBlock)
-k_adj_rpw32(const double2 *__restrict__ X, const int *__restrict__ bcell,
-            const float2 *__restrict__ Vnext, const double *__restrict__ theta, Batch B,
-            float2 *__restrict__ xr, float2 *__restrict__ gX, float2 *__restrict__ gV,
-            float2 *__restrict__ abar, float *__restrict__ galpha, int *__restrict__ tkey,
-            int *__restrict__ tfar, int tstamp) {
-  SCENE_SLOT(B);
-  const double2 x64 = X[s];
-  xr[s] = rel_coord(x64, bcell[s], g, P);
+// pterm/invrho) for slot s with binning cell bc, given the slot's gX: also
+// forms the step's relative coordinates xr
+__device__ __forceinline__ void rpw_slot(const RpwArgs &R, int s, int bc, float2 gx,
+                                         const Scene32 &S, const Batch &B) {
+  const PhysF32 &P = S.P;
+  const Grid &g = S.g;
+  const double2 x64 = R.X[s];
+  R.xr[s] = rel_coord(x64, bc, g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
-  tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
-  check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
-  float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
-  float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
+  R.tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
+  check_tight(ic, bc, g, P, S, B, R.tfar, R.tstamp);
+  float al = interp_alpha(R.theta + S.toff, ic, P, nullptr, nullptr);
+  float2 gv = R.gV[s], vp = R.Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
   float D = __builtin_amdgcn_rcpf(fmaf(P.dt, al, 1.f));
   float dtD = P.dt * D;
-  abar[s] = make_float2(dtD * gvx, dtD * gvy);
-  galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
-  gV[s] = make_float2(D * gvx, D * gvy);
+  R.abar[s] = make_float2(dtD * gvx, dtD * gvy);
+  R.galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
+  R.gV[s] = make_float2(D * gvx, D * gvy);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__ gX, Batch B) {
+  const int blk = xcd_block();
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const int sl = (blk - B.boff[scn]) * B.bs + (int)threadIdx.x;
+  if (sl >= S.g.N) return;
+  const int s = S.off + sl;
+  rpw_slot(R, s, bcell[s], gX[s], S, B);
 }
 
 // adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185).
@@ -1049,11 +1058,12 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 }
 
 // adjoint P2: adj_pass2_kernel! and gX += gXs (src/adjoint.jl:126-156,187-191)
+template <bool kFuseR>
 __global__ void __launch_bounds__(kBlock)
 k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
            const float2 *__restrict__ gXs, const int *__restrict__ bcell,
            const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
-           const int *__restrict__ lane, Batch B, float2 *__restrict__ gX) {
+           const int *__restrict__ lane, Batch B, float2 *__restrict__ gX, RpwArgs R) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   __shared__ float sgr[kStageCap];
@@ -1092,7 +1102,10 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   }
   float2 gs = gXs[s];
   float2 go = gX[s];
-  gX[s] = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
+  const float2 gxn = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
+  gX[s] = gxn;
+  // the R pass of the previous step (same slot order) on this step's final gX
+  if (kFuseR) rpw_slot(R, s, bc, gxn, S, B);
 }
 
 }  // namespace
@@ -1161,9 +1174,8 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey, int *tfar,
                          int tstamp) {
-  if (B.N > 0)
-    k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(X, bcell, Vnext, theta, B, xr, gX, gV, abar, galpha,
-                                          tkey, tfar, tstamp);
+  const RpwArgs R{X, Vnext, theta, xr, gV, abar, galpha, tkey, tfar, tstamp};
+  if (B.N > 0) k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(R, bcell, gX, B);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
@@ -1180,8 +1192,15 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
-                        const Batch &B, float2 *gX) {
-  if (B.N > 0) k_adj_p232<<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, lane, B, gX);
+                        const Batch &B, float2 *gX, const RpwArgs *prev) {
+  if (B.N <= 0) return;
+  if (prev) {
+    if (prev->xr == xr) throw std::logic_error("P2 reads xr while the fused R pass writes it");
+    k_adj_p232<true><<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, lane, B, gX, *prev);
+  } else {
+    k_adj_p232<false><<<B.nblk, B.bs, 0, s>>>(xr, grho, gXs, bcell, nm, cstart, lane, B, gX,
+                                                RpwArgs{});
+  }
 }
 
 }  // namespace sph
