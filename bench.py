@@ -21,6 +21,7 @@ dominant kernel's launches on the library's stream: `value` comes from the
 timed pass's wall clock and the roofline from its events.
 """
 import argparse
+import gc
 import json
 import math
 import os
@@ -70,10 +71,11 @@ SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint
                   "adjoint_p1": "k_adj_p1", "adjoint_p2": "k_adj_p2"},
           "f32c": {"density": "k_density32", "force_integrate": "k_force32",
                    "adjoint_r": "k_adj_r32", "adjoint_p1": "k_adj_p132<true>",
-                   "adjoint_p2": "k_adj_p232"},
+                   "adjoint_p2": "k_adj_p232<false>"},
+          # checkpointed sweep: most P2 launches also run the next step's R pass
           "f32c_ckpt": {"density": "k_density32", "force_integrate": "k_force32",
                         "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
-                        "adjoint_p2": "k_adj_p232"}}
+                        "adjoint_p2": "k_adj_p232<true>"}}
 
 
 def log(*a):
@@ -135,6 +137,7 @@ class Runner:
 
     def forward_only(self, precision, K, W):
         S = self.S
+        gc.collect()
         st = S.State(self.be, self.sc.X0, self.sc.V0, self.p, precision=precision)
         S.simulate_(st, self.theta, self.p, self.be, W)
         self.be.synchronize()
@@ -148,6 +151,7 @@ class Runner:
         profile: None (no events), "all", or the one kernel name to time."""
         S, be = self.S, self.be
         precision = form.split("_")[0]
+        gc.collect()   # release earlier passes' device memory outside the timed region
         ws = S.AdjointWorkspace(be, self.N, self.p, precision=precision)
         if form.endswith("_ckpt"):
             ck = S.Checkpointer(be, self.N, K, self.p, snapshots=self.args.snapshots,

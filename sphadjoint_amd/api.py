@@ -515,10 +515,21 @@ class Checkpointer:
         L.call("sph_ckpt_create", backend._h, C.byref(cp), self.precision, self.N, self.nsteps,
                int(snapshots), self.seg_epochs, self.skin, self.interval, int(maxnb), C.byref(h))
         self._h = h
-        sh = C.c_void_p()
-        L.call("sph_ckpt_state", self._h, C.byref(sh))
-        self.state = State._borrow(sh, self, backend, p, self.N, self.skin, self.precision)
+        self._stref = None
         self._theta = None
+
+    @property
+    def state(self):
+        """The work state (borrowed: it keeps the checkpointer alive, and the
+        checkpointer refers to it only weakly, so that no reference cycle
+        delays the release of the device memory)."""
+        st = self._stref() if self._stref is not None else None
+        if st is None:
+            sh = C.c_void_p()
+            L.call("sph_ckpt_state", self._h, C.byref(sh))
+            st = State._borrow(sh, self, self.backend, self.p, self.N, self.skin, self.precision)
+            self._stref = weakref.ref(st)
+        return st
 
     def forward(self, X0, V0, theta, check_flags=True):
         """X0, V0: (2, N) numpy arrays, or DeviceArrays."""
@@ -531,9 +542,10 @@ class Checkpointer:
             x0 = _as_2n(X0, self.N)
             v0 = _as_2n(V0, self.N)
             L.call("sph_ckpt_forward", self._h, _ptr(x0), _ptr(v0), self._theta._h)
+        st = self.state
         if check_flags:
-            check_neighbor_flags(self.state.nl)
-        return self.state
+            check_neighbor_flags(st.nl)
+        return st
 
     def backward(self, ws: AdjointWorkspace, *, seedX, seedV, theta=None):
         """seedX, seedV: (2, N) numpy arrays, or DeviceArrays."""
@@ -557,11 +569,9 @@ class Checkpointer:
                     backward_forward_epochs=int(o[3]), backward_taped_epochs=int(o[4]))
 
     def __del__(self):
+        # no borrowed state is alive here: each holds the checkpointer
         h = getattr(self, "_h", None)
         if h:
-            st = getattr(self, "state", None)
-            if st is not None:
-                st._h = None
             L.load().sph_ckpt_destroy(h)
             self._h = None
 
