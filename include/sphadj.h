@@ -202,7 +202,10 @@ int sph_tuning_set(sph_state *st, const int32_t *sizes);
  * the binomial (revolve) schedule, and each segment is recomputed onto a
  * one-segment tape right before its reverse sweep.  Segment starts are
  * neighbour-list rebuilds, so gradients equal the full-tape ones bit for bit.
- * Device memory: nsnap+1 states plus a tape of seg_epochs*interval steps. --- */
+ * Device memory: nsnap+1 snapshots (X, V, ids; the f32c form also keeps the
+ * rebuilt layout: binning cells, cell starts, thread order, so that a
+ * restored segment needs no cell-list build), one work state and a tape of
+ * seg_epochs*interval steps. --- */
 typedef struct sph_ckpt sph_ckpt;
 int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N, int64_t nsteps,
                     int64_t nsnap, int64_t seg_epochs, double skin, int64_t interval,
