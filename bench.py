@@ -13,7 +13,8 @@ Forms, each timed over the same K steps:
   f64        the plain FP64 path (reference arithmetic, full tape)
   f32c_ckpt  FP32 state with compensated sums, FP64 positions / J / gtheta,
              binomial checkpointing under a snapshot budget  -> `value`
-  f32c       the same arithmetic over a full tape
+  f32c       the same arithmetic over a full tape that also keeps each step's
+             (pterm, invrho) and exact neighbour masks (80 B per particle-step)
 One JSON line is printed.  Each form runs a warm-up pass, a pass with HIP
 events around every kernel (the per-kernel table, and the choice of the
 dominant kernel), and the timed pass, in which events bracket only the
@@ -69,10 +70,11 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_block_1000x1000.json"
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_busy_block_1000x1000.json")
 SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint_r": "k_adj_r",
                   "adjoint_p1": "k_adj_p1", "adjoint_p2": "k_adj_p2"},
+          # the f32c tapes keep (pterm, invrho) and the masks: most P2 launches
+          # also run the next step's pointwise R pass
           "f32c": {"density": "k_density32", "force_integrate": "k_force32",
-                   "adjoint_r": "k_adj_r32", "adjoint_p1": "k_adj_p132<true>",
-                   "adjoint_p2": "k_adj_p232<false>"},
-          # checkpointed sweep: most P2 launches also run the next step's R pass
+                   "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
+                   "adjoint_p2": "k_adj_p232<true>"},
           "f32c_ckpt": {"density": "k_density32", "force_integrate": "k_force32",
                         "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
                         "adjoint_p2": "k_adj_p232<true>"}}
@@ -172,7 +174,10 @@ class Runner:
         else:
             st = S.State(be, self.sc.X0, self.sc.V0, self.p, interval=self.args.interval,
                          precision=precision)
-            tape = S.Tape(be, self.N, K, self.p, precision=precision)
+            # the chosen form's tape keeps (pterm, invrho) and the exact masks;
+            # the plain FP64 form keeps X and V, as the reference's tape does
+            keep = dict(keep_pi=True, keep_lists=True) if precision == "f32c" else {}
+            tape = S.Tape(be, self.N, K, self.p, precision=precision, **keep)
             be.synchronize()
             if profile:
                 S.api.profile_enable(be, True, only=None if profile == "all" else profile)
@@ -221,7 +226,8 @@ def run_batched(nscenes, steps, seed=0):
     # forward + adjoint over a full tape (a warm-up pass, then the timed one)
     for _ in range(2):
         st = fresh()
-        tape = S.Tape(be, Ntot, steps, st.p, precision="f32c")
+        gc.collect()
+        tape = S.Tape(be, Ntot, steps, st.p, precision="f32c", keep_pi=True, keep_lists=True)
         ws = S.BatchWorkspace(be, st)
         zero = S.DeviceArray(be, N=Ntot)
         be.synchronize()

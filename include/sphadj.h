@@ -49,6 +49,8 @@ typedef enum sph_field_id {
 } sph_field_id;
 
 typedef enum sph_grad_id { SPH_GX = 0, SPH_GV = 1, SPH_GTHETA = 2 } sph_grad_id;
+/* what a tape keeps per step beyond X and V (sph_tape_create_keep) */
+typedef enum sph_tape_keep { SPH_TAPE_KEEP_PI = 1, SPH_TAPE_KEEP_LISTS = 2 } sph_tape_keep;
 
 /* SPHParams{T} (src/params.jl:14-28); gx defaults to 0 and gy to -9.81 there. */
 typedef struct sph_params {
@@ -123,6 +125,15 @@ int sph_simulate_graph(sph_state *st, const sph_field *theta, int64_t nsteps, in
 
 /* --- Tape (src/driver.jl:45-83) --- */
 int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out);
+/* A tape that also keeps each step's (pterm, invrho) from the density pass
+ * (keep & SPH_TAPE_KEEP_PI) and, in the f32c form, the pass's exact neighbour
+ * masks (keep & SPH_TAPE_KEEP_LISTS), so that backward! skips the density
+ * recompute of its R pass and the pair tests of P1.  No counterpart in the
+ * reference, whose tape keeps X and V only (src/driver.jl:45-83).  Extra
+ * device memory per particle-step: 16 B (FP64) or 8 B (f32c) for (pterm,
+ * invrho), 48 B for the masks.  Gradients equal a plain tape's bit for bit. */
+int sph_tape_create_keep(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, int keep,
+                         sph_tape **out);
 int sph_tape_destroy(sph_tape *tp);
 int sph_tape_reset(sph_tape *tp);                            /* reset!(tape) */
 int sph_tape_length(const sph_tape *tp, int64_t *n);         /* length(tape) */

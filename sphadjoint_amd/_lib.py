@@ -57,6 +57,7 @@ SIGNATURES = {
     "sph_simulate": (C.c_int, [_vp, _vp, _i64, _vp]),
     "sph_simulate_graph": (C.c_int, [_vp, _vp, _i64, C.POINTER(_i64)]),
     "sph_tape_create": (C.c_int, [_vp, _i64, _i64, C.c_int, _pp]),
+    "sph_tape_create_keep": (C.c_int, [_vp, _i64, _i64, C.c_int, C.c_int, _pp]),
     "sph_tape_destroy": (C.c_int, [_vp]),
     "sph_tape_reset": (C.c_int, [_vp]),
     "sph_tape_length": (C.c_int, [_vp, C.POINTER(_i64)]),

@@ -355,12 +355,23 @@ def step_(st: State, theta, p: SPHParams = None, backend: MI355XBackend = None):
 
 
 class Tape:
-    """Tape (src/driver.jl:45-83): the state before every step, on the device."""
+    """Tape (src/driver.jl:45-83): the state before every step, on the device.
+    keep_pi / keep_lists (the latter f32c only) also keep each step's
+    (pterm, invrho) and exact neighbour masks, so that backward_ skips the
+    density recompute and the pair tests (sph_tape_create_keep); same
+    gradients, more device memory."""
 
-    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, precision="f64"):
+    def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, precision="f64", *,
+                 keep_pi=False, keep_lists=False):
         self.backend, self.N = backend, int(N)
         h = C.c_void_p()
-        L.call("sph_tape_create", backend._h, self.N, int(nsteps), _PREC[precision], C.byref(h))
+        keep = (1 if keep_pi else 0) | (2 if keep_lists else 0)
+        if keep:
+            L.call("sph_tape_create_keep", backend._h, self.N, int(nsteps), _PREC[precision],
+                   keep, C.byref(h))
+        else:
+            L.call("sph_tape_create", backend._h, self.N, int(nsteps), _PREC[precision],
+                   C.byref(h))
         self._h = h
 
     def __len__(self):

@@ -1556,11 +1556,15 @@ int sph_simulate(sph_state *st, const sph_field *theta, int64_t nsteps, sph_tape
   });
 }
 
-int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out) {
+int sph_tape_create_keep(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, int keep,
+                         sph_tape **out) {
   return guarded([&] {
     if (!ctx || !out) fail(SPH_E_ARG, "NULL argument");
     need_precision(precision);
     if (N < 1 || capacity < 0) fail(SPH_E_ARG, "bad tape size");
+    if (keep & ~(SPH_TAPE_KEEP_PI | SPH_TAPE_KEEP_LISTS)) fail(SPH_E_ARG, "unknown keep flags %d", keep);
+    if ((keep & SPH_TAPE_KEEP_LISTS) && precision != SPH_F32C)
+      fail(SPH_E_ARG, "the FP64 form keeps no neighbour masks");
     HIPC(hipSetDevice(ctx->device));
     auto *tp = new sph_tape;
     try {
@@ -1572,17 +1576,31 @@ int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sp
       tp->X = dalloc<double2>(tot);
       if (precision == SPH_F32C) tp->V32 = dalloc<float2>(tot);
       else tp->V = dalloc<double2>(tot);
+      if (keep & SPH_TAPE_KEEP_PI) {
+        tp->keep_pi = true;
+        if (precision == SPH_F32C) tp->PI32 = dalloc<float2>(tot);
+        else tp->PI = dalloc<double2>(tot);
+      }
+      if (keep & SPH_TAPE_KEEP_LISTS)
+        tp->EM = dalloc<uint32_t>((size_t)capacity * (size_t)N * (3 * kMaskWords));
       tp->entry_epoch.assign((size_t)capacity + 2, -1);
     } catch (...) {
       dfree(tp->X);
       dfree(tp->V);
       dfree(tp->V32);
+      dfree(tp->PI);
+      dfree(tp->PI32);
+      dfree(tp->EM);
       delete tp;
       throw;
     }
     ctx_retain(ctx);
     *out = tp;
   });
+}
+
+int sph_tape_create(sph_ctx *ctx, int64_t N, int64_t capacity, int precision, sph_tape **out) {
+  return sph_tape_create_keep(ctx, N, capacity, precision, 0, out);
 }
 
 int sph_tape_destroy(sph_tape *tp) {
@@ -1920,9 +1938,10 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
           // the thread order of the epoch with its relative coordinates; a
-          // tape with each step's exact masks needs no mask list
+          // tape with each step's exact masks and (pterm, invrho) needs no mask
+          // list (without the latter the R pass recomputes the density over it)
           launch_xrel_f32c(s, X0, E.scell, B, ws->xr32, ws->lane);
-          if (!tape->EM)
+          if (!(tape->EM && tape->keep_pi))
             launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
                                  ws->flags);
         } else {
@@ -2666,16 +2685,11 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
                                 maxnb, SPH_LAYOUT_SLOT, &ck->work);
       if (rc != SPH_OK) throw Error(rc, g_err);
       const int64_t cap = ck->seg < nsteps ? ck->seg : nsteps;
-      rc = sph_tape_create(ctx, N, cap, precision, &ck->mini);
+      // the segment tape keeps (pterm, invrho) and, in the f32c form, the
+      // exact masks of every step
+      const int keep = SPH_TAPE_KEEP_PI | (precision == SPH_F32C ? SPH_TAPE_KEEP_LISTS : 0);
+      rc = sph_tape_create_keep(ctx, N, cap, precision, keep, &ck->mini);
       if (rc != SPH_OK) throw Error(rc, g_err);
-      ck->mini->keep_pi = true;
-      const size_t tot = (size_t)(cap + 1) * (size_t)N;
-      if (precision == SPH_F32C) {
-        ck->mini->PI32 = dalloc<float2>(tot);
-        ck->mini->EM = dalloc<uint32_t>((size_t)cap * (size_t)N * (3 * kMaskWords));
-      } else {
-        ck->mini->PI = dalloc<double2>(tot);
-      }
       ck->snaps.resize((size_t)ck->nsnap + 1);
       for (auto &q : ck->snaps) {
         q.X = dalloc<double2>((size_t)N);

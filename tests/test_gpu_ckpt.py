@@ -28,10 +28,10 @@ def be():
     return S.MI355XBackend(0)
 
 
-def full_tape(be, sc, nsteps, interval, cX, cV, precision):
+def full_tape(be, sc, nsteps, interval, cX, cV, precision, **keep):
     pg = S.SPHParams(**sc.params)
     st = S.State(be, sc.X0, sc.V0, pg, interval=interval, precision=precision)
-    tape = S.Tape(be, sc.N, nsteps, pg, precision=precision)
+    tape = S.Tape(be, sc.N, nsteps, pg, precision=precision, **keep)
     S.simulate_(st, sc.theta, pg, be, nsteps, tape=tape)
     X, V = st.X, st.V
     ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
@@ -85,6 +85,33 @@ def test_checkpointed_equals_full_tape(be, precision, nsteps, interval, snapshot
     assert stats["epochs"] == E
     assert stats["forward_epochs"] + stats["backward_forward_epochs"] == plan["forward_epochs"]
     assert stats["backward_taped_epochs"] == E - 1
+
+
+@pytest.mark.parametrize("precision,keep", [("f64", dict(keep_pi=True)),
+                                             ("f32c", dict(keep_pi=True)),
+                                             ("f32c", dict(keep_lists=True)),
+                                             ("f32c", dict(keep_pi=True, keep_lists=True))])
+@pytest.mark.parametrize("nsteps,interval", [(37, 4), (25, 1)])
+def test_keep_tape_equals_plain_tape(be, precision, keep, nsteps, interval):
+    """A tape that keeps (pterm, invrho) and/or the exact masks of each step
+    (the reverse sweep then skips the density recompute and the pair tests,
+    and in the f32c form fuses each R pass into the previous P2) gives the
+    plain tape's gradients bit for bit."""
+    say("keep tape vs plain tape", precision, keep, nsteps, interval)
+    sc = scenes.brinkman_56x37(nsteps=nsteps)
+    rng = np.random.default_rng(nsteps + 7)
+    cX = rng.standard_normal((2, sc.N))
+    cV = rng.standard_normal((2, sc.N))
+    want = full_tape(be, sc, nsteps, interval, cX, cV, precision)
+    got = full_tape(be, sc, nsteps, interval, cX, cV, precision, **keep)
+    for a, b, name in zip(got, want, ("X", "V", "gX", "gV", "gtheta")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+def test_keep_tape_rejects_fp64_lists(be):
+    sc = scenes.gradcheck_5x5()
+    with pytest.raises(Exception):
+        S.Tape(be, sc.N, 4, S.SPHParams(**sc.params), precision="f64", keep_lists=True)
 
 
 def test_checkpointer_reuse_and_gaussian_seed(be):
