@@ -252,7 +252,9 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     tboff += theta_strip_blocks(c.T.ngx, c.T.ngy);
     nboff += blocks_of(nodes);
   }
-  if (off >= (1LL << 27)) fail(SPH_E_ARG, "the f32c form packs slot offsets in 27 bits: N=%lld", off);
+  if (off > kMaxSlotsF32C)
+    fail(SPH_E_ARG, "the f32c form addresses its mask lists with 32-bit offsets: N=%lld above %lld",
+         off, kMaxSlotsF32C);
   if (coff >= (1LL << 30)) fail(SPH_E_ARG, "too many cells in all scenes (%lld)", coff);
   const int nsc = (int)scenes.size();
   std::vector<int> tb2s((size_t)tboff), nb2s((size_t)nboff);
@@ -1128,8 +1130,9 @@ void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const in
     }
     const sph_params *p = &ps[0];
     if (N < 1 || N > (int64_t)1 << 30) fail(SPH_E_ARG, "N=%lld out of range", (long long)N);
-    if (precision == SPH_F32C && N >= (int64_t)1 << 27)
-      fail(SPH_E_ARG, "the f32c form packs slot indices in 27 bits: N=%lld too large", (long long)N);
+    if (precision == SPH_F32C && N > kMaxSlotsF32C)
+      fail(SPH_E_ARG, "the f32c form addresses its mask lists (48 bytes per particle) with 32-bit "
+                      "offsets: N=%lld above %lld", (long long)N, kMaxSlotsF32C);
     if (!X0 || !V0) fail(SPH_E_ARG, "X0 and V0 are required");
     if (!(skin >= 0)) fail(SPH_E_ARG, "skin must be >= 0");
     if (interval < 1) fail(SPH_E_ARG, "interval must be >= 1");

@@ -105,6 +105,9 @@ struct RpwArgs {
 
 // bitmask neighbour list: 32-bit words per row (runs of up to 32*kMaskWords slots)
 constexpr int kMaskWords = 4;
+// the kernels address a mask list (3*kMaskWords words per slot) with 32-bit
+// byte offsets, which bounds the slots of an f32c state
+constexpr long long kMaxSlotsF32C = 0xFFFFFFFFLL / (3 * kMaskWords * 4);
 
 void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
                         int *counts);

@@ -92,6 +92,22 @@ def test_f32c_rebuild_cell_list_on_random_clouds(be, case):
         np.testing.assert_array_equal(a, b, err_msg=name)
 
 
+def test_f32c_rejects_more_slots_than_its_offsets_address(be):
+    """The f32c kernels address a mask list (48 bytes per slot) with 32-bit
+    byte offsets: a state beyond 89,478,485 particles is refused up front (no
+    host arrays are read before the check)."""
+    import ctypes as C
+    from sphadjoint_amd import _lib as L
+    from sphadjoint_amd import api
+    say("f32c slot bound")
+    sc = scenes.gradcheck_5x5()
+    cp = S.SPHParams(**sc.params).c_struct()
+    h = C.c_void_p()
+    with pytest.raises(L.SPHError, match="32-bit offsets"):
+        L.call("sph_state_create", be._h, C.byref(cp), api._PREC["f32c"], 89_478_486, None, None,
+               0.2, 4, 64, 0, C.byref(h))
+
+
 def _grads(be, sc, nsteps, interval, cX, cV, precision):
     pg = S.SPHParams(**sc.params)
     st = S.State(be, sc.X0, sc.V0, pg, interval=interval, precision=precision)
