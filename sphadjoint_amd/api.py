@@ -318,7 +318,13 @@ def check_neighbor_flags(nl: NeighborList):
     """check_neighbor_flags (src/neighbors.jl:282-290): warn, then reset."""
     f = nl.flags
     if f[0] != 0:
-        warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
+        st = nl._st
+        if st is not None and st.precision == L.SPH_F32C:
+            # the f32c lists hold 128 slots per cell row of a particle's 3x3 block
+            warnings.warn("neighbour list row overflow: a cell row of a 3x3 block holds more "
+                          "than 128 particles (f32c mask lists); pairs past it are left out")
+        else:
+            warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
     if f[1] != 0:
         warnings.warn("neighbour-list rebuild interval too long (a displacement exceeded "
                       "skin*h); reduce interval or increase skin")

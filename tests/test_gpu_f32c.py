@@ -92,6 +92,29 @@ def test_f32c_rebuild_cell_list_on_random_clouds(be, case):
         np.testing.assert_array_equal(a, b, err_msg=name)
 
 
+def test_f32c_row_overflow_flag(be):
+    """An f32c mask row covers 128 slots; a cluster that puts more particles in
+    one cell row of a 3x3 block raises the overflow flag (as maxnb does for the
+    FP64 list) and the step stays finite."""
+    say("f32c row overflow")
+    rng = np.random.default_rng(3)
+    h = 0.01
+    cs = 2 * h * 1.2
+    N = 400                     # ~400 particles in about one cell
+    X = np.vstack([0.5 + cs * rng.uniform(0.05, 0.95, N), 0.3 + cs * rng.uniform(0.05, 0.95, N)])
+    p = S.SPHParams(h=h, m=1e-4, rho0=1.0, c=10.0, mu=0.01, dt=1e-6, Lx=1.0, Ly=0.6, kw=1e4,
+                    ngx=9, ngy=7, gx=0.0, gy=-9.81)
+    g = S.State(be, X, np.zeros_like(X), p, precision="f32c")
+    S.build_neighbors_(g.nl)
+    assert g.nl.flags[0] == 1
+    with pytest.warns(UserWarning, match="128 particles"):
+        S.check_neighbor_flags(g.nl)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        S.simulate_(g, np.zeros((7, 9)), p, be, 2)
+    assert np.isfinite(g.X).all() and np.isfinite(g.V).all()
+
+
 def test_f32c_rejects_more_slots_than_its_offsets_address(be):
     """The f32c kernels address a mask list (48 bytes per slot) with 32-bit
     byte offsets: a state beyond 89,478,485 particles is refused up front (no
