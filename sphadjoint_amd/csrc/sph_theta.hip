@@ -115,11 +115,12 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
 // (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
 // strip sweeps, the keys of the slot run that covers the SPH cell ranges of
 // all its cells into LDS (coalesced); each cell then compares keys there and
-// reads fxy and galpha only for its own slots; the cell starts of the strip's
-// rows are staged too, so that a cell's sweep issues no dependent global loads
-// before its matches.  A strip whose runs exceed kThetaStage keys, that sweeps
-// more than kThetaRows rows or more than kThetaCols - 1 SPH cells per row,
-// reads keys and cell starts in global memory (same order, same sums).
+// reads positions and galpha only for its own slots, kMatchCap matches at a
+// time; the cell starts of the strip's rows are staged too, so that a cell's
+// sweep issues no dependent global loads before its matches.  A strip whose
+// runs exceed kThetaStage keys, that sweeps more than kThetaRows rows or more
+// than kThetaCols - 1 SPH cells per row, reads keys and cell starts in global
+// memory (same order, same sums).
 // Workgroups are dealt to the scenes of a batched state in order
 // (Scene32::tboff).
 constexpr int kThetaStage = 2048;
@@ -249,44 +250,46 @@ k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ X,
     s01 += (ga * (1.0 - f.x)) * f.y;
     s11 += (ga * f.x) * f.y;
   };
-  // the lane's matches, in sweep order: their positions first (LDS), then
-  // their data in batches of kMatchBatch independent loads
+  // the lane's matches in sweep order, collected up to kMatchCap at a time
+  // (their positions, in LDS), then their data read in batches of
+  // kMatchBatch independent loads; the sweep resumes where it stopped
   const int tid = (int)threadIdx.x;
-  int nmatch = 0;
-  for (int cy = cy0; cy <= cy1; ++cy) {
-    const int k = cy - cy0;
-    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + cy * g.nx + cx0];
-    const int e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + cy * g.nx + cx1 + 1];
-    const int ro = staged ? soff[k] - slo[k] : 0;
-    for (int p = b + lane; p < e; p += kLanes) {
-      if ((staged ? skey[p + ro] : key[p]) != mykey) continue;
-      if (nmatch < kMatchCap) smatch[nmatch][tid] = p;
-      ++nmatch;
+  int cy = cy0, p = 0, e = 0, ro = 0;
+  auto row = [&](int y) {   // start the sweep of SPH cell row y
+    const int k = y - cy0;
+    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + y * g.nx + cx0];
+    e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + y * g.nx + cx1 + 1];
+    ro = staged ? soff[k] - slo[k] : 0;
+    p = b + lane;
+  };
+  row(cy);
+  for (;;) {
+    int n = 0;
+    while (n < kMatchCap) {
+      if (p >= e) {
+        if (++cy > cy1) break;
+        row(cy);
+        continue;
+      }
+      if ((staged ? skey[p + ro] : key[p]) == mykey) smatch[n++][tid] = p;
+      p += kLanes;
     }
-  }
-  if (nmatch <= kMatchCap) {
-    for (int m0 = 0; m0 < nmatch; m0 += kMatchBatch) {
+    for (int m0 = 0; m0 < n; m0 += kMatchBatch) {
       double2 f[kMatchBatch];
       double ga[kMatchBatch];
 #pragma unroll
       for (int q = 0; q < kMatchBatch; ++q) {
-        if (m0 + q < nmatch) {
-          const int p = smatch[m0 + q][tid];
-          f[q] = X[p];
-          ga[q] = (double)galpha[p];
+        if (m0 + q < n) {
+          const int pm = smatch[m0 + q][tid];
+          f[q] = X[pm];
+          ga[q] = (double)galpha[pm];
         }
       }
 #pragma unroll
       for (int q = 0; q < kMatchBatch; ++q)
-        if (m0 + q < nmatch) add(frac(f[q]), ga[q]);
+        if (m0 + q < n) add(frac(f[q]), ga[q]);
     }
-  } else {
-    // more matches than the position list holds: the same sweep, summing as it goes
-    for (int cy = cy0; cy <= cy1; ++cy) {
-      const int b = cstart[coff + cy * g.nx + cx0], e = cstart[coff + cy * g.nx + cx1 + 1];
-      for (int p = b + lane; p < e; p += kLanes)
-        if (key[p] == mykey) add(frac(X[p]), (double)galpha[p]);
-    }
+    if (n < kMatchCap) break;   // the sweep has ended
   }
 #pragma unroll
   for (int m = kLanes / 2; m > 0; m >>= 1) {
