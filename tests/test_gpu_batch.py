@@ -38,13 +38,17 @@ def alone(be, sc, nsteps, seedV_scale):
     return st.X, st.V, J, ws.gX, ws.gV, ws.gtheta
 
 
+@pytest.mark.parametrize("keep", [False, True])
 @pytest.mark.parametrize("nsteps", [1, 9, 30])
-def test_batch_equals_scenes_alone(be, nsteps):
-    say("batch of 4 vs alone, steps", nsteps)
+def test_batch_equals_scenes_alone(be, nsteps, keep):
+    """A batch of four scenes equals each scene run alone (plain tapes), bit
+    for bit; with keep=True the batch's tape also keeps (pterm, invrho) and
+    the exact masks."""
+    say("batch of 4 vs alone, steps", nsteps, "keep", keep)
     scs = scenes.batch_line_search(seed=3, nscenes=4, nmin=300, nmax=5000, nsteps=nsteps)
     bst = S.BatchState(be, [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs])
     bth = bst.field([sc.theta for sc in scs])
-    tape = S.Tape(be, bst.N, nsteps, bst.p, precision="f32c")
+    tape = S.Tape(be, bst.N, nsteps, bst.p, precision="f32c", keep_pi=keep, keep_lists=keep)
     S.simulate_(bst, bth, bst.p, be, nsteps, tape=tape)
     J, seed = S.batch_objective(bst, 0.3, 0.2, 0.1)
     ws = S.BatchWorkspace(be, bst)

@@ -150,8 +150,9 @@ def test_checkpointer_backward_needs_forward(be):
         ck.backward(ws, seedX=z, seedV=z, theta=sc.theta)
 
 
-def test_torch_autograd_matches_adjoint(be):
-    say("torch autograd wrapper")
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_torch_autograd_matches_adjoint(be, precision):
+    say("torch autograd wrapper", precision)
     import torch
     from sphadjoint_amd import autograd as AG
     sc = scenes.brinkman_56x37(nsteps=20)
@@ -160,7 +161,7 @@ def test_torch_autograd_matches_adjoint(be):
     V0 = torch.tensor(sc.V0, dtype=torch.float64, requires_grad=True)
     res = {}
     for snaps in (None, 2):
-        sim = AG.Simulation(be, pg, sc.N, 20, snapshots=snaps)
+        sim = AG.Simulation(be, pg, sc.N, 20, snapshots=snaps, precision=precision)
         th = torch.tensor(sc.theta, dtype=torch.float64, requires_grad=True)
         XT, VT = AG.simulate(th, X0, V0, sim)
         J = AG.gaussian_target(XT, 0.3, 0.2, 0.1) + 0.5 * (VT * VT).sum()
@@ -174,13 +175,13 @@ def test_torch_autograd_matches_adjoint(be):
         else:
             assert torch.equal(a, b)
     # against the direct path, seeds formed the same way in torch
-    st = S.State(be, sc.X0, sc.V0, pg)
-    tape = S.Tape(be, sc.N, 20, pg)
+    st = S.State(be, sc.X0, sc.V0, pg, precision=precision)
+    tape = S.Tape(be, sc.N, 20, pg, precision=precision)
     S.simulate_(st, sc.theta, pg, be, 20, tape=tape)
     Xt = torch.tensor(st.X, requires_grad=True)
     Vt = torch.tensor(st.V, requires_grad=True)
     (AG.gaussian_target(Xt, 0.3, 0.2, 0.1) + 0.5 * (Vt * Vt).sum()).backward()
-    ws = S.AdjointWorkspace(be, sc.N, pg)
+    ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
     S.backward_(ws, tape, sc.theta, pg, be, seedX=Xt.grad.numpy(), seedV=Vt.grad.numpy())
     np.testing.assert_array_equal(res[None][1].numpy(), ws.gtheta)
     np.testing.assert_array_equal(res[None][2].numpy(), ws.gX)
