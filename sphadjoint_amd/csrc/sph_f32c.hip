@@ -926,17 +926,26 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
 // adjoint R without the density recompute (tapes that kept each step's
 // pterm/invrho) for slot s with binning cell bc, given the slot's gX: also
 // forms the step's relative coordinates xr
-__device__ __forceinline__ void rpw_slot(const RpwArgs &R, int s, int bc, float2 gx,
-                                         const Scene32 &S, const Batch &B) {
+// the R pass's own-slot inputs; a P2 launch that runs the pass loads them
+// before its pair loop, so that they arrive while the loop computes
+struct RpwIn {
+  double2 x64;
+  float2 gv, vp;
+};
+__device__ __forceinline__ RpwIn rpw_load(const RpwArgs &R, int s) {
+  return RpwIn{R.X[s], R.gV[s], R.Vnext[s]};
+}
+__device__ __forceinline__ void rpw_finish(const RpwArgs &R, const RpwIn &in, int s, int bc,
+                                           float2 gx, const Scene32 &S, const Batch &B) {
   const PhysF32 &P = S.P;
   const Grid &g = S.g;
-  const double2 x64 = R.X[s];
+  const double2 x64 = in.x64;
   R.xr[s] = rel_coord(x64, bc, g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   R.tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
   check_tight(ic, bc, g, P, S, B, R.tfar, R.tstamp);
   float al = interp_alpha(R.theta + S.toff, ic, P, nullptr, nullptr);
-  float2 gv = R.gV[s], vp = R.Vnext[s];
+  const float2 gv = in.gv, vp = in.vp;
   float gvx = gv.x + P.dt * gx.x;
   float gvy = gv.y + P.dt * gx.y;
   float D = __builtin_amdgcn_rcpf(fmaf(P.dt, al, 1.f));
@@ -944,6 +953,10 @@ __device__ __forceinline__ void rpw_slot(const RpwArgs &R, int s, int bc, float2
   R.abar[s] = make_float2(dtD * gvx, dtD * gvy);
   R.galpha[s] = -dtD * (vp.x * gvx + vp.y * gvy);
   R.gV[s] = make_float2(D * gvx, D * gvy);
+}
+__device__ __forceinline__ void rpw_slot(const RpwArgs &R, int s, int bc, float2 gx,
+                                         const Scene32 &S, const Batch &B) {
+  rpw_finish(R, rpw_load(R, s), s, bc, gx, S, B);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1077,6 +1090,8 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
                  sgr[i] = d.gr;
                });
   if (!valid_) return;
+  RpwIn rin{};
+  if (kFuseR) rin = rpw_load(R, s);
   const float2 xi = xr[s];
   const float gri = grho[s];
   const int bc = bcell[s];
@@ -1105,7 +1120,7 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   const float2 gxn = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
   gX[s] = gxn;
   // the R pass of the previous step (same slot order) on this step's final gX
-  if (kFuseR) rpw_slot(R, s, bc, gxn, S, B);
+  if (kFuseR) rpw_finish(R, rin, s, bc, gxn, S, B);
 }
 
 }  // namespace
