@@ -71,12 +71,13 @@ VALU_FILE = os.path.join(ROOT, "profiles", "valu_busy_block_1000x1000.json")
 SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint_r": "k_adj_r",
                   "adjoint_p1": "k_adj_p1", "adjoint_p2": "k_adj_p2"},
           # the f32c tapes keep (pterm, invrho) and the masks: most P2 launches
-          # also run the next step's pointwise R pass
+          # also run the next step's pointwise R pass, and P1 launches carry
+          # the step's gtheta gather
           "f32c": {"density": "k_density32", "force_integrate": "k_force32",
-                   "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
+                   "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p1_theta32<true>",
                    "adjoint_p2": "k_adj_p232<true>"},
           "f32c_ckpt": {"density": "k_density32", "force_integrate": "k_force32",
-                        "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p132<true>",
+                        "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p1_theta32<true>",
                         "adjoint_p2": "k_adj_p232<true>"}}
 
 

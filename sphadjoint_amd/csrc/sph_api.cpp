@@ -1996,24 +1996,31 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
           }
         });
       }
-      timed(c, K_THETA, [&] {
-        launch_theta_gather_keyed_f32(s, ws->tkey, Xk, ws->galpha32, E.cstart, ws->tfar,
-                                      ws->tstamp, B, tfirst, ws->tpart);
-      });
+      const ThetaKeyed TA{ws->tkey, Xk, ws->galpha32, E.cstart, ws->tfar, ws->tstamp,
+                          tfirst ? 1 : 0, ws->tpart};
       // the exact list of these positions: the forward's density pass kept it
       // on the tape, or the density recompute just wrote it; with stored
       // (pterm, invrho) and no stored list, P1 tests the pairs and writes it
       const uint32_t *exact = tape->EM ? tape->EM + off * (3 * kMaskWords) : ws->em;
-      timed(c, K_ADJ_P1, [&] {
-        if (tape->keep_pi && !tape->EM)
-          launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             nm, false, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
-                             ws->grho32, ws->em);
-        else
-          launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d, E.scell,
-                             exact, true, E.cstart, lane, BS.at(TK_ADJ_P1), ws->gXs32, ws->gV32,
-                             ws->grho32, nullptr);
-      });
+      const bool p1_exact = !(tape->keep_pi && !tape->EM);
+      const uint32_t *p1_nm = p1_exact ? exact : nm;
+      uint32_t *p1_em = p1_exact ? nullptr : ws->em;
+      const Batch &BP1 = BS.at(TK_ADJ_P1);
+      if (BP1.bs == kBlock) {
+        // the gtheta gather runs in P1's launch (timed with P1)
+        timed(c, K_ADJ_P1, [&] {
+          launch_adj_p1_theta_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d,
+                                   E.scell, p1_nm, p1_exact, E.cstart, lane, BP1, ws->gXs32,
+                                   ws->gV32, ws->grho32, p1_em, TA);
+        });
+      } else {
+        timed(c, K_THETA, [&] { launch_theta_gather_keyed_f32(s, TA, B); });
+        timed(c, K_ADJ_P1, [&] {
+          launch_adj_p1_f32c(s, Xk, ws->xr32, Vk, PIk, ws->abar32, ws->galpha32, theta->d,
+                             E.scell, p1_nm, p1_exact, E.cstart, lane, BP1, ws->gXs32, ws->gV32,
+                             ws->grho32, p1_em);
+        });
+      }
       // P2; when the next step of the sweep (k-1) is in this epoch and its
       // R pass needs no density recompute, each slot runs that R pass on its
       // final gX in the same launch (its xr goes to the spare buffer)
@@ -2626,7 +2633,10 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
       for (int k = 0; k < TK_COUNT; ++k) {
         const int64_t n = c->launches[kid[k]];
         if (n == 0) continue;
-        const double t = c->tot_ms[kid[k]] / (double)n;
+        // P1 at 256 threads carries the gtheta gather in its launch: compare
+        // P1 + gather at every size
+        const double t =
+            (c->tot_ms[kid[k]] + (k == TK_ADJ_P1 ? c->tot_ms[K_THETA] : 0.0)) / (double)n;
         if (t < best[k]) {
           best[k] = t;
           pick[k] = v;

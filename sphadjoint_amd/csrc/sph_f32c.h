@@ -170,16 +170,31 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,
                         const Batch &B, float2 *gX, const RpwArgs *prev = nullptr);
-// deterministic gtheta gather over the keys of the R passes (sph_theta.hip):
-// the theta-cell partials of one step, written (first) or added to part; the
-// gather sweeps the tight SPH ranges unless this step's R pass stored its
-// stamp in *tfar.  launch_theta_nodes_keyed_f32 adds the node sums of part to
-// gtheta (once per neighbour-list epoch).
-void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *X,
-                                   const float *galpha, const int *cstart, const int *tfar,
-                                   int tstamp, const Batch &B, bool first, double4 *part);
+// deterministic gtheta gather over the keys of the R passes: the theta-cell
+// partials of one step, written (first != 0) or added to part; the gather
+// sweeps the tight SPH ranges unless this step's R pass stored tstamp in
+// *tfar.  launch_theta_nodes_keyed_f32 (sph_theta.hip) adds the node sums of
+// part to gtheta (once per neighbour-list epoch).
+struct ThetaKeyed {
+  const int *key;
+  const double2 *X;
+  const float *galpha;
+  const int *cstart;
+  const int *tfar;
+  int tstamp, first;
+  double4 *part;
+};
+void launch_theta_gather_keyed_f32(hipStream_t s, const ThetaKeyed &A, const Batch &B);
 void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batch &B,
                                   double *gtheta);
+// launch_adj_p1_f32c and launch_theta_gather_keyed_f32 as one launch (same
+// results); B must have 256-thread workgroups (B.bs == kBlock)
+void launch_adj_p1_theta_f32c(hipStream_t s, const double2 *X, const float2 *xr,
+                              const float2 *V, const float2 *PI, const float2 *abar,
+                              const float *galpha, const double *theta, const int *bcell,
+                              const uint32_t *nm, bool exact, const int *cstart, const int *lane,
+                              const Batch &B, float2 *gXs, float2 *gV, float *grho, uint32_t *em,
+                              const ThetaKeyed &A);
 
 // float2 <-> double2 helpers and float2 layout remaps (sph_cells.hip)
 void launch_f2_to_d2(hipStream_t s, const float2 *src, int N, double2 *dst);

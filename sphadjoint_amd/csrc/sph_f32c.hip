@@ -182,8 +182,9 @@ __device__ __forceinline__ uint32_t ldu(Rsrc r, uint32_t off) {
 // of this workgroup's slots.  Threads past the scene's end (valid_ false) take
 // part in the workgroup's barriers and then leave.
 constexpr int kLaneWin = 256;
-#define SCENE_WIN(B, lane)                                                             \
-  const int blk_ = xcd_block();                                                        \
+#define SCENE_WIN(B, lane) SCENE_WIN_AT(B, lane, xcd_block())
+#define SCENE_WIN_AT(B, lane, blk)                                                     \
+  const int blk_ = (blk);                                                              \
   const int scn_ = (B).b2s[blk_];                                                      \
   const Scene32 S = (B).sc[scn_];                                                      \
   const int bsl_ = (blk_ - (B).boff[scn_]) * (B).bs;                                   \
@@ -970,21 +971,251 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
   rpw_slot(R, s, bcell[s], gX[s], S, B);
 }
 
+// ---- gtheta gather over the keys of the pointwise pass -----------------------
+
+// The same gather over per-slot keys written by the f32c adjoint's pointwise
+// pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
+// interp cell, so that a candidate costs one key compare; a match's FP64
+// fractions (fx, fy) are formed from its position exactly as the f32c
+// interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
+// contributions and their order are those of k_theta_cells.
+// A workgroup takes a strip of kThetaStrip theta cells of one theta row
+// (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
+// strip sweeps, the keys of the slot run that covers the SPH cell ranges of
+// all its cells into LDS (coalesced); each cell then compares keys there and
+// reads positions and galpha only for its own slots, kMatchCap matches at a
+// time; the cell starts of the strip's rows are staged too, so that a cell's
+// sweep issues no dependent global loads before its matches.  A strip whose
+// runs exceed kThetaStage keys, that sweeps more than kThetaRows rows or more
+// than kThetaCols - 1 SPH cells per row, reads keys and cell starts in global
+// memory (same order, same sums).
+// Workgroups are dealt to the scenes of a batched state in order
+// (Scene32::tboff).
+constexpr int kThetaStage = 2048;
+constexpr int kThetaRows = 12;
+constexpr int kThetaCols = 64;
+constexpr int kMatchCap = 8;
+constexpr int kMatchBatch = 4;
+struct ThetaStripLds {
+  int skey[kThetaStage];
+  int slo[kThetaRows], soff[kThetaRows + 1];
+  int scs[kThetaRows][kThetaCols];   // cstart of cells X0 .. X1+1 of each row
+  int sok, sx0;
+  int smatch[kMatchCap][kBlock];     // each lane's matched slots, in order
+};
+
+// workgroup blk of the gather; FP64 without contraction, so that each
+// contribution is k_theta_cells' product bit for bit
+__device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const ThetaKeyed &A,
+                                            const Batch &B) {
+#pragma clang fp contract(off)
+  static_assert(kThetaStrip * kThetaLanes == kBlock, "one strip per workgroup");
+  int *const skey = L.skey, *const slo = L.slo, *const soff = L.soff;
+  int (*const scs)[kThetaCols] = L.scs;
+  int (*const smatch)[kBlock] = L.smatch;
+  int &sok = L.sok, &sx0 = L.sx0;
+  const int *__restrict__ key = A.key;
+  const double2 *__restrict__ X = A.X;
+  const float *__restrict__ galpha = A.galpha;
+  const int *__restrict__ cstart = A.cstart;
+  const int tstamp = A.tstamp, first = A.first;
+  double4 *__restrict__ part = A.part;
+  const Scene32 &Sc = B.sc[B.tb2s[blk]];
+  const Grid g = Sc.g;
+  const ThetaGrid &T = Sc.T;
+  const int coff = Sc.coff, tcoff = Sc.tcoff;
+  const int ncx = T.ngx - 1, ncy = T.ngy - 1;
+  const int nstrip = (ncx + kThetaStrip - 1) / kThetaStrip;
+  const int lb = blk - Sc.tboff;
+  const int j0 = lb / nstrip;
+  const int ifirst = (lb % nstrip) * kThetaStrip;
+  const int ilast = min(ifirst + kThetaStrip, ncx) - 1;
+  const int lane = threadIdx.x & (kThetaLanes - 1);
+  const int i0 = ifirst + (int)threadIdx.x / kThetaLanes;
+  // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
+  // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
+  // found a slot further out this step, else one cell (theta_cell_range)
+  const int2 *tr = (*A.tfar == tstamp ? B.trange : B.trange_tight) + Sc.troff;
+  auto xrange = [&](int i, int &cx0, int &cx1) {
+    const int2 r = tr[i];
+    cx0 = r.x;
+    cx1 = r.y;
+  };
+  const int2 ry = tr[ncx + j0];
+  const int cy0 = ry.x, cy1 = ry.y;
+  const int nrow = cy1 - cy0 + 1;
+  // plan: row k of the strip stages slots [lo_k, lo_k + len_k) at soff[k]
+  if (threadIdx.x < 64) {
+    const int k = (int)threadIdx.x;
+    int X0, X1, dummy;
+    xrange(ifirst, X0, dummy);
+    xrange(ilast, dummy, X1);
+    int lo = 0, len = 0;
+    if (k < nrow && k < kThetaRows) {
+      const int base = coff + (cy0 + k) * g.nx;
+      lo = cstart[base + X0];
+      len = cstart[base + X1 + 1] - lo;
+    }
+    int inc = len;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int t = __shfl_up(inc, d, 64);
+      if (k >= d) inc += t;
+    }
+    if (k < kThetaRows) {
+      slo[k] = lo;
+      soff[k] = inc - len;
+    }
+    const int total = __shfl(inc, kThetaRows - 1, 64);
+    if (k == 0) {
+      soff[kThetaRows] = total;
+      sx0 = X0;
+      sok = (nrow <= kThetaRows && total <= kThetaStage && X1 - X0 + 2 <= kThetaCols) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  const bool staged = sok != 0;
+  if (staged) {
+    // cell starts of the rows, then the keys; each thread issues the loads of
+    // four elements before storing them
+    const int X0 = sx0;
+    const int ncol = g.nx - X0 + 1 < kThetaCols ? g.nx - X0 + 1 : kThetaCols;
+    const int ncs = nrow * kThetaCols, total = soff[kThetaRows];
+    int off[kThetaRows];
+#pragma unroll
+    for (int q = 0; q < kThetaRows; ++q) off[q] = soff[q];
+    for (int i0 = (int)threadIdx.x; i0 < ncs + total; i0 += 4 * kBlock) {
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        v[u] = 0;
+        if (i < ncs) {
+          const int k = i / kThetaCols, c = i % kThetaCols;
+          if (c < ncol) v[u] = cstart[coff + (cy0 + k) * g.nx + X0 + c];
+        } else if (i < ncs + total) {
+          const int t = i - ncs;
+          int k = 0;
+#pragma unroll
+          for (int q = 1; q < kThetaRows; ++q) k += (t >= off[q]) ? 1 : 0;
+          v[u] = key[slo[k] + (t - off[k])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * kBlock;
+        if (i < ncs) (&scs[0][0])[i] = v[u];
+        else if (i < ncs + total) skey[i - ncs] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+  if (i0 > ilast) return;          // the kThetaLanes lanes of a group leave together
+  const int mykey = tcoff + j0 + i0 * ncy;
+  const PhysF32 &P = Sc.P;
+  auto frac = [&](double2 x) {
+    double gx = x.x * P.inv_hx64, gy = x.y * P.inv_hy64;
+    gx = (gx > P.gxclamp) ? P.gxclamp : ((gx < 0.0) ? 0.0 : gx);
+    gy = (gy > P.gyclamp) ? P.gyclamp : ((gy < 0.0) ? 0.0 : gy);
+    return make_double2(gx - (double)i0, gy - (double)j0);
+  };
+  int cx0, cx1;
+  xrange(i0, cx0, cx1);
+  double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
+  auto add = [&](double2 f, double ga) {
+    s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
+    s10 += (ga * f.x) * (1.0 - f.y);
+    s01 += (ga * (1.0 - f.x)) * f.y;
+    s11 += (ga * f.x) * f.y;
+  };
+  // the lane's matches in sweep order, collected up to kMatchCap at a time
+  // (their positions, in LDS), then their data read in batches of
+  // kMatchBatch independent loads; the sweep resumes where it stopped
+  const int tid = (int)threadIdx.x;
+  int cy = cy0, p = 0, e = 0, ro = 0;
+  auto row = [&](int y) {   // start the sweep of SPH cell row y
+    const int k = y - cy0;
+    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + y * g.nx + cx0];
+    e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + y * g.nx + cx1 + 1];
+    ro = staged ? soff[k] - slo[k] : 0;
+    p = b + lane;
+  };
+  row(cy);
+  for (;;) {
+    int n = 0;
+    while (n < kMatchCap) {
+      if (p >= e) {
+        if (++cy > cy1) break;
+        row(cy);
+        continue;
+      }
+      if ((staged ? skey[p + ro] : key[p]) == mykey) smatch[n++][tid] = p;
+      p += kThetaLanes;
+    }
+    for (int m0 = 0; m0 < n; m0 += kMatchBatch) {
+      double2 f[kMatchBatch];
+      double ga[kMatchBatch];
+#pragma unroll
+      for (int q = 0; q < kMatchBatch; ++q) {
+        if (m0 + q < n) {
+          const int pm = smatch[m0 + q][tid];
+          f[q] = X[pm];
+          ga[q] = (double)galpha[pm];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kMatchBatch; ++q)
+        if (m0 + q < n) add(frac(f[q]), ga[q]);
+    }
+    if (n < kMatchCap) break;   // the sweep has ended
+  }
+#pragma unroll
+  for (int m = kThetaLanes / 2; m > 0; m >>= 1) {
+    s00 += __shfl_xor(s00, m, kThetaLanes);
+    s10 += __shfl_xor(s10, m, kThetaLanes);
+    s01 += __shfl_xor(s01, m, kThetaLanes);
+    s11 += __shfl_xor(s11, m, kThetaLanes);
+  }
+  if (lane == 0) {
+    double4 &pt = part[tcoff + j0 + i0 * ncy];
+    if (first) {
+      pt = make_double4(s00, s10, s01, s11);
+    } else {
+      const double4 o = pt;
+      pt = make_double4(o.x + s00, o.y + s10, o.z + s01, o.w + s11);
+    }
+  }
+}
+
+
+__global__ void __launch_bounds__(kBlock) k_theta_strip_keyed(ThetaKeyed A, Batch B) {
+  __shared__ ThetaStripLds L;
+  theta_strip(L, (int)blockIdx.x, A, B);
+}
+
 // adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185).
 // kExact: nm is already the exact list of these positions; otherwise pairs are
 // tested and the exact list is written to em for P2.
+struct P1Lds {
+  StageTab T;
+  float2 sx[kStageCapP1], spi[kStageCapP1], sab[kStageCapP1], sv[kStageCapP1];
+};
+#define P1_PARAMS                                                                           \
+  const double2 *__restrict__ X, const float2 *__restrict__ xr,                             \
+      const float2 *__restrict__ V, const float2 *__restrict__ PI,                          \
+      const float2 *__restrict__ abar, const float *__restrict__ galpha,                    \
+      const double *__restrict__ theta, const int *__restrict__ bcell,                      \
+      const uint32_t *__restrict__ nm, const int *__restrict__ cstart,                      \
+      const int *__restrict__ lane, Batch B, float2 *__restrict__ gXs,                      \
+      float2 *__restrict__ gV, float *__restrict__ grho, uint32_t *__restrict__ em
+#define P1_ARGS X, xr, V, PI, abar, galpha, theta, bcell, nm, cstart, lane, B, gXs, gV, grho, em
+
+// workgroup blk of P1
 template <bool kExact>
-__global__ void __launch_bounds__(kBlock)
-k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
-           const float2 *__restrict__ V, const float2 *__restrict__ PI,
-           const float2 *__restrict__ abar, const float *__restrict__ galpha,
-           const double *__restrict__ theta, const int *__restrict__ bcell,
-           const uint32_t *__restrict__ nm, const int *__restrict__ cstart,
-           const int *__restrict__ lane, Batch B, float2 *__restrict__ gXs,
-           float2 *__restrict__ gV, float *__restrict__ grho, uint32_t *__restrict__ em) {
-  __shared__ StageTab T;
-  __shared__ float2 sx[kStageCapP1], spi[kStageCapP1], sab[kStageCapP1], sv[kStageCapP1];
-  SCENE_WIN(B, lane);
+__device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
+  StageTab &T = L.T;
+  float2 *const sx = L.sx, *const spi = L.spi, *const sab = L.sab, *const sv = L.sv;
+  SCENE_WIN_AT(B, lane, blk);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCapP1);
   if (T.ok)
@@ -1068,6 +1299,35 @@ k_adj_p132(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   grho[s] = grv;
   float2 gvo = gV[s];
   gV[s] = make_float2(gvo.x + gvx.value(), gvo.y + gvy.value());
+}
+
+template <bool kExact>
+__global__ void __launch_bounds__(kBlock) k_adj_p132(P1_PARAMS) {
+  __shared__ P1Lds L;
+  adj_p1<kExact>(L, xcd_block(), P1_ARGS);
+}
+
+// P1 and the gtheta gather of the same step in one launch: both read only
+// what the pointwise pass and the tape hold, and neither reads what the other
+// writes.  Gather workgroups are spread evenly among the P1 workgroups (block
+// b is a gather block when floor((b+1) nt / n) > floor(b nt / n)), so that
+// they fill the SIMD time P1 leaves while its waves wait on LDS and memory,
+// and the two share one LDS allocation.  Each workgroup runs the body it
+// would run in its own kernel, so every result is unchanged.
+union P1ThetaLds {
+  P1Lds p1;
+  ThetaStripLds th;
+};
+template <bool kExact>
+__global__ void __launch_bounds__(kBlock) k_adj_p1_theta32(P1_PARAMS, ThetaKeyed A) {
+  __shared__ P1ThetaLds L;
+  const int lb = xcd_block();
+  const long long nt = B.ntblk, n = gridDim.x;
+  const int tb = (int)((long long)lb * nt / n);
+  if ((int)((long long)(lb + 1) * nt / n) > tb)
+    theta_strip(L.th, tb, A, B);
+  else
+    adj_p1<kExact>(L.p1, lb - tb, P1_ARGS);
 }
 
 // adjoint P2: adj_pass2_kernel! and gX += gXs (src/adjoint.jl:126-156,187-191)
@@ -1204,6 +1464,26 @@ void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const
   else
     k_adj_p132<false><<<B.nblk, B.bs, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
                                                 cstart, lane, B, gXs, gV, grho, em);
+}
+void launch_theta_gather_keyed_f32(hipStream_t s, const ThetaKeyed &A, const Batch &B) {
+  if (B.ntblk > 0) k_theta_strip_keyed<<<B.ntblk, kBlock, 0, s>>>(A, B);
+}
+void launch_adj_p1_theta_f32c(hipStream_t s, const double2 *X, const float2 *xr,
+                              const float2 *V, const float2 *PI, const float2 *abar,
+                              const float *galpha, const double *theta, const int *bcell,
+                              const uint32_t *nm, bool exact, const int *cstart, const int *lane,
+                              const Batch &B, float2 *gXs, float2 *gV, float *grho, uint32_t *em,
+                              const ThetaKeyed &A) {
+  if (B.bs != kBlock) throw std::logic_error("P1 + gtheta launch needs 256-thread workgroups");
+  if ((long long)B.nblk + B.ntblk > 0x7fffffffLL) throw std::logic_error("P1 + gtheta grid");
+  if (B.nblk + B.ntblk == 0) return;
+  const int n = B.nblk + B.ntblk;
+  if (exact)
+    k_adj_p1_theta32<true><<<n, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                                  cstart, lane, B, gXs, gV, grho, nullptr, A);
+  else
+    k_adj_p1_theta32<false><<<n, kBlock, 0, s>>>(X, xr, V, PI, abar, galpha, theta, bcell, nm,
+                                                   cstart, lane, B, gXs, gV, grho, em, A);
 }
 void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, const float2 *gXs,
                         const int *bcell, const uint32_t *nm, const int *cstart, const int *lane,

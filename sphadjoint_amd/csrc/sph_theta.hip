@@ -1,4 +1,6 @@
-// sph_theta.hip -- deterministic gtheta reduction (no float atomics).
+// sph_theta.hip -- deterministic gtheta reduction (no float atomics): the FP64
+// form's gather and the node sums of both forms (the f32c form's keyed gather
+// is in sph_f32c.hip, where P1 runs it in its own launch).
 //
 // The reference scatters each particle's four bilinear weights into gtheta
 // with float atomics (adj_design_kernel!, src/adjoint.jl:166-185), whose
@@ -105,210 +107,6 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
   }
 }
 
-// The same gather over per-slot keys written by the f32c adjoint's pointwise
-// pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
-// interp cell, so that a candidate costs one key compare; a match's FP64
-// fractions (fx, fy) are formed from its position exactly as the f32c
-// interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
-// contributions and their order are those of k_theta_cells.
-// A workgroup takes a strip of kThetaStrip theta cells of one theta row
-// (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
-// strip sweeps, the keys of the slot run that covers the SPH cell ranges of
-// all its cells into LDS (coalesced); each cell then compares keys there and
-// reads positions and galpha only for its own slots, kMatchCap matches at a
-// time; the cell starts of the strip's rows are staged too, so that a cell's
-// sweep issues no dependent global loads before its matches.  A strip whose
-// runs exceed kThetaStage keys, that sweeps more than kThetaRows rows or more
-// than kThetaCols - 1 SPH cells per row, reads keys and cell starts in global
-// memory (same order, same sums).
-// Workgroups are dealt to the scenes of a batched state in order
-// (Scene32::tboff).
-constexpr int kThetaStage = 2048;
-constexpr int kThetaRows = 12;
-constexpr int kThetaCols = 64;
-constexpr int kMatchCap = 8;
-constexpr int kMatchBatch = 4;
-
-template <class GA>
-__global__ void __launch_bounds__(kBlock)
-k_theta_strip_keyed(const int *__restrict__ key, const double2 *__restrict__ X,
-                    const GA *__restrict__ galpha, const int *__restrict__ cstart,
-                    const int *__restrict__ tfar, int tstamp, Batch B, int first,
-                    double4 *__restrict__ part) {
-  static_assert(kThetaStrip * kLanes == kBlock, "one strip per workgroup");
-  __shared__ int skey[kThetaStage];
-  __shared__ int slo[kThetaRows], soff[kThetaRows + 1];
-  __shared__ int scs[kThetaRows][kThetaCols];   // cstart of cells X0 .. X1+1 of each row
-  __shared__ int sok, sx0;
-  __shared__ int smatch[kMatchCap][kBlock];      // each lane's matched slots, in order
-  const int blk = blockIdx.x;
-  const Scene32 &Sc = B.sc[B.tb2s[blk]];
-  const Grid g = Sc.g;
-  const ThetaGrid &T = Sc.T;
-  const int coff = Sc.coff, tcoff = Sc.tcoff;
-  const int ncx = T.ngx - 1, ncy = T.ngy - 1;
-  const int nstrip = (ncx + kThetaStrip - 1) / kThetaStrip;
-  const int lb = blk - Sc.tboff;
-  const int j0 = lb / nstrip;
-  const int ifirst = (lb % nstrip) * kThetaStrip;
-  const int ilast = min(ifirst + kThetaStrip, ncx) - 1;
-  const int lane = threadIdx.x & (kLanes - 1);
-  const int i0 = ifirst + (int)threadIdx.x / kLanes;
-  // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
-  // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
-  // found a slot further out this step, else one cell (theta_cell_range)
-  const int2 *tr = (*tfar == tstamp ? B.trange : B.trange_tight) + Sc.troff;
-  auto xrange = [&](int i, int &cx0, int &cx1) {
-    const int2 r = tr[i];
-    cx0 = r.x;
-    cx1 = r.y;
-  };
-  const int2 ry = tr[ncx + j0];
-  const int cy0 = ry.x, cy1 = ry.y;
-  const int nrow = cy1 - cy0 + 1;
-  // plan: row k of the strip stages slots [lo_k, lo_k + len_k) at soff[k]
-  if (threadIdx.x < 64) {
-    const int k = (int)threadIdx.x;
-    int X0, X1, dummy;
-    xrange(ifirst, X0, dummy);
-    xrange(ilast, dummy, X1);
-    int lo = 0, len = 0;
-    if (k < nrow && k < kThetaRows) {
-      const int base = coff + (cy0 + k) * g.nx;
-      lo = cstart[base + X0];
-      len = cstart[base + X1 + 1] - lo;
-    }
-    int inc = len;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-      const int t = __shfl_up(inc, d, 64);
-      if (k >= d) inc += t;
-    }
-    if (k < kThetaRows) {
-      slo[k] = lo;
-      soff[k] = inc - len;
-    }
-    const int total = __shfl(inc, kThetaRows - 1, 64);
-    if (k == 0) {
-      soff[kThetaRows] = total;
-      sx0 = X0;
-      sok = (nrow <= kThetaRows && total <= kThetaStage && X1 - X0 + 2 <= kThetaCols) ? 1 : 0;
-    }
-  }
-  __syncthreads();
-  const bool staged = sok != 0;
-  if (staged) {
-    // cell starts of the rows, then the keys; each thread issues the loads of
-    // four elements before storing them
-    const int X0 = sx0;
-    const int ncol = g.nx - X0 + 1 < kThetaCols ? g.nx - X0 + 1 : kThetaCols;
-    const int ncs = nrow * kThetaCols, total = soff[kThetaRows];
-    int off[kThetaRows];
-#pragma unroll
-    for (int q = 0; q < kThetaRows; ++q) off[q] = soff[q];
-    for (int i0 = (int)threadIdx.x; i0 < ncs + total; i0 += 4 * kBlock) {
-      int v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
-        v[u] = 0;
-        if (i < ncs) {
-          const int k = i / kThetaCols, c = i % kThetaCols;
-          if (c < ncol) v[u] = cstart[coff + (cy0 + k) * g.nx + X0 + c];
-        } else if (i < ncs + total) {
-          const int t = i - ncs;
-          int k = 0;
-#pragma unroll
-          for (int q = 1; q < kThetaRows; ++q) k += (t >= off[q]) ? 1 : 0;
-          v[u] = key[slo[k] + (t - off[k])];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
-        if (i < ncs) (&scs[0][0])[i] = v[u];
-        else if (i < ncs + total) skey[i - ncs] = v[u];
-      }
-    }
-    __syncthreads();
-  }
-  if (i0 > ilast) return;          // the kLanes lanes of a group leave together
-  const int mykey = tcoff + j0 + i0 * ncy;
-  const PhysF32 &P = Sc.P;
-  auto frac = [&](double2 x) {
-    double gx = x.x * P.inv_hx64, gy = x.y * P.inv_hy64;
-    gx = (gx > P.gxclamp) ? P.gxclamp : ((gx < 0.0) ? 0.0 : gx);
-    gy = (gy > P.gyclamp) ? P.gyclamp : ((gy < 0.0) ? 0.0 : gy);
-    return make_double2(gx - (double)i0, gy - (double)j0);
-  };
-  int cx0, cx1;
-  xrange(i0, cx0, cx1);
-  double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
-  auto add = [&](double2 f, double ga) {
-    s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
-    s10 += (ga * f.x) * (1.0 - f.y);
-    s01 += (ga * (1.0 - f.x)) * f.y;
-    s11 += (ga * f.x) * f.y;
-  };
-  // the lane's matches in sweep order, collected up to kMatchCap at a time
-  // (their positions, in LDS), then their data read in batches of
-  // kMatchBatch independent loads; the sweep resumes where it stopped
-  const int tid = (int)threadIdx.x;
-  int cy = cy0, p = 0, e = 0, ro = 0;
-  auto row = [&](int y) {   // start the sweep of SPH cell row y
-    const int k = y - cy0;
-    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + y * g.nx + cx0];
-    e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + y * g.nx + cx1 + 1];
-    ro = staged ? soff[k] - slo[k] : 0;
-    p = b + lane;
-  };
-  row(cy);
-  for (;;) {
-    int n = 0;
-    while (n < kMatchCap) {
-      if (p >= e) {
-        if (++cy > cy1) break;
-        row(cy);
-        continue;
-      }
-      if ((staged ? skey[p + ro] : key[p]) == mykey) smatch[n++][tid] = p;
-      p += kLanes;
-    }
-    for (int m0 = 0; m0 < n; m0 += kMatchBatch) {
-      double2 f[kMatchBatch];
-      double ga[kMatchBatch];
-#pragma unroll
-      for (int q = 0; q < kMatchBatch; ++q) {
-        if (m0 + q < n) {
-          const int pm = smatch[m0 + q][tid];
-          f[q] = X[pm];
-          ga[q] = (double)galpha[pm];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kMatchBatch; ++q)
-        if (m0 + q < n) add(frac(f[q]), ga[q]);
-    }
-    if (n < kMatchCap) break;   // the sweep has ended
-  }
-#pragma unroll
-  for (int m = kLanes / 2; m > 0; m >>= 1) {
-    s00 += __shfl_xor(s00, m, kLanes);
-    s10 += __shfl_xor(s10, m, kLanes);
-    s01 += __shfl_xor(s01, m, kLanes);
-    s11 += __shfl_xor(s11, m, kLanes);
-  }
-  if (lane == 0) {
-    double4 &pt = part[tcoff + j0 + i0 * ncy];
-    if (first) {
-      pt = make_double4(s00, s10, s01, s11);
-    } else {
-      const double4 o = pt;
-      pt = make_double4(o.x + s00, o.y + s10, o.z + s01, o.w + s11);
-    }
-  }
-}
-
 // node sums of the theta-cell partials, per scene (workgroups dealt in order
 // from Scene32::nboff)
 __global__ void k_theta_nodes_b(const double4 *__restrict__ part, Batch B,
@@ -353,14 +151,6 @@ void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galph
 
 void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta) {
   k_theta_nodes<<<nblocks((long long)T.ngx * T.ngy), kBlock, 0, s>>>(part, T, gtheta);
-}
-
-void launch_theta_gather_keyed_f32(hipStream_t s, const int *key, const double2 *X,
-                                   const float *galpha, const int *cstart, const int *tfar,
-                                   int tstamp, const Batch &B, bool first, double4 *part) {
-  static_assert(kLanes == kThetaLanes, "theta-cell launch geometry");
-  k_theta_strip_keyed<float><<<B.ntblk, kBlock, 0, s>>>(key, X, galpha, cstart, tfar, tstamp, B,
-                                                        first ? 1 : 0, part);
 }
 
 void launch_theta_nodes_keyed_f32(hipStream_t s, const double4 *part, const Batch &B,
