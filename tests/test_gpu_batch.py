@@ -103,13 +103,16 @@ def test_autotune_keeps_results(be):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("size", [64, 128])
-def test_small_workgroups_stage_the_whole_window(be, size):
+@pytest.mark.parametrize("size,keep", [(64, "plain"), (128, "plain"), (64, "pi"),
+                                       (128, "pi_lists")])
+def test_small_workgroups_stage_the_whole_window(be, size, keep):
     """With 64- or 128-thread workgroups the pair passes still deal each
     256-slot window by height, so every workgroup stages its whole window's
     rows: forward + adjoint equal the 256-thread launches bit for bit, on a
-    batch (scene-local windows)."""
-    say("workgroup size", size)
+    batch (scene-local windows).  At 256 threads P1 and the gtheta gather
+    share one launch, at the smaller sizes they launch apart; the tapes cover
+    P1 over the exact list (plain, pi_lists) and over the mask list (pi)."""
+    say("workgroup size", size, "tape", keep)
     scs = scenes.batch_line_search(seed=5, nscenes=3, nmin=800, nmax=6000, nsteps=16)
     args = [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs]
     outs = []
@@ -119,7 +122,8 @@ def test_small_workgroups_stage_the_whole_window(be, size):
                  "adjoint_p2")
         S.tuning(bst, {n: sz for n in names})
         th = bst.field([sc.theta for sc in scs])
-        tape = S.Tape(be, bst.N, 16, bst.p, precision="f32c")
+        tape = S.Tape(be, bst.N, 16, bst.p, precision="f32c", keep_pi=keep != "plain",
+                      keep_lists=keep == "pi_lists")
         S.simulate_(bst, th, bst.p, be, 16, tape=tape)
         J, seed = S.batch_objective(bst, 0.3, 0.2, 0.1)
         ws = S.BatchWorkspace(be, bst)
