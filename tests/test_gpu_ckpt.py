@@ -238,6 +238,19 @@ def test_long_horizon_5333_steps(be):
     ws = S.AdjointWorkspace(be, sc.N, pg)
     S.backward_(ws, tape, sc.theta, pg, be, seedX=seed, seedV=np.zeros_like(seed))
     ref = (ws.gX, ws.gV, ws.gtheta)
+    # the horizon's own sensitivity, for the record: FP64 again from X0 moved
+    # by a relative 1e-10 (normal), and how far its gradients move
+    rng = np.random.default_rng(11)
+    X0p = sc.X0 * (1.0 + 1e-10 * rng.standard_normal(sc.X0.shape))
+    st = S.State(be, X0p, sc.V0, pg)
+    S.reset_(tape)
+    S.simulate_(st, sc.theta, pg, be, n, tape=tape)
+    Jp, seedp = S.target_objective(st, 0.6, 0.2, 0.1)
+    S.backward_(ws, tape, sc.theta, pg, be, seedX=seedp, seedV=np.zeros_like(seedp))
+    dp = [float(np.abs(a - b).max() / np.abs(b).max())
+          for a, b in zip((ws.gX, ws.gV, ws.gtheta), ref)]
+    print("   FP64 from X0 (1 + 1e-10 N(0,1)): J %.3e  gX %.3e  gV %.3e  gtheta %.3e"
+          % (abs(Jp - J64) / abs(J64), *dp))
     del tape
     ck = S.Checkpointer(be, sc.N, n, pg, snapshots=64, precision="f32c")
     wsf = S.AdjointWorkspace(be, sc.N, pg, precision="f32c")
@@ -253,6 +266,6 @@ def test_long_horizon_5333_steps(be):
             np.testing.assert_array_equal(a, b)
     eJ = abs(runs[0][0] - J64) / abs(J64)
     e = [float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(runs[0][1:], ref)]
-    print("   J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
+    print("   f32c vs FP64: J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
     assert eJ < 1e-4
     assert e[2] < 1e-2
