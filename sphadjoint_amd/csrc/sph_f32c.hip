@@ -1095,10 +1095,15 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
           if (c < ncol) v[u] = cstart[coff + (cy0 + k) * g.nx + X0 + c];
         } else if (i < ncs + total) {
           const int t = i - ncs;
+          // the run holding t; only the strip's nrow rows hold keys (nrow
+          // is the same in every lane, so the search stops there for all)
           int k = 0;
 #pragma unroll
-          for (int q = 1; q < kThetaRows; ++q) k += (t >= off[q]) ? 1 : 0;
-          v[u] = key[slo[k] + (t - off[k])];
+          for (int q = 1; q < kThetaRows; ++q) {
+            if (q >= nrow) break;
+            k += (t >= off[q]) ? 1 : 0;
+          }
+          v[u] = key[slo[k] + (t - soff[k])];
         }
       }
 #pragma unroll
