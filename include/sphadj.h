@@ -197,8 +197,10 @@ int sph_batch_objective_dev(sph_state *st, double xc, double yc, double sigma, d
  * workgroup size (64, 128 or 256) of the neighbour-list build, density,
  * force, adjoint R, P1 and P2 kernels by timing each candidate over `steps`
  * steps with a tape and their reverse sweep, and keeps the choice with the
- * state (tapes recorded afterwards inherit it).  Advances st by 6*steps
- * steps.  chosen (may be NULL) receives the 6 sizes in that order. */
+ * state (tapes recorded afterwards inherit it).  At 256 threads P1 also runs
+ * the step's gtheta gather in its launch, so P1's sizes are compared on P1 +
+ * gather.  Advances st by 6*steps steps.  chosen (may be NULL) receives the
+ * 6 sizes in that order. */
 int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *chosen);
 /* the tuned sizes of a state (6 entries, the order of sph_autotune), and
  * their transfer to another state of the same kind */
