@@ -67,7 +67,16 @@ typedef struct sph_workspace sph_workspace;
 /* --- context: replaces the KernelAbstractions `backend` object (src/driver.jl:24) --- */
 const char *sph_last_error(void);
 int sph_ctx_create(int device, sph_ctx **out);
+/* The same on a stream the caller owns (a hipStream_t passed as void*, e.g.
+ * the HIP queue of a Julia AMDGPU task): every call of the context issues into
+ * that stream, in the caller's order; the stream must belong to `device` and
+ * outlive the context, which never destroys it.  The reference's kernels
+ * launch on the backend's own queue (src/forward.jl:166-171). */
+int sph_ctx_create_stream(int device, void *stream, sph_ctx **out);
 int sph_ctx_destroy(sph_ctx *ctx);
+/* free and total device memory of the context's device (hipMemGetInfo); no
+ * counterpart in the reference */
+int sph_mem_info(sph_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 /* KernelAbstractions.synchronize(backend) (src/driver.jl:110,204) */
 int sph_sync(sph_ctx *ctx);
 

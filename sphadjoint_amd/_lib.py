@@ -36,7 +36,9 @@ _PP = C.POINTER(sph_params)
 SIGNATURES = {
     "sph_last_error": (C.c_char_p, []),
     "sph_ctx_create": (C.c_int, [C.c_int, _pp]),
+    "sph_ctx_create_stream": (C.c_int, [C.c_int, _vp, _pp]),
     "sph_ctx_destroy": (C.c_int, [_vp]),
+    "sph_mem_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     "sph_sync": (C.c_int, [_vp]),
     "sph_field_create": (C.c_int, [_vp, _i64, _i64, _vp, _pp]),
     "sph_field_set": (C.c_int, [_vp, _vp]),

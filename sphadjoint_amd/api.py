@@ -38,16 +38,29 @@ def _ptr(a):
 
 
 class MI355XBackend:
-    """One HIP device and one in-order stream (the KA backend's role)."""
+    """One HIP device and one in-order stream (the KA backend's role).  With
+    ``stream`` (a hipStream_t of the library's HIP runtime, as an int or
+    c_void_p) the backend issues into that caller-owned stream instead of a
+    stream of its own."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream=None):
         h = C.c_void_p()
-        L.call("sph_ctx_create", int(device), C.byref(h))
+        if stream is None:
+            L.call("sph_ctx_create", int(device), C.byref(h))
+        else:
+            L.call("sph_ctx_create_stream", int(device), C.c_void_p(int(getattr(stream, "value", stream))),
+                   C.byref(h))
         self._h = h
         self.device = device
 
     def synchronize(self):
         L.call("sph_sync", self._h)
+
+    def mem_info(self):
+        """(free, total) bytes of device memory."""
+        f, t = C.c_int64(), C.c_int64()
+        L.call("sph_mem_info", self._h, C.byref(f), C.byref(t))
+        return int(f.value), int(t.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -318,13 +331,8 @@ def check_neighbor_flags(nl: NeighborList):
     """check_neighbor_flags (src/neighbors.jl:282-290): warn, then reset."""
     f = nl.flags
     if f[0] != 0:
-        st = nl._st
-        if st is not None and st.precision == L.SPH_F32C:
-            # the f32c lists hold 128 slots per cell row of a particle's 3x3 block
-            warnings.warn("neighbour list row overflow: a cell row of a 3x3 block holds more "
-                          "than 128 particles (f32c mask lists); pairs past it are left out")
-        else:
-            warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
+        # (the f32c form never sets it: its passes test the slots past the masks)
+        warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
     if f[1] != 0:
         warnings.warn("neighbour-list rebuild interval too long (a displacement exceeded "
                       "skin*h); reduce interval or increase skin")

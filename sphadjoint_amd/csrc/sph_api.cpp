@@ -351,6 +351,7 @@ struct ProfRec {
 struct sph_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: the caller's stream (sph_ctx_create_stream)
   // objects created on this context plus the handle itself; the context is
   // freed when the handle has been destroyed and the last object is gone
   int refs = 1;
@@ -375,7 +376,7 @@ void ctx_release(sph_ctx *c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->evpool) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -804,6 +805,37 @@ int sph_ctx_create(int device, sph_ctx **out) {
       hip_check(e, "hipStreamCreateWithFlags");
     }
     *out = c;
+  });
+}
+
+int sph_ctx_create_stream(int device, void *stream, sph_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(SPH_E_ARG, "out is NULL");
+    if (!stream) fail(SPH_E_ARG, "stream is NULL (use sph_ctx_create for a stream of the library's own)");
+    int n = 0;
+    HIPC(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(SPH_E_ARG, "device %d not present (%d devices)", device, n);
+    HIPC(hipSetDevice(device));
+    hipStream_t hs = (hipStream_t)stream;
+    int sdev = -1;
+    HIPC(hipStreamGetDevice(hs, &sdev));
+    if (sdev != device) fail(SPH_E_ARG, "the stream belongs to device %d, not %d", sdev, device);
+    auto *c = new sph_ctx;
+    c->device = device;
+    c->stream = hs;
+    c->own_stream = false;
+    *out = c;
+  });
+}
+
+int sph_mem_info(sph_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+  return guarded([&] {
+    if (!ctx || !free_bytes || !total_bytes) fail(SPH_E_ARG, "NULL argument");
+    HIPC(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
   });
 }
 

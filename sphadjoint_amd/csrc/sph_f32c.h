@@ -114,7 +114,8 @@ void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *ce
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags);
 // first density pass after a rebuild, fused with the mask-list build: writes
-// the mask list nm, the exact list em, PI and the overflow flag (flags[0])
+// the mask list nm, the exact list em and PI (flags is not written: no pair is
+// ever left out, see kMaskRun)
 void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
                                const int *cstart, const int *lane, const Batch &B, float2 *PI,
                                uint32_t *nm, uint32_t *em, int *flags);

@@ -232,7 +232,10 @@ __device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &
 
 // Bitmask list: word w (32 bits) of row r of slot s sits at
 // nm[(w*3 + r)*N + s]; words beyond ceil(len/32) are neither written nor read.
-// Runs longer than kMaskRun slots are cut there and raise the overflow flag.
+// The masks of a run longer than kMaskRun slots cover its first kMaskRun
+// slots; every pass tests the slots past them directly (for_each_nb), so a
+// pile-up in one cell (particles beyond a wall gather in the clamped edge
+// cells) leaves no pair out.
 constexpr int kMaskRun = 32 * kMaskWords;
 
 // Loader index of a neighbour: slot j of row r is passed to the loader as
@@ -251,7 +254,9 @@ struct RowOff {
 // of the same layout, written to em, that later passes over the same
 // positions iterate without testing r2.  The loads of the next neighbour are
 // issued before the current pair is evaluated (two registers sets in turn),
-// so a word exposes one load latency, not one per pair.
+// so a word exposes one load latency, not one per pair.  Slots of a row run
+// past the kMaskRun slots its masks cover are tested directly (true
+// neighbours, r2min < r2 < (2h)^2, at these positions) after the masked ones.
 // one pair of for_each_nb: bit b of a word whose cells cx-1 / cx / cx+1 start
 // at bits <0 / s0 / s1; xm, x0, xp are the own x in those cells' frames
 template <class D, class Fn>
@@ -266,8 +271,9 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
 template <class Ld, class Fn>
 __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
                                             int bc, float2 xi, const Grid &g, int coff,
-                                            uint32_t Ntot, float cs, const RowOff &ro, Ld &&ld,
-                                            Fn &&fn) {
+                                            uint32_t Ntot, const PhysF32 &P, const RowOff &ro,
+                                            Ld &&ld, Fn &&fn) {
+  const float cs = P.cs;
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
@@ -304,6 +310,20 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
         }
       }
       if (em) em[widx] = hit;
+    }
+    // slots of a row run beyond the kMaskRun slots its masks cover (a pile-up,
+    // e.g. particles beyond a wall in a clamped edge cell): tested directly,
+    // in slot order after the masked ones, so no pair is left out
+    if (run.len > kMaskRun) {
+      const int rb = run.rb + ro.o[r];
+      for (int b = kMaskRun; b < run.len; ++b) {
+        const auto d = ld(rb + b);
+        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+        const float dx = xs - d.x.x;
+        const float dy = yi - d.x.y;
+        const float r2 = fmaf(dx, dx, dy * dy);
+        if (r2 > P.r2min && r2 < P.r2max) fn(d, dx, dy, r2);
+      }
     }
   }
 }
@@ -452,7 +472,6 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int bc = bcell[s];
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
-  bool over = false;
   auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -460,11 +479,8 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
       if (jy < 0 || jy >= g.ny) continue;
       const RowRun run = row_run(cst, c.x, jy, g, S.coff);
       const float yi = xi.y - (float)(r - 1) * P.cs;
-      int len = run.len;
-      if (len > kMaskRun) {
-        over = true;
-        len = kMaskRun;
-      }
+      // masks cover the first kMaskRun slots of the run; the passes test the rest
+      const int len = run.len < kMaskRun ? run.len : kMaskRun;
       const int rb = run.rb + ro.o[r];
       for (int w = 0; w * 32 < len; ++w) {
         uint32_t m = 0;
@@ -488,7 +504,6 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
     rows(RowOff{}, [&](int i) __attribute__((always_inline)) {
       return ld2(xrr, (uint32_t)i * 8u);
     });
-  if (over) flags[0] = 1;
 }
 
 // Thread -> slot assignment of the pair passes.  A pair pass walks the three
@@ -558,7 +573,7 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
   // largest term of the sum: every later term is positive and no larger, so
   // each step's error term is exact
   FSum acc(1.f);
-  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P.cs, ro, ld,
+  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       acc.add(density_term(r2, P));
@@ -631,7 +646,6 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int bc = bcell[s];
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
-  bool over = false;
   FSum acc(1.f);
   auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
 #pragma unroll
@@ -640,11 +654,7 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
       if (jy < 0 || jy >= g.ny) continue;
       const RowRun run = row_run(cst, c.x, jy, g, S.coff);
       const float yi = xi.y - (float)(r - 1) * P.cs;
-      int len = run.len;
-      if (len > kMaskRun) {
-        over = true;
-        len = kMaskRun;
-      }
+      const int len = run.len < kMaskRun ? run.len : kMaskRun;
       const int rb = run.rb + ro.o[r];
       for (int w = 0; w * 32 < len; ++w) {
         uint32_t m = 0, e = 0;
@@ -668,6 +678,15 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
         nm[widx] = m;
         em[widx] = e;
       }
+      // the run's slots past the masks, as the passes over these lists take them
+      for (int b = kMaskRun; b < run.len; ++b) {
+        const float2 xj = ld(rb + b);
+        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+        const float dx = xs - xj.x;
+        const float dy = yi - xj.y;
+        const float r2 = fmaf(dx, dx, dy * dy);
+        if (r2 > P.r2min && r2 < P.r2max) acc.add(density_term(r2, P));
+      }
     }
   };
   if (T.ok)
@@ -676,7 +695,6 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
     rows(RowOff{}, [&](int i) __attribute__((always_inline)) {
       return ld2(xrr, (uint32_t)i * 8u);
     });
-  if (over) flags[0] = 1;
   PI[s] = eos(density_finish(acc, P), P);
 }
 
@@ -723,13 +741,13 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   };
   // nm is the exact list of the density pass over these positions
   if (T.ok) {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) { return NbXPV{sx[i], spi[i], sv[i]}; },
                 pair);
   } else {
     const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
     const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
                   return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)};
@@ -1272,7 +1290,7 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   };
   uint32_t *emw = kExact ? nullptr : em;
   if (T.ok) {
-    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) {
                   return NbXPAV{sx[i], spi[i], sab[i], sv[i]};
                 },
@@ -1281,7 +1299,7 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
     const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
     const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
     const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
-    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
                   return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
@@ -1370,11 +1388,11 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   };
   // nm is the exact list of P1 over these positions
   if (T.ok) {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, stage_rowoff(T, bc, g),
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) { return NbXG{sx[i], sgr[i]}; }, pair);
   } else {
     const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P.cs, RowOff{},
+    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   return NbXG{ld2(xrr, (uint32_t)i * 8u), ld1(grr, (uint32_t)i * 4u)};
                 },

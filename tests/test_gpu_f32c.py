@@ -92,27 +92,64 @@ def test_f32c_rebuild_cell_list_on_random_clouds(be, case):
         np.testing.assert_array_equal(a, b, err_msg=name)
 
 
-def test_f32c_row_overflow_flag(be):
-    """An f32c mask row covers 128 slots; a cluster that puts more particles in
-    one cell row of a 3x3 block raises the overflow flag (as maxnb does for the
-    FP64 list) and the step stays finite."""
-    say("f32c row overflow")
+@pytest.mark.parametrize("case", ["cluster", "below_floor"])
+def test_f32c_long_row_runs_keep_every_pair(be, case):
+    """An f32c mask row covers 128 slots of a row run; the slots past them are
+    tested directly, so a run of several hundred slots leaves no pair out and
+    raises no overflow flag: the f32c step tracks an FP64 state whose list is
+    large enough to hold every neighbour (maxnb 512).  "cluster": 400
+    particles in about one cell; "below_floor": a block that has sunk a few
+    cells into the floor, the layers below it clamped into the margin row, as
+    the reference's penalty wall lets the 1M block do after ~2300 steps."""
+    say("f32c long row runs", case)
     rng = np.random.default_rng(3)
-    h = 0.01
-    cs = 2 * h * 1.2
-    N = 400                     # ~400 particles in about one cell
-    X = np.vstack([0.5 + cs * rng.uniform(0.05, 0.95, N), 0.3 + cs * rng.uniform(0.05, 0.95, N)])
-    p = S.SPHParams(h=h, m=1e-4, rho0=1.0, c=10.0, mu=0.01, dt=1e-6, Lx=1.0, Ly=0.6, kw=1e4,
-                    ngx=9, ngy=7, gx=0.0, gy=-9.81)
-    g = S.State(be, X, np.zeros_like(X), p, precision="f32c")
+    if case == "cluster":
+        h = 0.01
+        cs = 2 * h * 1.2
+        N = 400
+        X = np.vstack([0.5 + cs * rng.uniform(0.05, 0.95, N),
+                       0.3 + cs * rng.uniform(0.05, 0.95, N)])
+        p = dict(h=h, m=1e-4, rho0=1.0, c=10.0, mu=0.01, dt=1e-6, Lx=1.0, Ly=0.6, kw=1e4,
+                 ngx=9, ngy=7, gx=0.0, gy=-9.81)
+        nsteps, theta = 6, np.zeros((7, 9))
+    else:
+        dx = 0.01
+        p = scenes.wcsph_params(dx, 1.0, 0.6, 9, 7)
+        X = scenes.lattice(40, 40, dx, x0=0.3, y0=-0.25) + rng.uniform(-1e-4, 1e-4, (2, 1600))
+        nsteps, theta = 12, np.zeros((7, 9))
+    V = np.zeros_like(X)
+    pg = S.SPHParams(**p)
+    g = S.State(be, X, V, pg, precision="f32c")
+    o = S.State(be, X, V, pg, maxnb=512)
     S.build_neighbors_(g.nl)
-    assert g.nl.flags[0] == 1
-    with pytest.warns(UserWarning, match="128 particles"):
-        S.check_neighbor_flags(g.nl)
+    S.build_neighbors_(o.nl)
+    cnt = o.nl.counts
+    print(f"   neighbours per particle: max {cnt.max()}")
+    assert g.nl.flags.tolist() == [0, 0] and o.nl.flags[0] == 0
+    g = S.State(be, X, V, pg, precision="f32c")
+    o = S.State(be, X, V, pg, maxnb=512)
+    tg = S.Tape(be, g.N, nsteps, pg, precision="f32c", keep_pi=True, keep_lists=True)
+    to = S.Tape(be, o.N, nsteps, pg)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        S.simulate_(g, np.zeros((7, 9)), p, be, 2)
+        S.simulate_(g, theta, None, be, nsteps, tape=tg)
+        S.simulate_(o, theta, None, be, nsteps, tape=to)
+    ex = np.abs(g.X - o.X).max()
+    er = relerr(g.rho, o.rho)
+    ev = relerr(g.V, o.V)
+    print(f"   X abs {ex:.3e}  rho rel {er:.3e}  V rel {ev:.3e}")
     assert np.isfinite(g.X).all() and np.isfinite(g.V).all()
+    assert er < 1e-6 and ev < 1e-4 and ex < 1e-8
+    # the adjoint passes walk the same tails
+    grads = []
+    for st, tp, prec, mx in ((g, tg, "f32c", 64), (o, to, "f64", 512)):
+        J, seed = S.target_objective(st, float(X[0].mean()), float(X[1].mean()), 0.05)
+        ws = S.AdjointWorkspace(be, st.N, pg, precision=prec)
+        S.backward_(ws, tp, theta, pg, be, seedX=seed, seedV=np.zeros_like(seed), maxnb=mx)
+        grads.append((ws.gX, ws.gV))
+    e = [relerr(a, b) for a, b in zip(*grads)]
+    print("   gX rel %.3e  gV rel %.3e" % tuple(e))
+    assert max(e) < 1e-4
 
 
 def test_f32c_rejects_more_slots_than_its_offsets_address(be):
@@ -169,9 +206,11 @@ def test_f32c_adjoint_brinkman_200(be):
     eJ = abs(J32 - J64) / abs(J64)
     e = [relerr(gX32, gX64), relerr(gV32, gV64), relerr(gth32, gth64)]
     print("   J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
-    # measured on MI355X: J 1.1e-6, gX 7.6e-6, gV 8.7e-6, gtheta 2.4e-6 (FP32 velocities)
+    # measured on MI355X: J 8.1e-7, gX 5.5e-6, gV 5.9e-6, gtheta 2.4e-6 (FP32 velocities);
+    # dJ/dtheta is held to the FP32 form's accuracy bar, 1e-5
     assert eJ < 1e-5
-    assert max(e) < 1e-4
+    assert e[2] < 1e-5
+    assert max(e[:2]) < 2e-5
 
 
 def test_f32c_repeat_runs_bit_identical(be):
