@@ -268,7 +268,10 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
   if (fn(d, dx, dy, fmaf(dx, dx, dy * dy))) hit |= 1u << b;
 }
 
-template <class Ld, class Fn>
+// kPipe: issue the next neighbour's loads before evaluating the current pair
+// (for loaders from global memory); loaders from LDS walk one pair at a time,
+// the wavefronts sharing a SIMD hiding the LDS latency.
+template <bool kPipe = true, class Ld, class Fn>
 __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
                                             int bc, float2 xi, const Grid &g, int coff,
                                             uint32_t Ntot, const PhysF32 &P, const RowOff &ro,
@@ -291,6 +294,13 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
       if (m != 0) {
         const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
         const int wb = run.rb + 32 * w + ro.o[r];
+        if constexpr (!kPipe) {
+          do {
+            const int b = __builtin_ctz(m);
+            m &= m - 1;
+            nb_pair(ld(wb + b), b, s0, s1, xm, xi.x, xp, yi, hit, fn);
+          } while (m != 0);
+        } else {
         int b0 = __builtin_ctz(m);
         m &= m - 1;
         auto d0 = ld(wb + b0);
@@ -307,6 +317,7 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
           d0 = ld(wb + b0);
           nb_pair(d1, b1, s0, s1, xm, xi.x, xp, yi, hit, fn);
           if (!more0) break;
+        }
         }
       }
       if (em) em[widx] = hit;
@@ -348,7 +359,8 @@ constexpr int kStageSegs = 3 * kStageOwnRows;
 struct StageTab {
   int lo[kStageSegs];         // first slot of each segment
   int off[kStageSegs + 1];    // LDS position of each segment; off[kStageSegs] = total
-  int ya, ok;
+  int delta[kStageSegs];      // lo - off: slot = LDS position + delta of its segment
+  int ya, ok, nseg;           // nseg: segments in use (3 per own cell row)
 };
 
 // Plans the staging of the window of slots [w0, w1) (w1 > w0); every thread
@@ -380,11 +392,13 @@ __device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ 
     if (k < kStageSegs) {
       T.lo[k] = lo;
       T.off[k] = inc - len;
+      T.delta[k] = lo - (inc - len);
     }
     if (k == 0) {
       T.off[kStageSegs] = total;
       T.ya = a.y;
       T.ok = (nrow <= kStageOwnRows && total <= cap) ? 1 : 0;
+      T.nseg = 3 * (nrow < kStageOwnRows ? nrow : kStageOwnRows);
     }
   }
   __syncthreads();
@@ -394,10 +408,14 @@ __device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ 
 // index i.  Called by every thread of the workgroup when T.ok.  Each thread
 // issues the loads of kStageUnroll elements before storing any of them, so a
 // window's copy costs one load latency per kStageUnroll*blockDim elements.
+// The segment of an element is found among the T.nseg segments in use (the
+// count is the same in every lane, so the search is a scalar loop) and its
+// slot offset read from T.delta.
 constexpr int kStageUnroll = 4;
 template <class Ld, class St>
 __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) {
   const int total = T.off[kStageSegs];
+  const int nseg = T.nseg;
   int off[kStageSegs];
 #pragma unroll
   for (int q = 0; q < kStageSegs; ++q) off[q] = T.off[q];
@@ -407,11 +425,14 @@ __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) 
     for (int u = 0; u < kStageUnroll; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < total) {
-        // segment of i: the last one starting at or before i (off ascends)
+        // segment of i: the last one in use starting at or before i (off ascends)
         int k = 0;
 #pragma unroll
-        for (int q = 1; q < kStageSegs; ++q) k += (i >= off[q]) ? 1 : 0;
-        d[u] = ld(T.lo[k] + (i - off[k]));
+        for (int q = 1; q < kStageSegs; ++q) {
+          if (q >= nseg) break;
+          k += (i >= off[q]) ? 1 : 0;
+        }
+        d[u] = ld(i + T.delta[k]);
       }
     }
 #pragma unroll
@@ -465,7 +486,9 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   stage_plan(T, bcell, cst, w0, w1, g, S.coff, kStageCap);
-  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (T.ok)
+    stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
+               [&](int i, float2 v) { sx[i] = v; });
   if (sl >= S.g.N) return;
   const float rc2 = (float)(S.cs64 * S.cs64);
   const float2 xi = xr[s];
@@ -565,7 +588,7 @@ __device__ __forceinline__ float2 density_finish(const FSum &acc, const PhysF32 
   return make_float2(P.rho0 + drho, drho);
 }
 
-template <class Ld>
+template <bool kPipe, class Ld>
 __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, int s, int bc,
                                                float2 xi, const Grid &g, int coff, uint32_t Ntot,
                                                const PhysF32 &P, const RowOff &ro, Ld &&ld) {
@@ -573,7 +596,7 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
   // largest term of the sum: every later term is positive and no larger, so
   // each step's error term is exact
   FSum acc(1.f);
-  for_each_nb(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, ro, ld,
+  for_each_nb<kPipe>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       acc.add(density_term(r2, P));
@@ -590,9 +613,9 @@ __device__ __forceinline__ float2 density_slot(Rsrc nmr, uint32_t *em, Rsrc cst,
                                                float2 xi, const Grid &g, int coff, uint32_t Ntot,
                                                const PhysF32 &P) {
   if (T.ok)
-    return density_pair(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, stage_rowoff(T, bc, g),
+    return density_pair<false>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, stage_rowoff(T, bc, g),
                         [&](int i) __attribute__((always_inline)) { return NbX{sx[i]}; });
-  return density_pair(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, RowOff{},
+  return density_pair<true>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, RowOff{},
                       [&](int i) __attribute__((always_inline)) {
                         return NbX{ld2(xrr, (uint32_t)i * 8u)};
                       });
@@ -614,7 +637,9 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   SCENE_WIN(B, lane);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
-  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (T.ok)
+    stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
+               [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const int bc = bcell[s];
   float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
@@ -639,7 +664,9 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
-  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (T.ok)
+    stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
+               [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const float rc2 = (float)(S.cs64 * S.cs64);
   const float2 xi = xr[s];
@@ -712,8 +739,14 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   SCENE_WIN(B, lane);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   if (T.ok)
-    stage_copy(T, [&](int j) { return NbXPV{xr[j], PI[j], V[j]}; },
+    stage_copy(T,
+               [&](int j) {
+                 const uint32_t o8 = (uint32_t)j * 8u;   // one offset for the three arrays
+                 return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)};
+               },
                [&](int i, const NbXPV &d) {
                  sx[i] = d.x;
                  spi[i] = d.pi;
@@ -741,12 +774,10 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   };
   // nm is the exact list of the density pass over these positions
   if (T.ok) {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) { return NbXPV{sx[i], spi[i], sv[i]}; },
                 pair);
   } else {
-    const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
-    const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
     for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
@@ -922,7 +953,9 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   SCENE_WIN(B, lane);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
-  if (T.ok) stage_copy(T, [&](int j) { return xr[j]; }, [&](int i, float2 v) { sx[i] = v; });
+  if (T.ok)
+    stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
+               [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const double2 x64 = X[s];
   const int bc = bcell[s];
@@ -1241,8 +1274,15 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   SCENE_WIN_AT(B, lane, blk);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCapP1);
+  const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
+  const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
+  const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
   if (T.ok)
-    stage_copy(T, [&](int j) { return NbXPAV{xr[j], PI[j], abar[j], V[j]}; },
+    stage_copy(T,
+               [&](int j) {
+                 const uint32_t o8 = (uint32_t)j * 8u;   // one offset for the four arrays
+                 return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
+               },
                [&](int i, const NbXPAV &d) {
                  sx[i] = d.x;
                  spi[i] = d.pi;
@@ -1290,15 +1330,12 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   };
   uint32_t *emw = kExact ? nullptr : em;
   if (T.ok) {
-    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) {
                   return NbXPAV{sx[i], spi[i], sab[i], sv[i]};
                 },
                 pair);
   } else {
-    const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
-    const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
-    const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
     for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
@@ -1366,8 +1403,12 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   SCENE_WIN(B, lane);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
   if (T.ok)
-    stage_copy(T, [&](int j) { return NbXG{xr[j], grho[j]}; },
+    stage_copy(T,
+               [&](int j) {
+                 return NbXG{ld2(xrr, (uint32_t)j * 8u), ld1(grr, (uint32_t)j * 4u)};
+               },
                [&](int i, const NbXG &d) {
                  sx[i] = d.x;
                  sgr[i] = d.gr;
@@ -1388,10 +1429,9 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   };
   // nm is the exact list of P1 over these positions
   if (T.ok) {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) { return NbXG{sx[i], sgr[i]}; }, pair);
   } else {
-    const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
     for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   return NbXG{ld2(xrr, (uint32_t)i * 8u), ld1(grr, (uint32_t)i * 4u)};
