@@ -453,8 +453,9 @@ struct sph_state {
   // the next step's density pass builds it (set by the checkpointer, which
   // rebuilds before a snapshot and restores the rebuilt layout)
   bool pending_list = false;
-  // current layout; for the f32c form scell holds the binning cell of each slot
-  int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
+  // current layout; for the f32c form scell holds the binning cell of each
+  // slot, for the FP64 form bcell does (scell is the sweep cell there)
+  int *ids = nullptr, *scell = nullptr, *cstart = nullptr, *bcell = nullptr;
   float2 *V32 = nullptr, *V32n = nullptr, *xr = nullptr, *xrn = nullptr, *PI32 = nullptr;
   // neighbour list of the current layout: FP64 form transposed (entry m of
   // slot s at nl[m*N + s]); f32c form row bitmasks (nm)
@@ -495,6 +496,7 @@ struct TapeEpoch {
   const sph_state *owner = nullptr;
   int64_t first = 0;  // first tape entry of the epoch
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
+  int *bcell = nullptr;  // FP64 form: binning cells (the f32c form's scell is its binning cell)
 };
 
 struct sph_tape {
@@ -539,9 +541,13 @@ struct sph_workspace {
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
   int *lane = nullptr;                                     // f32c form: pair-pass thread order
   int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
-  int *tfar = nullptr;        // f32c form: stamp of the last step in which a slot lay
-                              // beyond its key's tight SPH range
-  int tstamp = 0;             // f32c form: stamp of the current step (1, 2, ...)
+  unsigned long long *tlev = nullptr;  // gtheta sweep level of the reverse steps (raise_level)
+  int tstamp = 0;             // stamp of the current reverse step (1, 2, ...)
+  // FP64 form: the one-cell gtheta sweep ranges (theta columns, then rows) of
+  // the cell grid they were made for
+  int2 *trange64 = nullptr;
+  int tr_nx = -1, tr_ny = -1;
+  double tr_cs = 0.0;
   int64_t nl_cap = 0;                                      // maxnb the list was sized for
   int64_t ntnode = 0, ntcell = 0;  // design-field nodes and cells (all scenes)
   int *ids_cur = nullptr;  // slot order of the adjoint state between reverse sweeps
@@ -558,7 +564,8 @@ namespace {
 void state_free(sph_state *st) {
   int **ints[] = {&st->ids, &st->scell, &st->cstart, &st->cellof, &st->counts, &st->cl_cstart,
                   &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
-                  &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt, &st->lane};
+                  &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt, &st->lane,
+                  &st->bcell};
   for (auto pp : ints) dfree(*pp);
   double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
                     &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
@@ -638,8 +645,8 @@ void do_build_neighbors(sph_state *st, bool defer_list = false) {
     std::swap(st->V32, st->V32n);
   } else {
     timed(st->ctx, K_PERMUTE, [&] {
-      launch_permute(s, st->perm, st->X, st->V, st->Xref, check, st->g, st->kc, st->Xn, st->Vn,
-                     st->Xref2, st->scell, st->flags);
+      launch_permute(s, st->perm, st->cellof, st->X, st->V, st->Xref, check, st->g, st->kc,
+                     st->Xn, st->Vn, st->Xref2, st->scell, st->bcell, st->flags);
     });
     std::swap(st->V, st->Vn);
   }
@@ -688,6 +695,7 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
     e.ids = dalloc<int>(tp->N);
     e.scell = dalloc<int>(tp->N);
     e.cstart = dalloc<int>(tp->ncell + 1);
+    if (st->bcell) e.bcell = dalloc<int>(tp->N);
   }
   e.owner = st;
   e.serial = st->epoch;
@@ -695,6 +703,10 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
   hipStream_t s = st->ctx->stream;
   HIPC(hipMemcpyAsync(e.ids, st->ids, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
   HIPC(hipMemcpyAsync(e.scell, st->scell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
+  if (st->bcell) {
+    if (!e.bcell) e.bcell = dalloc<int>(tp->N);
+    HIPC(hipMemcpyAsync(e.bcell, st->bcell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
+  }
   HIPC(hipMemcpyAsync(e.cstart, st->cstart, sizeof(int) * (tp->ncell + 1), hipMemcpyDeviceToDevice, s));
   tp->epochs.push_back(e);
   return tp->epochs.back();
@@ -1224,6 +1236,7 @@ void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const in
       st->scell = dalloc<int>(n);
       st->cstart = dalloc<int>(nc + 1);
       const bool f32c = precision == SPH_F32C;
+      if (!f32c) st->bcell = dalloc<int>(n);
       st->X = dalloc<double2>(n);
       st->Xn = dalloc<double2>(n);
       st->Xref = dalloc<double2>(n);
@@ -1646,6 +1659,7 @@ int sph_tape_destroy(sph_tape *tp) {
       for (auto &e : *v) {
         dfree(e.ids);
         dfree(e.scell);
+        dfree(e.bcell);
         dfree(e.cstart);
       }
     dfree(tp->X);
@@ -1727,7 +1741,8 @@ void ws_free(sph_workspace *ws) {
   dfree(ws->em);
   dfree(ws->lane);
   dfree(ws->tkey);
-  dfree(ws->tfar);
+  dfree(ws->tlev);
+  dfree(ws->trange64);
 }
 
 }  // extern "C"
@@ -1891,15 +1906,15 @@ int sph_workspace_get(sph_workspace *ws, int which, double *host) {
 namespace {
 
 void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
+  if (!ws->tlev) {
+    ws->tlev = dalloc<unsigned long long>(1);
+    HIPC(hipMemsetAsync(ws->tlev, 0, sizeof(unsigned long long), ws->ctx->stream));
+  }
   if (ws->precision == SPH_F32C) {
     if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->lane) ws->lane = dalloc<int>((size_t)ws->N);
     if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);
-    if (!ws->tfar) {
-      ws->tfar = dalloc<int>(1);
-      HIPC(hipMemsetAsync(ws->tfar, 0, sizeof(int), ws->ctx->stream));
-    }
     return;
   }
   if (ws->nl_cap < maxnb) {
@@ -1925,6 +1940,19 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const Batch &B = tape->bs.B;   // f32c form: untuned launches
   const BatchStore &BS = tape->bs;
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
+  if (!f32c && (ws->tr_nx != g.nx || ws->tr_ny != g.ny || ws->tr_cs != T.cs)) {
+    // the FP64 R passes compare each slot's binning cell with these ranges
+    std::vector<int2> tr;
+    for (int i = 0; i < T.ngx - 1; ++i) tr.push_back(theta_cell_range(i, T.ngx - 1, T.hx, T.cs, g.nx));
+    for (int j = 0; j < T.ngy - 1; ++j) tr.push_back(theta_cell_range(j, T.ngy - 1, T.hy, T.cs, g.ny));
+    dfree(ws->trange64);
+    ws->trange64 = dalloc<int2>(tr.size());
+    HIPC(hipMemcpyAsync(ws->trange64, tr.data(), sizeof(int2) * tr.size(), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+    ws->tr_nx = g.nx;
+    ws->tr_ny = g.ny;
+    ws->tr_cs = T.cs;
+  }
   int cur = tape->entry_epoch[tape->n];
   // gtheta: the theta-cell partials of an epoch's steps add up in tpart; its
   // node sums go to gtheta when the sweep leaves the epoch
@@ -2018,17 +2046,17 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
           if (tape->keep_pi) {
             PIk = tape->PI32 + off;
             launch_adj_rpw_f32c(s, Xk, E.scell, Vnext, theta->d, BS.at(TK_ADJ_R), ws->xr32,
-                                ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tfar,
+                                ws->gX32, ws->gV32, ws->abar32, ws->galpha32, ws->tkey, ws->tlev,
                                 ws->tstamp);
           } else {
             launch_xrel_f32c(s, Xk, E.scell, B, ws->xr32);
             launch_adj_r_f32c(s, Xk, ws->xr32, Vnext, theta->d, E.scell, nm, E.cstart, lane,
                               BS.at(TK_ADJ_R), ws->PI32, ws->gX32, ws->gV32, ws->abar32,
-                              ws->galpha32, ws->em, ws->tkey, ws->tfar, ws->tstamp);
+                              ws->galpha32, ws->em, ws->tkey, ws->tlev, ws->tstamp);
           }
         });
       }
-      const ThetaKeyed TA{ws->tkey, Xk, ws->galpha32, E.cstart, ws->tfar, ws->tstamp,
+      const ThetaKeyed TA{ws->tkey, Xk, ws->galpha32, E.cstart, ws->tlev, ws->tstamp,
                           tfirst ? 1 : 0, ws->tpart};
       // the exact list of these positions: the forward's density pass kept it
       // on the tape, or the density recompute just wrote it; with stored
@@ -2061,7 +2089,7 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       if (fuse) {
         r_stamp = ws->tstamp % 2000000000 + 1;
         prev = RpwArgs{Xk - n, Vk, theta->d, ws->xr32b, ws->gV32, ws->abar32,
-                       ws->galpha32, ws->tkey, ws->tfar, r_stamp};
+                       ws->galpha32, ws->tkey, ws->tlev, r_stamp};
       }
       timed(c, K_ADJ_P2, [&] {
         launch_adj_p2_f32c(s, ws->xr32, ws->grho32, ws->gXs32, E.scell, exact, E.cstart, lane,
@@ -2085,17 +2113,21 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         cur = e;
       }
       const double2 *PIk = ws->PI;
+      ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
+      const LevelArgs LA{E.bcell, ws->trange64, ws->tlev, ws->tstamp, T.ngx - 1, T.ngy};
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
           PIk = tape->PI + off;
-          launch_adj_rpw_f64(s, Xk, Vnext, theta->d, g, P, ws->gX, ws->gV, ws->abar, ws->galpha);
+          launch_adj_rpw_f64(s, Xk, Vnext, theta->d, g, P, ws->gX, ws->gV, ws->abar, ws->galpha,
+                             LA);
         } else {
           launch_adj_r_f64(s, Xk, Vnext, theta->d, nl, nbcnt, g, P, ws->PI, ws->gX, ws->gV,
-                           ws->abar, ws->galpha);
+                           ws->abar, ws->galpha, LA);
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_cells_f64(s, Xk, ws->galpha, E.cstart, g, T, tfirst, ws->tpart);
+        launch_theta_cells_f64(s, Xk, ws->galpha, E.cstart, g, T, tfirst, ws->tpart, ws->tlev,
+                               ws->tstamp);
       });
       timed(c, K_ADJ_P1, [&] {
         launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, ws->gXs,

@@ -156,14 +156,16 @@ __global__ void k_rank(const int *__restrict__ cellof, const int *__restrict__ c
   new_ids[t] = id;
 }
 
-__global__ void k_permute(const int *__restrict__ perm, const double2 *__restrict__ X,
-                          const double2 *__restrict__ V, const double2 *__restrict__ Xref,
-                          int check, Grid g, CellConstF64 k, double2 *__restrict__ Xn,
-                          double2 *__restrict__ Vn, double2 *__restrict__ Xrefn,
-                          int *__restrict__ scell, int *__restrict__ flags) {
+__global__ void k_permute(const int *__restrict__ perm, const int *__restrict__ cellof,
+                          const double2 *__restrict__ X, const double2 *__restrict__ V,
+                          const double2 *__restrict__ Xref, int check, Grid g, CellConstF64 k,
+                          double2 *__restrict__ Xn, double2 *__restrict__ Vn,
+                          double2 *__restrict__ Xrefn, int *__restrict__ scell,
+                          int *__restrict__ bcell, int *__restrict__ flags) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int o = perm[t];
+  bcell[t] = cellof[o];
   double2 x = X[o];
   if (check) {
     double2 xr = Xref[o];
@@ -264,12 +266,13 @@ void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int 
                  const int *tmp_slot, int N, int *perm, int *new_ids) {
   if (N > 0) k_rank<<<nblocks(N), kBlock, 0, s>>>(cellof, cstart, tmp_id, tmp_slot, N, perm, new_ids);
 }
-void launch_permute(hipStream_t s, const int *perm, const double2 *X, const double2 *V,
-                    const double2 *Xref, int check, Grid g, CellConstF64 k, double2 *Xn,
-                    double2 *Vn, double2 *Xrefn, int *scell, int *flags) {
+void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
+                    const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,
+                    double2 *Xn, double2 *Vn, double2 *Xrefn, int *scell, int *bcell,
+                    int *flags) {
   if (g.N > 0)
-    k_permute<<<nblocks(g.N), kBlock, 0, s>>>(perm, X, V, Xref, check, g, k, Xn, Vn, Xrefn, scell,
-                                              flags);
+    k_permute<<<nblocks(g.N), kBlock, 0, s>>>(perm, cellof, X, V, Xref, check, g, k, Xn, Vn,
+                                              Xrefn, scell, bcell, flags);
 }
 void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, const int *scell,
                         const int *cstart, Grid g, CellConstF64 k, int maxnb, long long sm,

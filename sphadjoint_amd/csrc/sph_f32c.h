@@ -99,7 +99,8 @@ struct RpwArgs {
   float2 *xr;            // out: relative coordinates of the step
   float2 *gV, *abar;
   float *galpha;
-  int *tkey, *tfar;
+  int *tkey;
+  unsigned long long *tlev;   // the step's gtheta sweep level (raise_level)
   int tstamp;
 };
 
@@ -148,17 +149,19 @@ void launch_snapshot_f32c(hipStream_t s, bool to_snap, const Batch &B, double2 *
                           float2 *xr, double2 *qX, float2 *qV, int *qids, int *qscell,
                           int *qlane, int *qcstart);
 // R passes also write each slot's interp-cell key for the keyed gtheta gather
-// (both R passes also store tstamp in *tfar when a slot lies beyond its interp
-// cell's tight SPH range, Batch::trange_tight; tstamp differs from step to step)
+// and raise the step's sweep level in *tlev (raise_level, with stamp tstamp,
+// which differs from step to step) for a slot beyond its interp cell's tight
+// SPH range (Batch::trange_tight; level 1) or beyond the one-cell range
+// (Batch::trange; the margin that reaches it)
 void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *Vnext,
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, int *tfar, int tstamp);
+                       int *tkey, unsigned long long *tlev, int tstamp);
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha, int *tkey, int *tfar,
-                         int tstamp);
+                         float2 *gV, float2 *abar, float *galpha, int *tkey,
+                         unsigned long long *tlev, int tstamp);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
@@ -173,15 +176,16 @@ void launch_adj_p2_f32c(hipStream_t s, const float2 *xr, const float *grho, cons
                         const Batch &B, float2 *gX, const RpwArgs *prev = nullptr);
 // deterministic gtheta gather over the keys of the R passes: the theta-cell
 // partials of one step, written (first != 0) or added to part; the gather
-// sweeps the tight SPH ranges unless this step's R pass stored tstamp in
-// *tfar.  launch_theta_nodes_keyed_f32 (sph_theta.hip) adds the node sums of
-// part to gtheta (once per neighbour-list epoch).
+// sweeps the SPH ranges of the step's level (*tlev at stamp tstamp: 0 the
+// tight ranges, 1 the one-cell ranges, m >= 2 an m-cell margin).
+// launch_theta_nodes_keyed_f32 (sph_theta.hip) adds the node sums of part to
+// gtheta (once per neighbour-list epoch).
 struct ThetaKeyed {
   const int *key;
   const double2 *X;
   const float *galpha;
   const int *cstart;
-  const int *tfar;
+  const unsigned long long *tlev;
   int tstamp, first;
   double4 *part;
 };

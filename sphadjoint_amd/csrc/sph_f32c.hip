@@ -926,17 +926,24 @@ k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
   if (c >= 0 && head_lane == lane) atomicAdd(&counts[c], run_len);
 }
 
-// stores this step's stamp in *tfar when slot s (binning cell bc) lies beyond
-// the tight SPH range of its interp cell ic, so that the step's gtheta gather
-// sweeps the wider ranges (a stamp of an earlier step selects nothing)
-__device__ __forceinline__ void check_tight(const InterpF &ic, int bc, const Grid &g,
+// Raises this step's gtheta sweep level when slot s (binning cell bc) lies
+// beyond the tight SPH range of its interp cell ic (level 1: the one-cell
+// ranges), or beyond the one-cell range (the margin that reaches it), so that
+// the step's gather visits every slot (a level of an earlier step's stamp
+// selects nothing).
+__device__ __forceinline__ void check_level(const InterpF &ic, int bc, const Grid &g,
                                             const PhysF32 &P, const Scene32 &S, const Batch &B,
-                                            int *tfar, int tstamp) {
+                                            unsigned long long *tlev, int tstamp) {
   const int i0 = ic.l00 / P.ngy, j0 = ic.l00 - i0 * P.ngy;
   const int2 c = cell_xy(bc, g);
+  const int rowoff = S.troff + (P.ngx - 1);
   const int2 rx = B.trange_tight[S.troff + i0];
-  const int2 ry = B.trange_tight[S.troff + (P.ngx - 1) + j0];
-  if (c.x < rx.x || c.x > rx.y || c.y < ry.x || c.y > ry.y) *tfar = tstamp;
+  const int2 ry = B.trange_tight[rowoff + j0];
+  if (c.x < rx.x || c.x > rx.y || c.y < ry.x || c.y > ry.y) {
+    const int m = max(margin_beyond(c.x, B.trange[S.troff + i0]),
+                      margin_beyond(c.y, B.trange[rowoff + j0]));
+    raise_level(tlev, tstamp, m > 1 ? m : 1);
+  }
 }
 
 // adjoint R: density recompute, _adj_integrate!, _adj_drag! (src/adjoint.jl:193-228)
@@ -947,7 +954,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
           float2 *__restrict__ PI, float2 *__restrict__ gX, float2 *__restrict__ gV,
           float2 *__restrict__ abar, float *__restrict__ galpha, uint32_t *__restrict__ em,
-          int *__restrict__ tkey, int *__restrict__ tfar, int tstamp) {
+          int *__restrict__ tkey, unsigned long long *__restrict__ tlev, int tstamp) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
@@ -963,7 +970,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
-  check_tight(ic, bcell[s], g, P, S, B, tfar, tstamp);
+  check_level(ic, bcell[s], g, P, S, B, tlev, tstamp);
   float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   float gvx = gv.x + P.dt * gx.x;
@@ -995,7 +1002,7 @@ __device__ __forceinline__ void rpw_finish(const RpwArgs &R, const RpwIn &in, in
   R.xr[s] = rel_coord(x64, bc, g, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   R.tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
-  check_tight(ic, bc, g, P, S, B, R.tfar, R.tstamp);
+  check_level(ic, bc, g, P, S, B, R.tlev, R.tstamp);
   float al = interp_alpha(R.theta + S.toff, ic, P, nullptr, nullptr);
   const float2 gv = in.gv, vp = in.vp;
   float gvx = gv.x + P.dt * gx.x;
@@ -1085,14 +1092,20 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
   const int i0 = ifirst + (int)threadIdx.x / kThetaLanes;
   // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
   // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
-  // found a slot further out this step, else one cell (theta_cell_range)
-  const int2 *tr = (*A.tfar == tstamp ? B.trange : B.trange_tight) + Sc.troff;
+  // found a slot further out this step (level 1: one cell, theta_cell_range;
+  // level m >= 2: the one-cell range widened by m - 1 cells on each side)
+  const int lv = level_of(A.tlev, tstamp);
+  const int2 *tr = (lv >= 1 ? B.trange : B.trange_tight) + Sc.troff;
+  const int wid = lv > 1 ? lv - 1 : 0;
+  auto widen = [&](int2 r, int n) {
+    return make_int2(max(r.x - wid, 0), min(r.y + wid, n - 1));
+  };
   auto xrange = [&](int i, int &cx0, int &cx1) {
-    const int2 r = tr[i];
+    const int2 r = widen(tr[i], g.nx);
     cx0 = r.x;
     cx1 = r.y;
   };
-  const int2 ry = tr[ncx + j0];
+  const int2 ry = widen(tr[ncx + j0], g.ny);
   const int cy0 = ry.x, cy1 = ry.y;
   const int nrow = cy1 - cy0 + 1;
   // plan: row k of the strip stages slots [lo_k, lo_k + len_k) at soff[k]
@@ -1503,16 +1516,17 @@ void launch_adj_r_f32c(hipStream_t s, const double2 *X, const float2 *xr, const 
                        const double *theta, const int *bcell, const uint32_t *nm,
                        const int *cstart, const int *lane, const Batch &B, float2 *PI,
                        float2 *gX, float2 *gV, float2 *abar, float *galpha, uint32_t *em,
-                       int *tkey, int *tfar, int tstamp) {
+                       int *tkey, unsigned long long *tlev, int tstamp) {
   if (B.N > 0)
     k_adj_r32<<<B.nblk, B.bs, 0, s>>>(X, xr, Vnext, theta, bcell, nm, cstart, lane, B, PI, gX,
-                                        gV, abar, galpha, em, tkey, tfar, tstamp);
+                                        gV, abar, galpha, em, tkey, tlev, tstamp);
 }
 void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, const float2 *Vnext,
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
-                         float2 *gV, float2 *abar, float *galpha, int *tkey, int *tfar,
+                         float2 *gV, float2 *abar, float *galpha, int *tkey,
+                         unsigned long long *tlev,
                          int tstamp) {
-  const RpwArgs R{X, Vnext, theta, xr, gV, abar, galpha, tkey, tfar, tstamp};
+  const RpwArgs R{X, Vnext, theta, xr, gV, abar, galpha, tkey, tlev, tstamp};
   if (B.N > 0) k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(R, bcell, gX, B);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,

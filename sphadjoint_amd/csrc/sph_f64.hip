@@ -196,6 +196,18 @@ k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
   if (tapeV) tapeV[s] = vi;
 }
 
+// The sweep level of the step's gtheta gather (k_theta_cells, a one-cell
+// margin unless raised) must reach slot s, whose binning cell at the epoch's
+// rebuild is L.bcell[s] and whose interp cell is l00: raise it when a larger
+// margin is needed (a slot that moved more than a cell since the rebuild).
+__device__ __forceinline__ void sweep_level(const LevelArgs &L, const Grid &g, int s, int l00) {
+  const int i0 = l00 / L.ngy, j0 = l00 - i0 * L.ngy;
+  const int c = L.bcell[s];
+  const int cy = c / g.nx, cx = c - cy * g.nx;
+  const int m = max(margin_beyond(cx, L.trange[i0]), margin_beyond(cy, L.trange[L.ncx + j0]));
+  if (m >= 2) raise_level(L.lev, L.stamp, m);
+}
+
 // Adjoint pass R: density recompute, _adj_integrate! and _adj_drag!
 // (src/adjoint.jl:193-228).  v' is the taped V_{n+1}, which is bit-identical
 // to the (V + dt a) D that the reference recomputes.
@@ -204,7 +216,7 @@ k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
         const double *__restrict__ theta, const int *__restrict__ nl,
         const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ PI,
         double2 *__restrict__ gX, double2 *__restrict__ gV, double2 *__restrict__ abar,
-        double *__restrict__ galpha) {
+        double *__restrict__ galpha, LevelArgs L) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s];
@@ -212,6 +224,7 @@ k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
   double inv = 1.0 / rho;
   PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
   Interp ic = interp_cell(xi.x, xi.y, P);
+  sweep_level(L, g, s, ic.l00);
   double al = interp_alpha(theta, ic, P, nullptr, nullptr);
   double2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   double gvx = gv.x + P.dt * gx.x;
@@ -227,11 +240,13 @@ k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
 __global__ void __launch_bounds__(kBlock)
 k_adj_rpw(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
           const double *__restrict__ theta, Grid g, PhysF64 P, double2 *__restrict__ gX,
-          double2 *__restrict__ gV, double2 *__restrict__ abar, double *__restrict__ galpha) {
+          double2 *__restrict__ gV, double2 *__restrict__ abar, double *__restrict__ galpha,
+          LevelArgs L) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= g.N) return;
   double2 xi = X[s];
   Interp ic = interp_cell(xi.x, xi.y, P);
+  sweep_level(L, g, s, ic.l00);
   double al = interp_alpha(theta, ic, P, nullptr, nullptr);
   double2 gx = gX[s], gv = gV[s], vp = Vnext[s];
   double gvx = gv.x + P.dt * gx.x;
@@ -402,16 +417,17 @@ void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const d
 }
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
-                      double2 *gX, double2 *gV, double2 *abar, double *galpha) {
+                      double2 *gX, double2 *gV, double2 *abar, double *galpha,
+                      const LevelArgs &L) {
   if (g.N > 0)
     k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV, abar,
-                                            galpha);
+                                            galpha, L);
 }
 void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
                         const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
-                        double2 *abar, double *galpha) {
+                        double2 *abar, double *galpha, const LevelArgs &L) {
   if (g.N > 0)
-    k_adj_rpw<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, g, P, gX, gV, abar, galpha);
+    k_adj_rpw<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, g, P, gX, gV, abar, galpha, L);
 }
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,

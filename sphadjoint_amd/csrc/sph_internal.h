@@ -94,6 +94,21 @@ __device__ __forceinline__ int cell_coord_i32(double v, int n) {
   return (int)(c - 1);
 }
 
+// Sweep level of one reverse step's gtheta gather: 0 tight ranges (a quarter
+// cell, f32c form), 1 a one-cell margin, m >= 2 an m-cell margin (the SPH
+// cells holding a theta cell's positions, widened by m cells on each side:
+// a slot that moved m cells since the rebuild is still swept).  The R pass
+// of the step raises it with an atomic max of (stamp << 32 | level) for every
+// slot its level would miss (the maximum does not depend on the order of the
+// updates); the gather reads the level of its own stamp, 0 for any other.
+__device__ __forceinline__ void raise_level(unsigned long long *lev, int stamp, int level) {
+  atomicMax(lev, ((unsigned long long)(unsigned)stamp << 32) | (unsigned)level);
+}
+__device__ __forceinline__ int level_of(const unsigned long long *lev, int stamp) {
+  const unsigned long long v = *lev;
+  return (unsigned)(v >> 32) == (unsigned)stamp ? (int)(unsigned)v : 0;
+}
+
 // The three contiguous slot runs of a 3x3 sweep around cell `sc`, in the
 // reference's order (ddy outer, ddx inner; src/neighbors.jl:220-231).
 struct Runs {
@@ -148,9 +163,11 @@ void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *c
                  int *tmp_id, int *tmp_slot);
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
                  const int *tmp_slot, int N, int *perm, int *new_ids);
-void launch_permute(hipStream_t s, const int *perm, const double2 *X, const double2 *V,
-                    const double2 *Xref, int check, Grid g, CellConstF64 k, double2 *Xn,
-                    double2 *Vn, double2 *Xrefn, int *scell, int *flags);
+// also writes bcell[slot] = the slot's binning cell (cellof, the division rule)
+void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
+                    const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,
+                    double2 *Xn, double2 *Vn, double2 *Xrefn, int *scell, int *bcell,
+                    int *flags);
 void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, const int *scell,
                         const int *cstart, Grid g, CellConstF64 k, int maxnb, long long sm,
                         long long si, int *indices, int *nbcount, int *flags);
@@ -172,12 +189,27 @@ void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const d
                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF64 P,
                       double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
                       double *alpha_out);
+// what an R pass needs to raise its step's gtheta sweep level (raise_level):
+// each slot's binning cell at the epoch's rebuild, the level word and the
+// step's stamp, and the theta grid the gather sweeps for
+struct LevelArgs {
+  const int *bcell;
+  const int2 *trange;   // one-cell-margin ranges: ncx theta columns, then the theta rows
+  unsigned long long *lev;
+  int stamp, ncx, ngy;
+};
+// the margin (>= 2) a sweep needs to reach binning cell c when c lies beyond
+// its one-cell-margin range w (theta_cell_range, which clamps to the axis), else 0
+__host__ __device__ inline int margin_beyond(int c, int2 w) {
+  return c < w.x ? w.x + 1 - c : (c > w.y ? c - w.y + 1 : 0);
+}
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
-                      double2 *gX, double2 *gV, double2 *abar, double *galpha);
+                      double2 *gX, double2 *gV, double2 *abar, double *galpha,
+                      const LevelArgs &L);
 void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
                         const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
-                        double2 *abar, double *galpha);
+                        double2 *abar, double *galpha, const LevelArgs &L);
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
                        const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
@@ -192,8 +224,10 @@ int objective_blocks(int N);
 // deterministic gtheta gather (sph_theta.hip): the theta-cell partials of one
 // step, written (first step of a neighbour-list epoch) or added to part; the
 // node sums of part are added to gtheta once per epoch (launch_theta_nodes_f64)
+// (the step's sweep level, raised by its R pass: the margin is max(1, level) cells)
 void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
-                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part);
+                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part,
+                            const unsigned long long *lev, int stamp);
 void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta);
 
 // design-field chain of the topology optimiser (sph_design.hip): an

@@ -38,7 +38,7 @@ template <class GA>
 __global__ void __launch_bounds__(kBlock)
 k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
               const int *__restrict__ cstart, Grid g, ThetaGrid T, int first,
-              double4 *__restrict__ part) {
+              double4 *__restrict__ part, const unsigned long long *__restrict__ lev, int stamp) {
   const int ncx = T.ngx - 1, ncy = T.ngy - 1;
   const int lane = threadIdx.x & (kLanes - 1);
   // theta cells in x-fastest order, so that neighbouring groups read
@@ -47,11 +47,15 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
   if (t >= ncx * ncy) return;      // the kLanes lanes of a group leave together
   const int i0 = t % ncx;
   const int j0 = t / ncx;
-  // SPH cell range holding particles with interp cell (i0, j0), +1 margin
-  int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - 1;
-  int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + 1;
-  int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - 1;
-  int cy1 = (j0 == ncy - 1) ? g.ny - 1 : cell_of((double)(j0 + 1) * T.hy, T.cs, g.ny) + 1;
+  // SPH cell range holding particles with interp cell (i0, j0), widened by a
+  // one-cell margin, or by the step's larger sweep level (raised by its R pass
+  // for a slot that moved further since the rebuild)
+  const int lv = level_of(lev, stamp);
+  const int mg = lv > 1 ? lv : 1;
+  int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - mg;
+  int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + mg;
+  int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - mg;
+  int cy1 = (j0 == ncy - 1) ? g.ny - 1 : cell_of((double)(j0 + 1) * T.hy, T.cs, g.ny) + mg;
   cx0 = cx0 < 0 ? 0 : cx0;
   cy0 = cy0 < 0 ? 0 : cy0;
   cx1 = cx1 > g.nx - 1 ? g.nx - 1 : cx1;
@@ -143,10 +147,12 @@ __global__ void k_theta_nodes(const double4 *__restrict__ part, ThetaGrid T,
 }  // namespace
 
 void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
-                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part) {
+                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part,
+                            const unsigned long long *lev, int stamp) {
   const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
   k_theta_cells<double><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T,
-                                                                    first ? 1 : 0, part);
+                                                                    first ? 1 : 0, part, lev,
+                                                                    stamp);
 }
 
 void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta) {

@@ -325,3 +325,54 @@ def test_gtheta_bit_identical_over_runs(be, precision):
         outs.append((ws.gtheta, ws.gX, ws.gV))
     for a, b in zip(outs[0], outs[1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_gtheta_reaches_particles_several_cells_from_their_binning_cell(be, precision):
+    """The gtheta gather sweeps, for each theta cell, the SPH cells around it
+    with a margin for the displacement since the last rebuild.  Isolated fast
+    particles (no neighbours, so the reverse rebuild schedule does not matter)
+    moving 2-3 cells per 20-step epoch need a wider margin than one cell: the
+    R pass raises the step's sweep level and every particle's design term is
+    still gathered.  gtheta against the oracle's atomic scatter (which never
+    misses one): 1e-12 in FP64, the FP32 form's 1e-5."""
+    say("gtheta with particles several cells from their binning cell", precision)
+    rng = np.random.default_rng(17)
+    p = dict(h=0.02, m=4e-4, rho0=1.0, c=10.0, mu=0.02, dt=1e-3, Lx=1.0, Ly=0.6, kw=1e4,
+             ngx=9, ngy=7, gx=0.0, gy=-9.81)
+    # a drifting lattice: every particle moves at (5, 3) m/s, so the spacing
+    # (0.12) stays above the cutoff 2h(1 + skin) = 0.048 and no pair ever forms
+    xs, ys = np.meshgrid(0.2 + 0.12 * np.arange(5), 0.15 + 0.12 * np.arange(3))
+    X0 = np.vstack([xs.ravel(), ys.ravel()]) + rng.uniform(-0.01, 0.01, (2, xs.size))
+    V0 = np.vstack([np.full(xs.size, 5.0), np.full(xs.size, 3.0)])
+    theta = 5.0 * rng.uniform(size=(7, 9))
+    n, interval = 40, 20
+    sc = scenes.Scene("fast_isolated", p, X0, V0, theta, n, interval=interval)
+    pg, po = S.SPHParams(**p), O.SPHParams(**p)
+    g = S.State(be, X0, V0, pg, interval=interval, precision=precision)
+    tg = S.Tape(be, sc.N, n, pg, precision=precision)
+    o = O.State(X0, V0, po, interval=interval, layout="slot")
+    to = O.Tape(sc.N, n, po)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        S.simulate_(g, theta, pg, be, n, tape=tg)
+        O.simulate_(o, theta, po, n, to)
+    moved = np.abs(g.X - X0).max() / (2 * p["h"] * 1.2)
+    print(f"   displacement over the horizon: {moved:.2f} cells; X vs oracle "
+          f"{np.abs(g.X - o.X).max():.2e}")
+    assert (o.neighbors()[1] == 0).all()          # no pair in the last list either
+    J, seed = S.target_objective(g, 0.5, 0.3, 0.2)
+    Jo, so = O.target_objective(o.X, 0.5, 0.3, 0.2)
+    wg = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
+    wo = O.AdjointWorkspace(sc.N, po)
+    z = np.zeros_like(seed)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        S.backward_(wg, tg, theta, pg, be, seedX=seed, seedV=z, interval=interval)
+        O.backward_(wo, to, theta, po, seedX=so, seedV=z, interval=interval, layout="slot")
+    e = float(np.abs(wg.gtheta - wo.gtheta).max() / np.abs(wo.gtheta).max())
+    print(f"   gtheta max-norm rel err {e:.3e}; nodes carrying gradient "
+          f"{int((np.abs(wo.gtheta) > 0).sum())}")
+    assert moved > 4
+    assert e < (1e-12 if precision == "f64" else 1e-5)
