@@ -119,6 +119,11 @@ int sph_build_neighbors(sph_state *st);
 int sph_neighbors_get(sph_state *st, int32_t *indices, int32_t *counts);
 /* nl.flags: [0] row overflow, [1] displacement beyond skin*h (src/neighbors.jl:282-290) */
 int sph_flags_get(sph_state *st, int32_t *flags2);
+/* the same and [2]: a position that was not finite at a rebuild (NaN or
+ * infinite coordinates bin into a clamped edge cell, as the reference's
+ * unsafe_trunc does, src/neighbors.jl:38-42); no counterpart in the reference */
+int sph_flags_get_all(sph_state *st, int32_t *flags3);
+/* resets all three flags */
 int sph_flags_clear(sph_state *st);
 
 /* step!(st, theta, p, backend) (src/forward.jl:161-173) */

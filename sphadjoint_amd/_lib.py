@@ -54,6 +54,7 @@ SIGNATURES = {
     "sph_build_neighbors": (C.c_int, [_vp]),
     "sph_neighbors_get": (C.c_int, [_vp, _vp, _vp]),
     "sph_flags_get": (C.c_int, [_vp, _vp]),
+    "sph_flags_get_all": (C.c_int, [_vp, _vp]),
     "sph_flags_clear": (C.c_int, [_vp]),
     "sph_step": (C.c_int, [_vp, _vp]),
     "sph_simulate": (C.c_int, [_vp, _vp, _i64, _vp]),

@@ -330,12 +330,17 @@ class State:
 def check_neighbor_flags(nl: NeighborList):
     """check_neighbor_flags (src/neighbors.jl:282-290): warn, then reset."""
     f = nl.flags
+    f3 = np.zeros(3, np.int32)
+    L.call("sph_flags_get_all", nl._st._h, _ptr(f3))
     if f[0] != 0:
         # (the f32c form never sets it: its passes test the slots past the masks)
         warnings.warn(f"neighbour list row overflow (maxnb={nl.maxnb}); increase maxnb")
     if f[1] != 0:
         warnings.warn("neighbour-list rebuild interval too long (a displacement exceeded "
                       "skin*h); reduce interval or increase skin")
+    if f3[2] != 0:
+        warnings.warn("non-finite particle positions at a neighbour-list rebuild (the "
+                      "simulation has diverged); they were binned into an edge cell")
     L.call("sph_flags_clear", nl._st._h)
     return f
 

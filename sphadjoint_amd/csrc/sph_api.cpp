@@ -78,6 +78,9 @@ void dfree(T *&p) {
 }
 
 constexpr int64_t kAgeNever = INT64_MAX;
+// state flags: [0] neighbour-list row overflow, [1] displacement beyond
+// skin*h, [2] a position that is not finite at a rebuild
+constexpr int kNumFlags = 3;
 
 // Derived FP64 constants with the reference's operation order in T = Float64.
 PhysF64 make_phys(const sph_params &p) {
@@ -470,6 +473,7 @@ struct sph_state {
   int *cellof = nullptr, *counts = nullptr, *cl_cstart = nullptr, *cursor = nullptr;
   int *tmp_id = nullptr, *tmp_slot = nullptr, *perm = nullptr, *new_ids = nullptr;
   int *bsum = nullptr;
+  int *bigcells = nullptr;  // cells of the last build with more than 128 entries (rank)
   int cl_view = 0;  // where the last build's (starts, order) live: 0 none, 1 cl buffers, 2 layout
   int *flags = nullptr;
   // auxiliary outputs: 0 = zeros (no step yet), 1 = recomputable from the
@@ -564,6 +568,7 @@ namespace {
 void state_free(sph_state *st) {
   int **ints[] = {&st->ids, &st->scell, &st->cstart, &st->cellof, &st->counts, &st->cl_cstart,
                   &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
+                  &st->bigcells,
                   &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt, &st->lane,
                   &st->bcell};
   for (auto pp : ints) dfree(*pp);
@@ -589,17 +594,20 @@ void do_build(sph_state *st) {
   sph_ctx *c = st->ctx;
   timed(c, K_ASSIGN, [&] {
     // counts are zero here: cleared at creation, then by each build's scan
-    if (st->precision == SPH_F32C) launch_assign_f32c(s, st->X, st->bs.B, st->cellof, st->counts);
-    else launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts);
+    if (st->precision == SPH_F32C)
+      launch_assign_f32c(s, st->X, st->bs.B, st->cellof, st->counts, st->flags);
+    else launch_assign(s, st->X, st->g, st->kc.cs, st->cellof, st->counts, st->flags);
   });
   timed(c, K_SCAN, [&] {
-    launch_scan_excl(s, st->counts, st->g.ncell, st->cl_cstart, st->cursor, st->bsum);
+    launch_scan_excl(s, st->counts, st->g.ncell, st->cl_cstart, st->cursor, st->bsum,
+                     st->bigcells);
   });
   timed(c, K_FILL, [&] {
     launch_fill(s, st->cellof, st->ids, N, st->cursor, st->tmp_id, st->tmp_slot);
   });
   timed(c, K_RANK, [&] {
-    launch_rank(s, st->cellof, st->cl_cstart, st->tmp_id, st->tmp_slot, N, st->perm, st->new_ids);
+    launch_rank(s, st->cellof, st->cl_cstart, st->tmp_id, st->tmp_slot, N, st->bigcells,
+                st->perm, st->new_ids);
   });
   st->cl_view = 1;  // the cell list of this build is (cl_cstart, new_ids)
 }
@@ -1269,7 +1277,8 @@ void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const in
       st->perm = dalloc<int>(n);
       st->new_ids = dalloc<int>(n);
       st->bsum = dalloc<int>(nc / 1024 + 2);
-      st->flags = dalloc<int>(2);
+      st->bigcells = dalloc<int>(nc + 1);
+      st->flags = dalloc<int>(kNumFlags);
       st->a_id = dalloc<double2>(n);
       st->pi_id = dalloc<double2>(n);
       st->a_slot = dalloc<double2>(n);
@@ -1297,7 +1306,7 @@ void state_create_impl(sph_ctx *ctx, const sph_params *ps, int64_t nsc, const in
         HIPC(hipMemcpyAsync(st->V, V0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
         HIPC(hipMemsetAsync(st->PI, 0, sizeof(double2) * n, s));
       }
-      HIPC(hipMemsetAsync(st->flags, 0, sizeof(int) * 2, s));
+      HIPC(hipMemsetAsync(st->flags, 0, sizeof(int) * kNumFlags, s));
       HIPC(hipMemsetAsync(st->a_id, 0, sizeof(double2) * n, s));
       HIPC(hipMemsetAsync(st->pi_id, 0, sizeof(double2) * n, s));
       HIPC(hipMemsetAsync(st->alpha_id, 0, sizeof(double) * n, s));
@@ -1486,10 +1495,19 @@ int sph_flags_get(sph_state *st, int32_t *flags2) {
   });
 }
 
+int sph_flags_get_all(sph_state *st, int32_t *flags3) {
+  return guarded([&] {
+    if (!st || !flags3) fail(SPH_E_ARG, "NULL argument");
+    HIPC(hipMemcpyAsync(flags3, st->flags, sizeof(int) * kNumFlags, hipMemcpyDeviceToHost,
+                        st->ctx->stream));
+    HIPC(hipStreamSynchronize(st->ctx->stream));
+  });
+}
+
 int sph_flags_clear(sph_state *st) {
   return guarded([&] {
     if (!st) fail(SPH_E_ARG, "NULL argument");
-    launch_fill_i32(st->ctx->stream, st->flags, 2, 0);
+    launch_fill_i32(st->ctx->stream, st->flags, kNumFlags, 0);
   });
 }
 

@@ -24,11 +24,14 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 }
 
 __global__ void k_assign(const double2 *__restrict__ X, Grid g, double cs, int *__restrict__ cellof,
-                         int *__restrict__ counts) {
+                         int *__restrict__ counts, int *__restrict__ flags) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   int c = -1;
   if (s < g.N) {
     double2 x = X[s];
+    // a NaN or infinite position bins into a clamped cell, as the reference's
+    // unsafe_trunc does; it is reported (flags[2])
+    if (!isfinite(x.x) || !isfinite(x.y)) flags[2] = 1;
     int cx = cell_coord_div(x.x / cs, g.nx);
     int cy = cell_coord_div(x.y / cs, g.ny);
     c = cx + cy * g.nx;
@@ -75,7 +78,14 @@ __device__ __forceinline__ int block_excl_scan(int v, int &total) {
   return off + inc - v;
 }
 
-__global__ void k_scan_partial(const int *__restrict__ in, int n, int *__restrict__ bsum) {
+// the list of cells with more than kRankSmall entries (big[0] = their count,
+// big[1..] = the cells, in no particular order): k_rank's extra workgroups
+// rank them by a bitmap of their ids instead of the per-entry count
+constexpr int kRankSmall = 128;
+
+__global__ void k_scan_partial(const int *__restrict__ in, int n, int *__restrict__ bsum,
+                               int *__restrict__ big) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) big[0] = 0;   // read by k_scan_final and k_rank
   const int base = blockIdx.x * kScanTile;
   int v = 0;
 #pragma unroll
@@ -93,7 +103,8 @@ __global__ void k_scan_partial(const int *__restrict__ in, int n, int *__restric
 // gives the same sum).  The counts are cleared once read, ready for the next
 // build's atomics.
 __global__ void k_scan_final(int *__restrict__ in, int n, const int *__restrict__ bsum,
-                             int *__restrict__ out, int *__restrict__ cursor) {
+                             int *__restrict__ out, int *__restrict__ cursor,
+                             int *__restrict__ big) {
   const int base = blockIdx.x * kScanTile;
   int pre = 0;
   for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) pre += bsum[b];
@@ -116,6 +127,7 @@ __global__ void k_scan_final(int *__restrict__ in, int n, const int *__restrict_
     if (i < n) {
       out[i] = ex;
       if (cursor) cursor[i] = ex;
+      if (big && vals[k] > kRankSmall) big[1 + atomicAdd(&big[0], 1)] = i;
     }
     ex += vals[k];
     if (i == n - 1) out[n] = ex;
@@ -140,15 +152,103 @@ __global__ void k_fill(const int *__restrict__ cellof, const int *__restrict__ i
   }
 }
 
-__global__ void k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
-                       const int *__restrict__ tmp_id, const int *__restrict__ tmp_slot, int N,
-                       int *__restrict__ perm, int *__restrict__ new_ids) {
+// Rank of one big cell's entries [b, e) by particle id (ids are distinct): a
+// workgroup sets one bit per id in an LDS bitmap of a window of kRankWin ids,
+// prefixes the popcounts of its words, and places each entry at b + (ids
+// below it); windows cover the cell's id span [lo, hi] in turn.  O(entries +
+// span / 32) per window instead of O(entries^2); the same ranks, hence the
+// same order, as the per-entry count.
+constexpr int kRankWords = 2048;
+constexpr int kRankWin = 32 * kRankWords;
+constexpr int kRankWords_t = kRankWords / kBlock;   // words per thread in the prefix
+__device__ void rank_big_cell(int b, int e, const int *__restrict__ tmp_id,
+                              const int *__restrict__ tmp_slot, int *__restrict__ perm,
+                              int *__restrict__ new_ids, unsigned *bits, int *pre) {
+  __shared__ int red[2 * (kBlock / 64)];
+  const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int q = b + tid; q < e; q += kBlock) {
+    const int id = tmp_id[q];
+    lo = min(lo, id);
+    hi = max(hi, id);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d, 64));
+    hi = max(hi, __shfl_xor(hi, d, 64));
+  }
+  if (lane == 0) {
+    red[w] = lo;
+    red[kBlock / 64 + w] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < kBlock / 64; ++v) {
+    lo = min(lo, red[v]);
+    hi = max(hi, red[kBlock / 64 + v]);
+  }
+  __syncthreads();
+  int base = 0;   // entries with ids below the current window
+  for (long long w0 = lo; w0 <= hi; w0 += kRankWin) {
+    for (int k = tid; k < kRankWords; k += kBlock) bits[k] = 0u;
+    __syncthreads();
+    for (int q = b + tid; q < e; q += kBlock) {
+      const long long d = (long long)tmp_id[q] - w0;
+      if (d >= 0 && d < kRankWin) atomicOr(&bits[d >> 5], 1u << (d & 31));
+    }
+    __syncthreads();
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRankWords_t; ++k) cnt += __popc(bits[tid * kRankWords_t + k]);
+    int total;
+    int ex = block_excl_scan(cnt, total);
+#pragma unroll
+    for (int k = 0; k < kRankWords_t; ++k) {
+      pre[tid * kRankWords_t + k] = ex;
+      ex += __popc(bits[tid * kRankWords_t + k]);
+    }
+    __syncthreads();
+    for (int q = b + tid; q < e; q += kBlock) {
+      const int id = tmp_id[q];
+      const long long d = (long long)id - w0;
+      if (d >= 0 && d < kRankWin) {
+        const int wd = (int)(d >> 5);
+        const int t = b + base + pre[wd] + __popc(bits[wd] & ((1u << (d & 31)) - 1u));
+        perm[t] = tmp_slot[q];
+        new_ids[t] = id;
+      }
+    }
+    base += total;
+    __syncthreads();
+  }
+}
+
+// Rank inside the cell by particle id: entry p of a cell with at most
+// kRankSmall entries counts the smaller ids of its cell; the last kRankBig
+// workgroups take the big cells listed by the scan (big), one at a time.
+constexpr int kRankBig = 64;
+__global__ void __launch_bounds__(kBlock)
+k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
+       const int *__restrict__ tmp_id, const int *__restrict__ tmp_slot, int N,
+       const int *__restrict__ big, int *__restrict__ perm, int *__restrict__ new_ids) {
+  __shared__ unsigned bits[kRankWords];
+  __shared__ int pre[kRankWords];
+  const int nb = (N + kBlock - 1) / kBlock;
+  if ((int)blockIdx.x >= nb) {
+    const int nbig = big[0];
+    for (int k = (int)blockIdx.x - nb; k < nbig; k += kRankBig) {
+      const int c = big[1 + k];
+      rank_big_cell(cstart[c], cstart[c + 1], tmp_id, tmp_slot, perm, new_ids, bits, pre);
+    }
+    return;
+  }
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   int so = tmp_slot[p];
   int id = tmp_id[p];
   int c = cellof[so];
   int b = cstart[c], e = cstart[c + 1];
+  if (e - b > kRankSmall) return;   // a big cell
   int rank = 0;
   for (int q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1 : 0;
   int t = b + rank;
@@ -250,21 +350,25 @@ __global__ void k_copy2(const double2 *__restrict__ src, int N, double2 *__restr
 void launch_fill_i32(hipStream_t s, int *p, int n, int v) {
   if (n > 0) k_fill_i32<<<nblocks(n), kBlock, 0, s>>>(p, n, v);
 }
-void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts) {
-  if (g.N > 0) k_assign<<<nblocks(g.N), kBlock, 0, s>>>(X, g, cs, cellof, counts);
+void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts,
+                   int *flags) {
+  if (g.N > 0) k_assign<<<nblocks(g.N), kBlock, 0, s>>>(X, g, cs, cellof, counts, flags);
 }
-void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum) {
+void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum,
+                      int *big) {
   int nb = (n + kScanTile - 1) / kScanTile;
-  k_scan_partial<<<nb, kBlock, 0, s>>>(in, n, bsum);
-  k_scan_final<<<nb, kBlock, 0, s>>>(in, n, bsum, out, cursor);
+  k_scan_partial<<<nb, kBlock, 0, s>>>(in, n, bsum, big);
+  k_scan_final<<<nb, kBlock, 0, s>>>(in, n, bsum, out, cursor, big);
 }
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
                  int *tmp_id, int *tmp_slot) {
   if (N > 0) k_fill<<<nblocks(N), kBlock, 0, s>>>(cellof, ids, N, cursor, tmp_id, tmp_slot);
 }
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
-                 const int *tmp_slot, int N, int *perm, int *new_ids) {
-  if (N > 0) k_rank<<<nblocks(N), kBlock, 0, s>>>(cellof, cstart, tmp_id, tmp_slot, N, perm, new_ids);
+                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids) {
+  if (N > 0)
+    k_rank<<<nblocks(N) + kRankBig, kBlock, 0, s>>>(cellof, cstart, tmp_id, tmp_slot, N, big,
+                                                    perm, new_ids);
 }
 void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                     const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,

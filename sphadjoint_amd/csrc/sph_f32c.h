@@ -110,8 +110,9 @@ constexpr int kMaskWords = 4;
 // byte offsets, which bounds the slots of an f32c state
 constexpr long long kMaxSlotsF32C = 0xFFFFFFFFLL / (3 * kMaskWords * 4);
 
+// (flags[2] = 1 when a position is not finite)
 void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
-                        int *counts);
+                        int *counts, int *flags);
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags);
 // first density pass after a rebuild, fused with the mask-list build: writes

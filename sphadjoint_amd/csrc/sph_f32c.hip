@@ -906,7 +906,7 @@ k_snapshot32(int to_snap, Batch B, double2 *__restrict__ X, float2 *__restrict__
 // wave-aggregated atomics
 __global__ void __launch_bounds__(kBlock)
 k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
-           int *__restrict__ counts) {
+           int *__restrict__ counts, int *__restrict__ flags) {
   const int blk = blockIdx.x;     // all lanes of the workgroup take part in the run search
   const int scn = B.b2s[blk];
   const Scene32 S = B.sc[scn];
@@ -915,6 +915,7 @@ k_assign32(const double2 *__restrict__ X, Batch B, int *__restrict__ cellof,
   if (sl < S.g.N) {
     const int s = S.off + sl;
     const double2 x = X[s];
+    if (!isfinite(x.x) || !isfinite(x.y)) flags[2] = 1;   // binned into a clamped cell
     const int cx = cell_coord_div(x.x / S.cs64, S.g.nx);
     const int cy = cell_coord_div(x.y / S.cs64, S.g.ny);
     c = S.coff + cx + cy * S.g.nx;
@@ -1462,8 +1463,8 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
 }  // namespace
 
 void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
-                        int *counts) {
-  if (B.N > 0) k_assign32<<<B.nblk, B.bs, 0, s>>>(X, B, cellof, counts);
+                        int *counts, int *flags) {
+  if (B.N > 0) k_assign32<<<B.nblk, B.bs, 0, s>>>(X, B, cellof, counts, flags);
 }
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
                           const Batch &B, uint32_t *nm, int *flags) {

@@ -156,13 +156,17 @@ __device__ __forceinline__ NbRow nb_row(const int *__restrict__ nl,
 
 // cell list (sph_cells.hip)
 void launch_fill_i32(hipStream_t s, int *p, int n, int v);
-void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts);
-// exclusive prefix of in[0..n) into out[0..n] (and cursor); clears in
-void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum);
+// (flags[2] = 1 when a position is not finite)
+void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cellof, int *counts,
+                   int *flags);
+// exclusive prefix of in[0..n) into out[0..n] (and cursor); clears in; lists
+// the cells with more than 128 entries in big (big[0] = count, n + 1 entries)
+void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum,
+                      int *big);
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
                  int *tmp_id, int *tmp_slot);
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
-                 const int *tmp_slot, int N, int *perm, int *new_ids);
+                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids);
 // also writes bcell[slot] = the slot's binning cell (cellof, the division rule)
 void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                     const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,
