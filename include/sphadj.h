@@ -208,16 +208,21 @@ int sph_batch_objective_dev(sph_state *st, double xc, double yc, double sigma, d
                             double *seed);
 
 /* Autotuner of the f32c form (no counterpart in the reference): picks the
- * workgroup size (64, 128 or 256) of the neighbour-list build, density,
- * force, adjoint R, P1 and P2 kernels by timing each candidate over `steps`
- * steps with a tape and their reverse sweep, and keeps the choice with the
- * state (tapes recorded afterwards inherit it).  At 256 threads P1 also runs
- * the step's gtheta gather in its launch, so P1's sizes are compared on P1 +
- * gather.  Advances st by 6*steps steps.  chosen (may be NULL) receives the
- * 6 sizes in that order. */
+ * launch configuration of the neighbour-list build, density, force, adjoint
+ * R, P1 and P2 kernels -- the workgroup size (64, 128 or 256) and whether the
+ * pass stages its window's neighbour rows in LDS or reads them from global
+ * memory -- by timing each of the six candidates over `steps` steps with a
+ * tape that keeps (pterm, invrho) and the exact masks (the kind the
+ * checkpointed and batched paths record) and their reverse sweep, and keeps
+ * the choice with the state (tapes recorded afterwards inherit it).  At 256
+ * threads P1 also runs the step's gtheta gather in its launch, so P1 is
+ * compared on P1 + gather.  Advances st by 12*steps steps.  chosen (may be
+ * NULL) receives the 6 configurations in that order: the workgroup size,
+ * negated when the kernel reads without LDS staging.  Every configuration
+ * gives the same results bit for bit. */
 int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *chosen);
-/* the tuned sizes of a state (6 entries, the order of sph_autotune), and
- * their transfer to another state of the same kind */
+/* the tuned configurations of a state (6 entries, as sph_autotune reports
+ * them), and their transfer to another state of the same kind */
 int sph_tuning_get(const sph_state *st, int32_t *sizes);
 int sph_tuning_set(sph_state *st, const int32_t *sizes);
 

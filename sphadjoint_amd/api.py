@@ -445,8 +445,10 @@ def simulate_graph_(st: State, theta, p: SPHParams, backend: MI355XBackend, nste
 
 
 def autotune_(st: State, theta, steps=4):
-    """Picks the workgroup size of the f32c pair kernels on this state (advances
-    it by 6*steps steps).  Returns {kernel: workgroup size}."""
+    """Picks the launch configuration of the f32c pair kernels on this state
+    (advances it by 12*steps steps): the workgroup size (64, 128, 256),
+    negated for a kernel that is faster reading its neighbours from global
+    memory than from LDS-staged rows.  Returns {kernel: configuration}."""
     f = _field(theta, st.p, st.backend)
     out = np.zeros(6, np.int32)
     L.call("sph_autotune", st._h, f._h, int(steps), _ptr(out))

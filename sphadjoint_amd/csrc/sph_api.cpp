@@ -175,7 +175,9 @@ ThetaGrid make_theta_grid(const sph_params &p, double rc) {
 // each per-slot workgroup size, and the size each tuned kernel launches with.
 struct BatchStore {
   Batch B{};              // the 256-thread geometry (untuned kernels)
-  Batch Bv[3]{};          // per kSlotBlocks entry
+  // per launch configuration: kSlotBlocks entry v with LDS staging (v) and
+  // without it (v + 3, the pair passes then read neighbours from global memory)
+  Batch Bv[6]{};
   int cfg[TK_COUNT] = {2, 2, 2, 2, 2, 2};
   std::vector<Scene32> host;
   Scene32 *sc = nullptr;
@@ -316,6 +318,10 @@ void batch_build(BatchStore &bs, std::vector<Scene32> scenes) {
     B.ncell = (int)coff;
     B.ntnode = (int)toff;
     B.ntcell = (int)tcoff;
+  }
+  for (int v = 0; v < 3; ++v) {
+    bs.Bv[v + 3] = bs.Bv[v];
+    bs.Bv[v + 3].nostage = 1;
   }
   bs.B = bs.Bv[2];
 }
@@ -2640,12 +2646,16 @@ void ck_free(sph_ckpt *ck) {
 
 extern "C" {
 
-// the tuned workgroup sizes of an f32c state (TK_COUNT entries)
+// the tuned launch configuration of an f32c state (TK_COUNT entries): the
+// workgroup size, negated when the kernel reads neighbours without LDS staging
 int sph_tuning_get(const sph_state *st, int32_t *sizes) {
   return guarded([&] {
     if (!st || !sizes) fail(SPH_E_ARG, "NULL argument");
     if (st->precision != SPH_F32C) fail(SPH_E_ARG, "the f32c form has tuned kernels");
-    for (int k = 0; k < TK_COUNT; ++k) sizes[k] = kSlotBlocks[st->bs.cfg[k]];
+    for (int k = 0; k < TK_COUNT; ++k) {
+      const int v = st->bs.cfg[k];
+      sizes[k] = kSlotBlocks[v % 3] * (v >= 3 ? -1 : 1);
+    }
   });
 }
 
@@ -2656,19 +2666,25 @@ int sph_tuning_set(sph_state *st, const int32_t *sizes) {
     int v[TK_COUNT];
     for (int k = 0; k < TK_COUNT; ++k) {
       v[k] = -1;
-      for (int j = 0; j < 3; ++j)
+      for (int j = 0; j < 3; ++j) {
         if (sizes[k] == kSlotBlocks[j]) v[k] = j;
-      if (v[k] < 0) fail(SPH_E_ARG, "workgroup size %d is not 64, 128 or 256", (int)sizes[k]);
+        if (sizes[k] == -kSlotBlocks[j]) v[k] = j + 3;
+      }
+      if (v[k] < 0)
+        fail(SPH_E_ARG, "launch configuration %d is not +-64, +-128 or +-256", (int)sizes[k]);
     }
     for (int k = 0; k < TK_COUNT; ++k) st->bs.cfg[k] = v[k];
   });
 }
 
-// Workgroup size of each tuned f32c kernel: for every candidate size, a
-// warm-up pass and a timed pass of `steps` steps with a tape plus the reverse
-// sweep over them (zero seeds: the same instruction stream), every kernel
-// timed with events; each kernel keeps its fastest size.  Advances st by
-// 6*steps steps.  chosen (may be NULL) receives the TK_COUNT sizes.
+// Launch configuration of each tuned f32c kernel: for every candidate (three
+// workgroup sizes, each with and without LDS staging of the neighbour rows), a
+// warm-up pass and a timed pass of `steps` steps with a tape of the kind the
+// checkpointed and batched paths record (keeping (pterm, invrho) and the exact
+// masks) plus the reverse sweep over them (zero seeds: the same instruction
+// stream), every kernel timed with events; each kernel keeps its fastest
+// configuration.  Advances st by 12*steps steps.  chosen (may be NULL)
+// receives the TK_COUNT configurations as sph_tuning_get reports them.
 int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *chosen) {
   sph_tape *tape = nullptr;
   sph_workspace *ws = nullptr;
@@ -2678,7 +2694,8 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
     if (steps < 1) fail(SPH_E_ARG, "steps must be >= 1");
     sph_ctx *c = st->ctx;
     if (c->prof) fail(SPH_E_STATE, "turn profiling off before tuning");
-    int r = sph_tape_create(c, st->N, steps, SPH_F32C, &tape);
+    int r = sph_tape_create_keep(c, st->N, steps, SPH_F32C,
+                                 SPH_TAPE_KEEP_PI | SPH_TAPE_KEEP_LISTS, &tape);
     if (r != SPH_OK) throw Error(r, g_err);
     workspace_create_impl(c, st->N, &st->p, SPH_F32C, st->ntnode,
                           st->bs.B.nscenes > 1 ? st->bs.B.ntcell
@@ -2692,7 +2709,7 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
       best[k] = 1e300;
       pick[k] = 2;
     }
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 6; ++v) {
       for (int k = 0; k < TK_COUNT; ++k) st->bs.cfg[k] = v;
       for (int pass = 0; pass < 2; ++pass) {
         HIPC(hipStreamSynchronize(c->stream));
@@ -2727,7 +2744,7 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
     }
     for (int k = 0; k < TK_COUNT; ++k) {
       st->bs.cfg[k] = pick[k];
-      if (chosen) chosen[k] = kSlotBlocks[pick[k]];
+      if (chosen) chosen[k] = kSlotBlocks[pick[k] % 3] * (pick[k] >= 3 ? -1 : 1);
       c->tot_ms[kid[k]] = 0.0;
       c->launches[kid[k]] = 0;
     }

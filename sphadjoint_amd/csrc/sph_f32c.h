@@ -54,6 +54,7 @@ struct Batch {
   // trange_tight a quarter cell (used for a step in which no slot is further out)
   const int2 *trange, *trange_tight;
   int bs;
+  int nostage;   // 1: the pair passes read neighbours from global memory (no LDS staging)
   int nscenes, nblk, ntblk, nnblk;
   int N, ncell, ntnode, ntcell;
 };

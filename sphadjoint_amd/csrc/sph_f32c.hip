@@ -485,7 +485,7 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int s = sl < S.g.N ? S.off + sl : w0;
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
-  stage_plan(T, bcell, cst, w0, w1, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0, w1, g, S.coff, B.nostage ? 0 : kStageCap);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
@@ -636,7 +636,7 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
@@ -663,7 +663,7 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   SCENE_WIN(B, lane);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
@@ -738,7 +738,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sx[kStageCap], spi[kStageCap], sv[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   if (T.ok)
@@ -960,7 +960,7 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
@@ -1287,7 +1287,7 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   float2 *const sx = L.sx, *const spi = L.spi, *const sab = L.sab, *const sv = L.sv;
   SCENE_WIN_AT(B, lane, blk);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCapP1);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapP1);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
@@ -1416,7 +1416,7 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   __shared__ float sgr[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
   if (T.ok)
     stage_copy(T,

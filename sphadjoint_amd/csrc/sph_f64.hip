@@ -99,16 +99,27 @@ __device__ __forceinline__ double density_at(const double2 *__restrict__ X, doub
 
 // _nl_build! (src/neighbors.jl:192-245) into slot space: the candidates of
 // the 3x3 block around the sweep cell, kept when r2min < r2 < rc2.
+// Each lane keeps its entries in LDS as 16-bit offsets into the row run they
+// come from (the sweep visits the three runs in order, so per-row counts
+// tell the rows apart), and the wavefront writes them out entry by entry:
+// at each m the lanes store consecutive words of nl[m*N + .], instead of
+// every lane storing into its own line at its own m during the sweep.
+// Lists longer than kNlLds entries (maxnb > kNlLds) and offsets past 16 bits
+// (a row run of more than 65,535 slots) are stored directly.
+constexpr int kNlLds = 64;
 __global__ void __launch_bounds__(kBlock)
 k_nl_build(const double2 *__restrict__ X, const int *__restrict__ scell,
            const int *__restrict__ cstart, Grid g, CellConstF64 k, int maxnb,
            int *__restrict__ nl, int *__restrict__ nbcnt, int *__restrict__ flags) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
+  __shared__ unsigned short ent[kNlLds][kBlock];
+  const int tid = (int)threadIdx.x;
+  int s = xcd_block() * blockDim.x + tid;
+  if (s >= g.N) return;   // (no barrier below: each lane uses its own LDS column)
   const double2 xi = X[s];
   Runs r = sweep_runs(cstart, scell[s], g.nx, g.ny);
-  int m = 0;
-  bool over = false;
+  int m = 0, mlds = 0;        // entries, and how many of them sit in LDS (a prefix)
+  int cnt[3] = {0, 0, 0};     // LDS entries from each row run
+  bool lds = maxnb <= kNlLds, over = false;
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
     for (int j = r.b[w]; j < r.e[w]; ++j) {
@@ -118,12 +129,27 @@ k_nl_build(const double2 *__restrict__ X, const int *__restrict__ scell,
       double r2 = dx * dx + dy * dy;
       if (k.r2min < r2 && r2 < k.rc2) {
         if (m < maxnb) {
-          nl[(size_t)m * g.N + s] = j;
+          const int b = j - r.b[w];
+          if (lds && b < 65536) {
+            ent[m][tid] = (unsigned short)b;
+            cnt[w] += 1;
+            mlds = m + 1;
+          } else {
+            lds = false;
+            nl[(size_t)m * g.N + s] = j;
+          }
           ++m;
         } else {
           over = true;
         }
       }
+    }
+  }
+  for (int q = 0; q < kNlLds; ++q) {
+    if (!__any(q < mlds)) break;
+    if (q < mlds) {
+      const int base = q < cnt[0] ? r.b[0] : (q < cnt[0] + cnt[1] ? r.b[1] : r.b[2]);
+      nl[(size_t)q * g.N + s] = base + (int)ent[q][tid];
     }
   }
   nbcnt[s] = m;

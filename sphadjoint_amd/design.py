@@ -150,7 +150,14 @@ class Optimizer:
 
     def optimize(self, rd0=None, *, maxit=40, step0=0.05, batched=False, tries=6):
         """Projected gradient with backtracking (scripts/03_optimize.jl:287-341).
-        Returns (rd as numpy, J history, iterations run)."""
+        Returns (rd as numpy, J history, iterations run).  batched=True scores
+        each iteration's candidates in one batched state, which exists in the
+        f32c form only: it takes the sequential search's decisions when the
+        optimizer's own precision is f32c, and is refused otherwise."""
+        if batched and self.precision != "f32c":
+            raise ValueError("batched candidate evaluation runs in the f32c form; an "
+                             f"optimizer of precision {self.precision!r} would compare its "
+                             "incumbent J with f32c candidates (use precision='f32c')")
         ch = self.chain
         rd = ch.field(np.ones(ch.shape) if rd0 is None else rd0)
         step = float(step0)

@@ -85,8 +85,8 @@ def test_autotune_keeps_results(be):
     ref = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
     st = S.State(be, sc.X0, sc.V0, pg, precision="f32c")
     chosen = S.autotune_(st, sc.theta, steps=4)
-    assert set(chosen.values()) <= {64, 128, 256}
-    S.simulate_(ref, sc.theta, pg, be, 24)          # the tuner advanced st by 24 steps
+    assert set(chosen.values()) <= {64, 128, 256, -64, -128, -256}
+    S.simulate_(ref, sc.theta, pg, be, 48)          # the tuner advanced st by 12*4 steps
     np.testing.assert_array_equal(st.X, ref.X)
     np.testing.assert_array_equal(st.V, ref.V)
     # forward + adjoint with the tuned sizes equal the default ones
@@ -104,6 +104,7 @@ def test_autotune_keeps_results(be):
 
 
 @pytest.mark.parametrize("size,keep", [(64, "plain"), (128, "plain"), (64, "pi"),
+                                       (-256, "pi_lists"), (-64, "plain"),
                                        (128, "pi_lists")])
 def test_small_workgroups_stage_the_whole_window(be, size, keep):
     """With 64- or 128-thread workgroups the pair passes still deal each
@@ -111,7 +112,9 @@ def test_small_workgroups_stage_the_whole_window(be, size, keep):
     rows: forward + adjoint equal the 256-thread launches bit for bit, on a
     batch (scene-local windows).  At 256 threads P1 and the gtheta gather
     share one launch, at the smaller sizes they launch apart; the tapes cover
-    P1 over the exact list (plain, pi_lists) and over the mask list (pi)."""
+    P1 over the exact list (plain, pi_lists) and over the mask list (pi).  A
+    negative size launches the passes without LDS staging (neighbours read
+    from global memory): the same results again."""
     say("workgroup size", size, "tape", keep)
     scs = scenes.batch_line_search(seed=5, nscenes=3, nmin=800, nmax=6000, nsteps=16)
     args = [(S.SPHParams(**sc.params), sc.X0, sc.V0) for sc in scs]

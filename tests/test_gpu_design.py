@@ -147,6 +147,9 @@ def test_line_search_f64_improves(be):
     sc, rd, cfg, (tgt, sig) = _design_scene(nsteps=20)
     p = S.SPHParams(**sc.params)
     opt = Optimizer(be, p, sc.X0, sc.V0, sc.nsteps, tgt, sig, interval=1, **cfg)
+    # batched candidates exist in the f32c form only: an FP64 optimizer refuses them
+    with pytest.raises(ValueError, match="f32c"):
+        opt.optimize(rd, maxit=1, batched=True)
     rd_out, hist, it = opt.optimize(rd, maxit=4)
     assert len(hist) >= 2 and all(b < a for a, b in zip(hist, hist[1:]))
     assert rd_out.mean() >= 0.75 - 1e-15 and rd_out.min() >= 0 and rd_out.max() <= 1
