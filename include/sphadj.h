@@ -180,6 +180,18 @@ int sph_backward_dev(sph_workspace *ws, sph_tape *tape, const sph_field *theta,
                      int64_t maxnb, int layout);
 int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
                              double *seed);
+/* sph_target_objective_dev for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  It allocates nothing and waits for nothing:
+ * the objective's partial sums go to a scratch array of the library's
+ * (static device memory, one copy per GPU, held until the process ends; at
+ * most 16384 partials, so N <= 4,194,304 -- larger states are refused with
+ * SPH_E_ARG), and J (one double) and seed ((2,N) doubles, id order, may be
+ * NULL) are device pointers the caller owns, written in stream order.  The
+ * results equal sph_target_objective_dev's bit for bit.  Calls that overlap
+ * on one GPU share its scratch array and race. */
+int sph_target_objective_noalloc(sph_state *st, double xc, double yc, double sigma, double *J,
+                                 double *seed);
 
 /* --- batched scenes (the FP32-compensated form).  No counterpart in the
  * reference, which runs one scene per State; its line search evaluates the

@@ -78,6 +78,7 @@ SIGNATURES = {
     "sph_buffer_write": (C.c_int, [_vp, _vp, _i64]),
     "sph_backward_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _d, _i64, _i64, C.c_int]),
     "sph_target_objective_dev": (C.c_int, [_vp, _d, _d, _d, C.POINTER(_d), _vp]),
+    "sph_target_objective_noalloc": (C.c_int, [_vp, _d, _d, _d, _vp, _vp]),
     "sph_ckpt_forward_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
     "sph_ckpt_backward_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "sph_batch_create": (C.c_int, [_vp, _i64, _PP, _vp, _vp, _vp, _d, _i64, _i64, _pp]),

@@ -361,6 +361,7 @@ struct sph_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false: the caller's stream (sph_ctx_create_stream)
+  double *scratch = nullptr;  // this device's copy of `scratch` (scratch_address)
   // objects created on this context plus the handle itself; the context is
   // freed when the handle has been destroyed and the last object is gone
   int refs = 1;
@@ -373,7 +374,25 @@ struct sph_ctx {
   int64_t launches[K_COUNT] = {};
 };
 
+// Scratch memory of the allocation-free entry point
+// (sph_target_objective_noalloc): the objective's per-workgroup partial sums,
+// kScratchPartials of them at most (N <= kScratchPartials * 256 particles).
+// Each GPU the library runs on holds its own copy until the process ends;
+// every call on a GPU shares that copy, so calls that overlap on one GPU
+// race.  Host code never names the array after set-up: scratch_address()
+// gives the current device's copy, which the context keeps.
+constexpr int kScratchPartials = 16384;
+static __device__ __align__(256) double scratch[kScratchPartials + 2];
+
 namespace {
+
+// the address of the current device's copy of `scratch`
+double *scratch_address() {
+  void *p = nullptr;
+  HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) fail(SPH_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  return static_cast<double *>(p);
+}
 
 void ctx_retain(sph_ctx *c) { c->refs += 1; }
 
@@ -823,8 +842,10 @@ int sph_ctx_create(int device, sph_ctx **out) {
     HIPC(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) fail(SPH_E_ARG, "device %d not present (%d devices)", device, n);
     HIPC(hipSetDevice(device));
+    double *scr = scratch_address();
     auto *c = new sph_ctx;
     c->device = device;
+    c->scratch = scr;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete c;
@@ -846,8 +867,10 @@ int sph_ctx_create_stream(int device, void *stream, sph_ctx **out) {
     int sdev = -1;
     HIPC(hipStreamGetDevice(hs, &sdev));
     if (sdev != device) fail(SPH_E_ARG, "the stream belongs to device %d, not %d", sdev, device);
+    double *scr = scratch_address();
     auto *c = new sph_ctx;
     c->device = device;
+    c->scratch = scr;
     c->stream = hs;
     c->own_stream = false;
     *out = c;
@@ -2280,6 +2303,26 @@ int sph_target_objective(sph_state *st, double xc, double yc, double sigma, doub
 int sph_target_objective_dev(sph_state *st, double xc, double yc, double sigma, double *J,
                              double *seed) {
   return objective_impl(st, xc, yc, sigma, J, seed, true);
+}
+
+// sph_target_objective_dev without library memory: the partial sums go to this
+// device's scratch array, J and the id-order seed straight to the caller's
+// device memory (the partial kernel writes each slot's seed at its id), and
+// nothing is waited for.  No allocation, no memset, no host transfer.
+int sph_target_objective_noalloc(sph_state *st, double xc, double yc, double sigma, double *J,
+                                 double *seed) {
+  return guarded([&] {
+    if (!st || !J) fail(SPH_E_ARG, "NULL argument");
+    if (is_batched(st)) fail(SPH_E_ARG, "batched states have one J per scene: sph_batch_objective");
+    const int nb = objective_blocks((int)st->N);
+    if (nb > kScratchPartials)
+      fail(SPH_E_ARG, "N=%lld needs %d objective partials; the scratch array holds %d "
+                      "(N <= %d)", (long long)st->N, nb, kScratchPartials,
+           kScratchPartials * kBlock);
+    if (!st->ctx->scratch) fail(SPH_E_STATE, "the context has no scratch address");
+    launch_objective_f64(st->ctx->stream, st->X, (int)st->N, xc, yc, sigma, st->ctx->scratch, nb,
+                         J, reinterpret_cast<double2 *>(seed), seed ? st->ids : nullptr);
+  });
 }
 
 }  // extern "C"

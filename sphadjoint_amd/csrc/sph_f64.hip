@@ -386,9 +386,10 @@ k_adj_p2(const double2 *__restrict__ X, const double *__restrict__ grho,
 // target_objective (src/driver.jl:219-234) in FP64: per-block partial sums
 // in a fixed order, then one block sums the partials, so J is bit-identical
 // from run to run.
+// (seed[s] in slot order, or seed[ids[s]] in particle-id order when ids is given)
 __global__ void k_objective_partial(const double2 *__restrict__ X, int N, double xc, double yc,
                                     double s2, double *__restrict__ partials,
-                                    double2 *__restrict__ seed) {
+                                    double2 *__restrict__ seed, const int *__restrict__ ids) {
   __shared__ double red[kBlock];
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
@@ -398,7 +399,7 @@ __global__ void k_objective_partial(const double2 *__restrict__ X, int N, double
     double dy = x.y - yc;
     double e = exp(-(dx * dx + dy * dy) / (2.0 * s2));
     v = -e;
-    if (seed) seed[s] = make_double2((e * dx) / s2, (e * dy) / s2);
+    if (seed) seed[ids ? ids[s] : s] = make_double2((e * dx) / s2, (e * dy) / s2);
   }
   red[threadIdx.x] = v;
   __syncthreads();
@@ -469,9 +470,10 @@ void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, cons
 }
 int objective_blocks(int N) { return N > 0 ? nblocks(N) : 1; }
 void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,
-                          double sigma, double *partials, int nb, double *J, double2 *seed) {
+                          double sigma, double *partials, int nb, double *J, double2 *seed,
+                          const int *ids) {
   double s2 = sigma * sigma;
-  if (N > 0) k_objective_partial<<<nb, kBlock, 0, s>>>(X, N, xc, yc, s2, partials, seed);
+  if (N > 0) k_objective_partial<<<nb, kBlock, 0, s>>>(X, N, xc, yc, s2, partials, seed, ids);
   else launch_fill_i32(s, (int *)partials, 2, 0);
   k_objective_final<<<1, kBlock, 0, s>>>(partials, N > 0 ? nb : 1, J);
 }

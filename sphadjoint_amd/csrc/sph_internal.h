@@ -220,9 +220,12 @@ void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const 
                        double2 *gV, double *grho);
 void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
                        const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gX);
+// J = the fixed-order sum of nblocks partials (partials: nblocks scratch
+// doubles, at least 2); seed (may be NULL) in slot order, or in particle-id
+// order through ids when ids is given
 void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,
                           double sigma, double *partials, int nblocks, double *J,
-                          double2 *seed);
+                          double2 *seed, const int *ids = nullptr);
 int objective_blocks(int N);
 
 // deterministic gtheta gather (sph_theta.hip): the theta-cell partials of one
