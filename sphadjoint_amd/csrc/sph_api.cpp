@@ -652,9 +652,11 @@ void materialise_aux(sph_state *st) {
                       nullptr, nullptr, st->a_slot, st->alpha_slot);
     launch_scatter2f(s, st->PI32, st->ids, N, st->pi_id);
   } else {
-    launch_density_f64(s, st->Xn, st->nl, st->nbcnt, st->g, st->P, st->PI, st->rho_slot, nullptr);
+    const EpochCells ec{st->bcell, st->scell, st->cstart};
+    launch_density_f64(s, st->Xn, st->nl, st->nbcnt, st->g, st->P, ec, st->PI, st->rho_slot,
+                       nullptr);
     launch_force_f64(s, st->Xn, st->Vn, st->PI, st->theta_last, st->nl, st->nbcnt, st->g, st->P,
-                     nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
+                     ec, nullptr, nullptr, nullptr, st->a_slot, st->alpha_slot);
     launch_scatter2(s, st->PI, st->ids, N, st->pi_id);
   }
   launch_scatter2(s, st->a_slot, st->ids, N, st->a_id);
@@ -795,11 +797,12 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
     st->aux_state = 1;
     return;
   }
+  const EpochCells ec{st->bcell, st->scell, st->cstart};
   timed(st->ctx, K_DENSITY, [&] {
-    launch_density_f64(s, st->X, st->nl, st->nbcnt, st->g, st->P, PI, nullptr, tX);
+    launch_density_f64(s, st->X, st->nl, st->nbcnt, st->g, st->P, ec, PI, nullptr, tX);
   });
   timed(st->ctx, K_FORCE, [&] {
-    launch_force_f64(s, st->X, st->V, PI, theta, st->nl, st->nbcnt, st->g, st->P, st->Xn,
+    launch_force_f64(s, st->X, st->V, PI, theta, st->nl, st->nbcnt, st->g, st->P, ec, st->Xn,
                      st->Vn, tV, nullptr, nullptr);
   });
   std::swap(st->X, st->Xn);
@@ -2162,13 +2165,14 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       const double2 *PIk = ws->PI;
       ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
       const LevelArgs LA{E.bcell, ws->trange64, ws->tlev, ws->tstamp, T.ngx - 1, T.ngy};
+      const EpochCells EC{E.bcell, E.scell, E.cstart};
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
           PIk = tape->PI + off;
           launch_adj_rpw_f64(s, Xk, Vnext, theta->d, g, P, ws->gX, ws->gV, ws->abar, ws->galpha,
                              LA);
         } else {
-          launch_adj_r_f64(s, Xk, Vnext, theta->d, nl, nbcnt, g, P, ws->PI, ws->gX, ws->gV,
+          launch_adj_r_f64(s, Xk, Vnext, theta->d, nl, nbcnt, g, P, EC, ws->PI, ws->gX, ws->gV,
                            ws->abar, ws->galpha, LA);
         }
       });
@@ -2177,11 +2181,11 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
                                ws->tstamp);
       });
       timed(c, K_ADJ_P1, [&] {
-        launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, ws->gXs,
-                          ws->gV, ws->grho);
+        launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, EC,
+                          ws->gXs, ws->gV, ws->grho);
       });
       timed(c, K_ADJ_P2, [&] {
-        launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, nl, nbcnt, g, P, ws->gX);
+        launch_adj_p2_f64(s, Xk, ws->grho, ws->gXs, nl, nbcnt, g, P, EC, ws->gX);
       });
     }
   }

@@ -15,9 +15,12 @@
 // Kernels: list build (per rebuild); forward density (pterm, invrho) and
 // force + implicit-drag integrate; adjoint R (density recompute, integrate
 // and drag adjoints), P1 (pair adjoint and design term), P2 (density adjoint,
-// gX += gXs).  gtheta is the deterministic gather of sph_theta.hip.
+// gX += gXs).  gtheta is the deterministic gather of sph_theta.hip.  The pair
+// passes copy the neighbour rows of their window of slots into LDS first
+// (sph_stage.h) and read the list's neighbours there.
 #pragma clang fp contract(off)
 #include "sph_internal.h"
+#include "sph_stage.h"
 
 namespace sph {
 
@@ -77,13 +80,98 @@ __device__ __forceinline__ double interp_alpha(const double *__restrict__ th, co
   return a;
 }
 
-// density sum of slot s (density_kernel!, src/forward.jl:25-65)
-__device__ __forceinline__ double density_at(const double2 *__restrict__ X, double2 xi,
+// ---- neighbour reads of the pair passes ------------------------------------
+// Each pass walks the slot's list in order and reads the neighbours' data
+// either from the state arrays or from the LDS rows of its window
+// (sph_stage.h); both give the same values, so the sums are the same.
+//
+// kPairMode, bit 0: stage the window's rows in LDS; bit 1: fetch the list
+// entries kAhead at a time, one group ahead of their use.  On a 1M-particle
+// block staging made every FP64 pass slower (force 95 -> 146 us, density
+// 49 -> 56 us: the L1-cached gathers are as fast as the LDS reads, and the
+// copy, the index map and the lower occupancy are extra), while fetching
+// ahead sped P1 up by 12 %; the default is 2 (build with
+// -DSPH_F64_PAIR_MODE=3 for the staged passes).
+#ifndef SPH_F64_PAIR_MODE
+#define SPH_F64_PAIR_MODE 2
+#endif
+#ifndef SPH_F64_AHEAD
+#define SPH_F64_AHEAD 4
+#endif
+constexpr int kPairMode = SPH_F64_PAIR_MODE;
+constexpr int kAhead = SPH_F64_AHEAD;
+// staging capacity in slots (a 256-slot window of a fluid block with ~8 slots
+// per cell stages ~830 slots; windows over two cell rows ~1100)
+constexpr int kStageF64 = 1024;
+
+// LDS position of list entry j of a slot whose own cell row is segment row k
+// of the window: the sweep's three rows ascend in slot order, so the row of
+// j is (j >= lo[k+1]) + (j >= lo[k+2]) (lo = INT_MAX for a row outside the
+// grid), and its position is j minus that segment's delta.
+struct StagedIdx {
+  int t1, t2, d0, d1, d2;
+  __device__ __forceinline__ int operator()(int j) const {
+    return j - (j >= t2 ? d2 : (j >= t1 ? d1 : d0));
+  }
+};
+__device__ __forceinline__ StagedIdx staged_idx(const StageTab &T, int bc, const Grid &g) {
+  const int k = (cell_xy(bc, g).y - T.ya) * 3;
+  return StagedIdx{T.lo[k + 1], T.lo[k + 2], T.delta[k], T.delta[k + 1], T.delta[k + 2]};
+}
+struct SlotIdx {
+  __device__ __forceinline__ int operator()(int j) const { return j; }
+};
+
+// Stages the rows of the window starting at slot w0 (st(i, ld(j)) stores
+// slot j's data at LDS position i); every thread of the workgroup calls it.
+// True when the calling wave reads from LDS: the window fits, and every slot
+// of the wave sweeps around its binning cell (a slot's sweep cell, from
+// x*(1/cs), and its binning cell, from x/cs, differ only where the two
+// roundings fall on either side of a cell boundary; such a wave reads from
+// global memory).
+template <class Ld, class St>
+__device__ __forceinline__ bool stage_rows(StageTab &T, const EpochCells &ec, int w0,
+                                           const Grid &g, Ld &&ld, St &&st) {
+  stage_plan_at(T, ec.bcell, [&](int i) { return ec.cstart[i]; }, w0, min(w0 + kBlock, g.N), g,
+                0, kStageF64);
+  if (!T.ok) return false;
+  stage_copy(T, ld, st);
+  const int s = w0 + (int)threadIdx.x;
+  return __all(s >= g.N || ec.scell[s] == ec.bcell[s]);
+}
+
+// fn(j) for the entries of row in order; with kAhead the entries of the next
+// group are loaded before the current group is evaluated
+template <bool kFetchAhead, class Fn>
+__device__ __forceinline__ void nl_walk(const NbRow &row, Fn &&fn) {
+  if constexpr (!kFetchAhead) {
+    for (int m = 0; m < row.n; ++m) fn(row.at(m));
+  } else {
+    int cur[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) cur[u] = u < row.n ? row.at(u) : 0;
+    for (int m = 0; m < row.n; m += kAhead) {
+      int nxt[kAhead];
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u)
+        nxt[u] = m + kAhead + u < row.n ? row.at(m + kAhead + u) : 0;
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u)
+        if (m + u < row.n) fn(cur[u]);
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) cur[u] = nxt[u];
+    }
+  }
+}
+
+// density sum of slot s (density_kernel!, src/forward.jl:25-65); xs[idx(j)]
+// is the position of list entry j
+template <class Idx>
+__device__ __forceinline__ double density_at(const double2 *__restrict__ xs, Idx idx, double2 xi,
                                              const NbRow &row, const PhysF64 &P) {
   double acc = 0.0;
-  for (int m = 0; m < row.n; ++m) {
-    const int j = row.at(m);
-    double2 xj = X[j];
+  nl_walk<(kPairMode & 2) != 0>(row, [&](int j) {
+    double2 xj = xs[idx(j)];
     double dx = xi.x - xj.x;
     double dy = xi.y - xj.y;
     double r2 = dx * dx + dy * dy;
@@ -93,8 +181,17 @@ __device__ __forceinline__ double density_at(const double2 *__restrict__ X, doub
       double u2 = u * u;
       acc += P.m * (((P.A * u2) * u2) * (2.0 * q + 1.0));
     }
-  }
+  });
   return acc + P.m * P.A;
+}
+
+// the density of slot s from the window's LDS rows sx when staged, else from X
+__device__ __forceinline__ double density_of(const double2 *__restrict__ X, const double2 *sx,
+                                             bool lds, const StageTab &T, const EpochCells &ec,
+                                             int s, double2 xi, const NbRow &row, const Grid &g,
+                                             const PhysF64 &P) {
+  if ((kPairMode & 1) && lds) return density_at(sx, staged_idx(T, ec.bcell[s], g), xi, row, P);
+  return density_at(X, SlotIdx{}, xi, row, P);
 }
 
 // _nl_build! (src/neighbors.jl:192-245) into slot space: the candidates of
@@ -156,14 +253,24 @@ k_nl_build(const double2 *__restrict__ X, const int *__restrict__ scell,
   if (over) flags[0] = 1;
 }
 
+// (the staged passes keep one array per neighbour quantity, of kStageF64
+// slots when staging is on)
+constexpr int kStageArr = (kPairMode & 1) ? kStageF64 : 1;
+
 __global__ void __launch_bounds__(kBlock)
 k_density(const double2 *__restrict__ X, const int *__restrict__ nl,
-          const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ PI,
-          double *__restrict__ rho_out, double2 *__restrict__ tapeX) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
+          const int *__restrict__ nbcnt, Grid g, PhysF64 P, EpochCells ec,
+          double2 *__restrict__ PI, double *__restrict__ rho_out, double2 *__restrict__ tapeX) {
+  __shared__ StageTab T;
+  __shared__ double2 sx[kStageArr];
+  const int w0 = xcd_block() * kBlock, s = w0 + (int)threadIdx.x;
+  bool lds = false;
+  if constexpr ((kPairMode & 1) != 0)
+    lds = stage_rows(T, ec, w0, g, [&](int j) { return X[j]; },
+                     [&](int i, double2 d) { sx[i] = d; });
   if (s >= g.N) return;
   double2 xi = X[s];
-  double rho = density_at(X, xi, nb_row(nl, nbcnt, s, g.N), P);
+  double rho = density_of(X, sx, lds, T, ec, s, xi, nb_row(nl, nbcnt, s, g.N), g, P);
   // eos_kernel! (src/forward.jl:68-75)
   double inv = 1.0 / rho;
   PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
@@ -173,21 +280,18 @@ k_density(const double2 *__restrict__ X, const int *__restrict__ nl,
 
 // accel_kernel! + integrate_kernel! (src/forward.jl:77-147), fused: the new
 // state goes to separate buffers because neighbours still read the old one.
-__global__ void __launch_bounds__(kBlock)
-k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
-        const double2 *__restrict__ PI, const double *__restrict__ theta,
-        const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
-        double2 *__restrict__ Xn, double2 *__restrict__ Vn, double2 *__restrict__ tapeV,
-        double2 *__restrict__ a_out, double *__restrict__ alpha_out) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
-  double2 xi = X[s], vi = V[s], pii = PI[s];
-  double pti = pii.x, iri = pii.y;
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+// pair sum of accel_kernel! (src/forward.jl:77-130) for slot s: xs, pis, vs
+// at idx(j) hold the position, (pterm, invrho) and velocity of entry j
+template <class Idx>
+__device__ __forceinline__ double2 force_sum(const double2 *__restrict__ xs,
+                                             const double2 *__restrict__ pis,
+                                             const double2 *__restrict__ vs, Idx idx, double2 xi,
+                                             double2 vi, double pti, double iri,
+                                             const NbRow &row, const PhysF64 &P) {
   double ax = 0.0, ay = 0.0;
-  for (int m = 0; m < row.n; ++m) {
-    const int j = row.at(m);
-    double2 xj = X[j];
+  nl_walk<(kPairMode & 2) != 0>(row, [&](int j) {
+    const int i = idx(j);
+    double2 xj = xs[i];
     double dx = xi.x - xj.x;
     double dy = xi.y - xj.y;
     double r2 = dx * dx + dy * dy;
@@ -195,16 +299,49 @@ k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
       double q = sqrt(r2) * P.invh;
       double u = 1.0 - 0.5 * q;
       double F = ((P.Fc * u) * u) * u;
-      double2 pij = PI[j];
+      double2 pij = pis[i];
       double Pij = P.m * (pti + pij.x);
       ax -= (Pij * F) * dx;
       ay -= (Pij * F) * dy;
       double C = ((P.twomumass * iri) * pij.y) * F;
-      double2 vj = V[j];
+      double2 vj = vs[i];
       ax += C * (vi.x - vj.x);
       ay += C * (vi.y - vj.y);
     }
-  }
+  });
+  return make_double2(ax, ay);
+}
+
+struct NbXPV64 {
+  double2 x, pi, v;
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
+        const double2 *__restrict__ PI, const double *__restrict__ theta,
+        const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
+        EpochCells ec, double2 *__restrict__ Xn, double2 *__restrict__ Vn,
+        double2 *__restrict__ tapeV, double2 *__restrict__ a_out,
+        double *__restrict__ alpha_out) {
+  __shared__ StageTab T;
+  __shared__ double2 sx[kStageArr], spi[kStageArr], sv[kStageArr];
+  const int w0 = xcd_block() * kBlock, s = w0 + (int)threadIdx.x;
+  bool lds = false;
+  if constexpr ((kPairMode & 1) != 0)
+    lds = stage_rows(T, ec, w0, g, [&](int j) { return NbXPV64{X[j], PI[j], V[j]}; },
+                     [&](int i, const NbXPV64 &d) {
+                       sx[i] = d.x;
+                       spi[i] = d.pi;
+                       sv[i] = d.v;
+                     });
+  if (s >= g.N) return;
+  double2 xi = X[s], vi = V[s], pii = PI[s];
+  double pti = pii.x, iri = pii.y;
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  double2 a2 = ((kPairMode & 1) && lds)
+                   ? force_sum(sx, spi, sv, staged_idx(T, ec.bcell[s], g), xi, vi, pti, iri, row, P)
+                   : force_sum(X, PI, V, SlotIdx{}, xi, vi, pti, iri, row, P);
+  double ax = a2.x, ay = a2.y;
   Interp ic = interp_cell(xi.x, xi.y, P);
   double al = interp_alpha(theta, ic, P, nullptr, nullptr);
   ax += P.gx + wall_accel(xi.x, P.Lx, P.kw);
@@ -240,13 +377,19 @@ __device__ __forceinline__ void sweep_level(const LevelArgs &L, const Grid &g, i
 __global__ void __launch_bounds__(kBlock)
 k_adj_r(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
         const double *__restrict__ theta, const int *__restrict__ nl,
-        const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ PI,
-        double2 *__restrict__ gX, double2 *__restrict__ gV, double2 *__restrict__ abar,
-        double *__restrict__ galpha, LevelArgs L) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
+        const int *__restrict__ nbcnt, Grid g, PhysF64 P, EpochCells ec,
+        double2 *__restrict__ PI, double2 *__restrict__ gX, double2 *__restrict__ gV,
+        double2 *__restrict__ abar, double *__restrict__ galpha, LevelArgs L) {
+  __shared__ StageTab T;
+  __shared__ double2 sx[kStageArr];
+  const int w0 = xcd_block() * kBlock, s = w0 + (int)threadIdx.x;
+  bool lds = false;
+  if constexpr ((kPairMode & 1) != 0)
+    lds = stage_rows(T, ec, w0, g, [&](int j) { return X[j]; },
+                     [&](int i, double2 d) { sx[i] = d; });
   if (s >= g.N) return;
   double2 xi = X[s];
-  double rho = density_at(X, xi, nb_row(nl, nbcnt, s, g.N), P);
+  double rho = density_of(X, sx, lds, T, ec, s, xi, nb_row(nl, nbcnt, s, g.N), g, P);
   double inv = 1.0 / rho;
   PI[s] = make_double2(((P.c2 * (rho - P.rho0)) * inv) * inv, inv);
   Interp ic = interp_cell(xi.x, xi.y, P);
@@ -285,21 +428,23 @@ k_adj_rpw(const double2 *__restrict__ X, const double2 *__restrict__ Vnext,
 
 // Adjoint pass P1: adj_pass1_kernel! (src/adjoint.jl:30-122) followed by the
 // design term of adj_design_kernel! (src/adjoint.jl:166-185) and gV += gVs.
-__global__ void __launch_bounds__(kBlock)
-k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
-         const double2 *__restrict__ PI, const double2 *__restrict__ abar,
-         const double *__restrict__ galpha, const double *__restrict__ theta,
-         const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
-         double2 *__restrict__ gXs, double2 *__restrict__ gV, double *__restrict__ grho) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
-  double2 xi = X[s], vi = V[s], ab = abar[s], pii = PI[s];
-  double pti = pii.x, iri = pii.y;
+// pair sums of adj_pass1_kernel! (src/adjoint.jl:30-122) for slot s: xs,
+// pis, abs, vs at idx(j) hold the position, (pterm, invrho), abar and
+// velocity of entry j
+struct P1Sums {
+  double gx, gy, gvx, gvy, grh, sumS;
+};
+template <class Idx>
+__device__ __forceinline__ P1Sums p1_sums(const double2 *__restrict__ xs,
+                                          const double2 *__restrict__ pis,
+                                          const double2 *__restrict__ abs,
+                                          const double2 *__restrict__ vs, Idx idx, double2 xi,
+                                          double2 vi, double2 ab, double pti, double iri,
+                                          const NbRow &row, const PhysF64 &P) {
   double gx = 0.0, gy = 0.0, gvx = 0.0, gvy = 0.0, grh = 0.0, sumS = 0.0;
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
-  for (int m = 0; m < row.n; ++m) {
-    const int j = row.at(m);
-    double2 xj = X[j];
+  nl_walk<(kPairMode & 2) != 0>(row, [&](int j) {
+    const int i = idx(j);
+    double2 xj = xs[i];
     double dx = xi.x - xj.x;
     double dy = xi.y - xj.y;
     double r2 = dx * dx + dy * dy;
@@ -309,12 +454,12 @@ k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
       double u = 1.0 - 0.5 * q;
       double F = ((P.Fc * u) * u) * u;
       double G = (P.Gc * u) * u;
-      double2 pij = PI[j];
+      double2 pij = pis[i];
       double Pij = P.m * (pti + pij.x);
       double Cbase = (P.twomumass * iri) * pij.y;
       double C = Cbase * F;
-      double2 abj = abar[j];
-      double2 vj = V[j];
+      double2 abj = abs[i];
+      double2 vj = vs[i];
       double dax = ab.x - abj.x;
       double day = ab.y - abj.y;
       double dvx = vi.x - vj.x;
@@ -332,7 +477,42 @@ k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
       grh -= (C * ad_v) * iri;
       sumS += F * ad_d;
     }
-  }
+  });
+  return P1Sums{gx, gy, gvx, gvy, grh, sumS};
+}
+
+struct NbXPAV64 {
+  double2 x, pi, ab, v;
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
+         const double2 *__restrict__ PI, const double2 *__restrict__ abar,
+         const double *__restrict__ galpha, const double *__restrict__ theta,
+         const int *__restrict__ nl, const int *__restrict__ nbcnt, Grid g, PhysF64 P,
+         EpochCells ec, double2 *__restrict__ gXs, double2 *__restrict__ gV,
+         double *__restrict__ grho) {
+  __shared__ StageTab T;
+  __shared__ double2 sx[kStageArr], spi[kStageArr], sab[kStageArr], sv[kStageArr];
+  const int w0 = xcd_block() * kBlock, s = w0 + (int)threadIdx.x;
+  bool lds = false;
+  if constexpr ((kPairMode & 1) != 0)
+    lds = stage_rows(T, ec, w0, g, [&](int j) { return NbXPAV64{X[j], PI[j], abar[j], V[j]}; },
+                     [&](int i, const NbXPAV64 &d) {
+                       sx[i] = d.x;
+                       spi[i] = d.pi;
+                       sab[i] = d.ab;
+                       sv[i] = d.v;
+                     });
+  if (s >= g.N) return;
+  double2 xi = X[s], vi = V[s], ab = abar[s], pii = PI[s];
+  double pti = pii.x, iri = pii.y;
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  const P1Sums q = ((kPairMode & 1) && lds)
+                       ? p1_sums(sx, spi, sab, sv, staged_idx(T, ec.bcell[s], g), xi, vi, ab, pti,
+                                 iri, row, P)
+                       : p1_sums(X, PI, abar, V, SlotIdx{}, xi, vi, ab, pti, iri, row, P);
+  double gx = q.gx, gy = q.gy, gvx = q.gvx, gvy = q.gvy, grh = q.grh, sumS = q.sumS;
   grh += (((-2.0 * P.m) * pti) * iri) * sumS;
   grh += (((P.c2 * P.m) * iri) * iri) * sumS;
   gx += ab.x * wall_daccel(xi.x, P.Lx, P.kw);
@@ -352,35 +532,63 @@ k_adj_p1(const double2 *__restrict__ X, const double2 *__restrict__ V,
 }
 
 // Adjoint pass P2: adj_pass2_kernel! (src/adjoint.jl:126-156) and gX += gXs.
-__global__ void __launch_bounds__(kBlock)
-k_adj_p2(const double2 *__restrict__ X, const double *__restrict__ grho,
-         const double2 *__restrict__ gXs, const int *__restrict__ nl,
-         const int *__restrict__ nbcnt, Grid g, PhysF64 P, double2 *__restrict__ gX) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= g.N) return;
-  double2 xi = X[s];
-  double gri = grho[s];
+// pair sum of adj_pass2_kernel! (src/adjoint.jl:126-156) for slot s: xs and
+// grs at idx(j) hold the position and rho-bar of entry j
+template <class Idx>
+__device__ __forceinline__ double2 p2_sum(const double2 *__restrict__ xs,
+                                          const double *__restrict__ grs, Idx idx, double2 xi,
+                                          double gri, const NbRow &row, const PhysF64 &P) {
   double gx = 0.0, gy = 0.0;
-  const NbRow row = nb_row(nl, nbcnt, s, g.N);
-  for (int m = 0; m < row.n; ++m) {
-    const int j = row.at(m);
-    double2 xj = X[j];
+  nl_walk<(kPairMode & 2) != 0>(row, [&](int j) {
+    const int i = idx(j);
+    double2 xj = xs[i];
     double dx = xi.x - xj.x;
     double dy = xi.y - xj.y;
     double r2 = dx * dx + dy * dy;
     if (r2 < P.r2max) {
       double q = sqrt(r2) * P.invh;
       double u = 1.0 - 0.5 * q;
-      double wgt = (P.m * (((P.Fc * u) * u) * u)) * (gri + grho[j]);
+      double wgt = (P.m * (((P.Fc * u) * u) * u)) * (gri + grs[i]);
       gx += wgt * dx;
       gy += wgt * dy;
     }
-  }
+  });
+  return make_double2(gx, gy);
+}
+
+struct NbXG64 {
+  double2 x;
+  double gr;
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_adj_p2(const double2 *__restrict__ X, const double *__restrict__ grho,
+         const double2 *__restrict__ gXs, const int *__restrict__ nl,
+         const int *__restrict__ nbcnt, Grid g, PhysF64 P, EpochCells ec,
+         double2 *__restrict__ gX) {
+  __shared__ StageTab T;
+  __shared__ double2 sx[kStageArr];
+  __shared__ double sgr[kStageArr];
+  const int w0 = xcd_block() * kBlock, s = w0 + (int)threadIdx.x;
+  bool lds = false;
+  if constexpr ((kPairMode & 1) != 0)
+    lds = stage_rows(T, ec, w0, g, [&](int j) { return NbXG64{X[j], grho[j]}; },
+                     [&](int i, const NbXG64 &d) {
+                       sx[i] = d.x;
+                       sgr[i] = d.gr;
+                     });
+  if (s >= g.N) return;
+  double2 xi = X[s];
+  double gri = grho[s];
+  const NbRow row = nb_row(nl, nbcnt, s, g.N);
+  const double2 q = ((kPairMode & 1) && lds)
+                        ? p2_sum(sx, sgr, staged_idx(T, ec.bcell[s], g), xi, gri, row, P)
+                        : p2_sum(X, grho, SlotIdx{}, xi, gri, row, P);
   double2 gs = gXs[s];
-  double sx = gs.x + gx;
-  double sy = gs.y + gy;
+  double tx = gs.x + q.x;
+  double ty = gs.y + q.y;
   double2 go = gX[s];
-  gX[s] = make_double2(go.x + sx, go.y + sy);
+  gX[s] = make_double2(go.x + tx, go.y + ty);
 }
 
 // target_objective (src/driver.jl:219-234) in FP64: per-block partial sums
@@ -431,24 +639,26 @@ void launch_nl_build_f64(hipStream_t s, const double2 *X, const int *scell, cons
     k_nl_build<<<nblocks(g.N), kBlock, 0, s>>>(X, scell, cstart, g, k, maxnb, nl, nbcnt, flags);
 }
 void launch_density_f64(hipStream_t s, const double2 *X, const int *nl, const int *nbcnt, Grid g,
-                        PhysF64 P, double2 *PI, double *rho, double2 *tapeX) {
-  if (g.N > 0) k_density<<<nblocks(g.N), kBlock, 0, s>>>(X, nl, nbcnt, g, P, PI, rho, tapeX);
+                        PhysF64 P, const EpochCells &ec, double2 *PI, double *rho,
+                        double2 *tapeX) {
+  if (g.N > 0)
+    k_density<<<nblocks(g.N), kBlock, 0, s>>>(X, nl, nbcnt, g, P, ec, PI, rho, tapeX);
 }
 void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF64 P,
-                      double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
-                      double *alpha_out) {
+                      const EpochCells &ec, double2 *Xn, double2 *Vn, double2 *tapeV,
+                      double2 *a_out, double *alpha_out) {
   if (g.N > 0)
-    k_force<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, theta, nl, nbcnt, g, P, Xn, Vn, tapeV, a_out,
-                                            alpha_out);
+    k_force<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, theta, nl, nbcnt, g, P, ec, Xn, Vn, tapeV,
+                                            a_out, alpha_out);
 }
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
-                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
-                      double2 *gX, double2 *gV, double2 *abar, double *galpha,
+                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                      double2 *PI, double2 *gX, double2 *gV, double2 *abar, double *galpha,
                       const LevelArgs &L) {
   if (g.N > 0)
-    k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, nl, nbcnt, g, P, PI, gX, gV, abar,
-                                            galpha, L);
+    k_adj_r<<<nblocks(g.N), kBlock, 0, s>>>(X, Vnext, theta, nl, nbcnt, g, P, ec, PI, gX, gV,
+                                            abar, galpha, L);
 }
 void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
                         const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
@@ -458,15 +668,16 @@ void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
 }
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
-                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
-                       double2 *gV, double *grho) {
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                       double2 *gXs, double2 *gV, double *grho) {
   if (g.N > 0)
-    k_adj_p1<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, abar, galpha, theta, nl, nbcnt, g, P, gXs,
-                                             gV, grho);
+    k_adj_p1<<<nblocks(g.N), kBlock, 0, s>>>(X, V, PI, abar, galpha, theta, nl, nbcnt, g, P, ec,
+                                             gXs, gV, grho);
 }
 void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
-                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gX) {
-  if (g.N > 0) k_adj_p2<<<nblocks(g.N), kBlock, 0, s>>>(X, grho, gXs, nl, nbcnt, g, P, gX);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                       double2 *gX) {
+  if (g.N > 0) k_adj_p2<<<nblocks(g.N), kBlock, 0, s>>>(X, grho, gXs, nl, nbcnt, g, P, ec, gX);
 }
 int objective_blocks(int N) { return N > 0 ? nblocks(N) : 1; }
 void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, double yc,

@@ -187,12 +187,19 @@ void launch_copy2(hipStream_t s, const double2 *src, int N, double2 *dst);
 // FP64 physics (sph_f64.hip)
 void launch_nl_build_f64(hipStream_t s, const double2 *X, const int *scell, const int *cstart,
                          Grid g, CellConstF64 k, int maxnb, int *nl, int *nbcnt, int *flags);
+// The layout of the list's epoch, for the LDS staging of the FP64 pair
+// passes: each slot's binning cell (bcell) and sweep cell (scell), and the
+// cell starts.
+struct EpochCells {
+  const int *bcell, *scell, *cstart;
+};
 void launch_density_f64(hipStream_t s, const double2 *X, const int *nl, const int *nbcnt, Grid g,
-                        PhysF64 P, double2 *PI, double *rho, double2 *tapeX);
+                        PhysF64 P, const EpochCells &ec, double2 *PI, double *rho,
+                        double2 *tapeX);
 void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                       const double *theta, const int *nl, const int *nbcnt, Grid g, PhysF64 P,
-                      double2 *Xn, double2 *Vn, double2 *tapeV, double2 *a_out,
-                      double *alpha_out);
+                      const EpochCells &ec, double2 *Xn, double2 *Vn, double2 *tapeV,
+                      double2 *a_out, double *alpha_out);
 // what an R pass needs to raise its step's gtheta sweep level (raise_level):
 // each slot's binning cell at the epoch's rebuild, the level word and the
 // step's stamp, and the theta grid the gather sweeps for
@@ -208,18 +215,19 @@ __host__ __device__ inline int margin_beyond(int c, int2 w) {
   return c < w.x ? w.x + 1 - c : (c > w.y ? c - w.y + 1 : 0);
 }
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
-                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *PI,
-                      double2 *gX, double2 *gV, double2 *abar, double *galpha,
+                      const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                      double2 *PI, double2 *gX, double2 *gV, double2 *abar, double *galpha,
                       const LevelArgs &L);
 void launch_adj_rpw_f64(hipStream_t s, const double2 *X, const double2 *Vnext,
                         const double *theta, Grid g, PhysF64 P, double2 *gX, double2 *gV,
                         double2 *abar, double *galpha, const LevelArgs &L);
 void launch_adj_p1_f64(hipStream_t s, const double2 *X, const double2 *V, const double2 *PI,
                        const double2 *abar, const double *galpha, const double *theta,
-                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gXs,
-                       double2 *gV, double *grho);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                       double2 *gXs, double2 *gV, double *grho);
 void launch_adj_p2_f64(hipStream_t s, const double2 *X, const double *grho, const double2 *gXs,
-                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, double2 *gX);
+                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
+                       double2 *gX);
 // J = the fixed-order sum of nblocks partials (partials: nblocks scratch
 // doubles, at least 2); seed (may be NULL) in slot order, or in particle-id
 // order through ids when ids is given
