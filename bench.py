@@ -168,16 +168,19 @@ class Form:
         S, be = R.S, R.be
         self.R, self.name, self.H = R, name, H
         self.precision = name.split("_")[0]
-        self.ckpt = name.endswith("_ckpt")
+        self.ckpt = "_ckpt" in name
+        # "_s5": the list builds sweep 5x5 half cells over quadrant-ordered cells
+        self.sweep = 5 if name.endswith("_s5") else 3
         self.ws = S.AdjointWorkspace(be, R.N, R.p, precision=self.precision)
         if self.ckpt:
             E = math.ceil(H / R.args.interval)
             seg = R.args.seg_epochs or max(1, math.ceil(E / (R.args.snapshots + 1)))
             self.ck = S.Checkpointer(be, R.N, H, R.p, snapshots=R.args.snapshots, seg_epochs=seg,
                                      interval=R.args.interval, precision=self.precision)
+            S.set_sweep_(self.ck, self.sweep)
         else:
             keep = {}
-            if name == "f32c":
+            if name in ("f32c", "f32c_s5"):
                 keep = dict(keep_pi=True, keep_lists=True)
             elif name == "f64_keep":
                 keep = dict(keep_pi=True)
@@ -193,6 +196,7 @@ class Form:
             gc.collect()
             self.st = S.State(be, R.sc.X0, R.sc.V0, R.p, interval=R.args.interval,
                               precision=self.precision)
+            S.set_sweep_(self.st, self.sweep)
             S.reset_(self.tape)
         be.synchronize()
         if profile:
@@ -226,6 +230,7 @@ class Form:
         R, S, be = self.R, self.R.S, self.R.be
         st = S.State(be, R.sc.X0, R.sc.V0, R.p, interval=R.args.interval,
                      precision=self.precision)
+        S.set_sweep_(st, self.sweep)
         be.synchronize()
         t0 = time.perf_counter()
         f = S.api._field(R.theta, R.p, be)
@@ -507,6 +512,12 @@ def main():
                 r = ratios(a, b)
                 speed[f"{a}/{b}"] = {"median": round(statistics.median(r), 3),
                                      "min": round(min(r), 3), "max": round(max(r), 3)}
+    for a in names:
+        b = a[:-3]
+        if a.endswith("_s5") and b in names:     # the half-cell sweep against the 3x3 one
+            r = ratios(a, b)
+            speed[f"{a}/{b}"] = {"median": round(statistics.median(r), 3),
+                                 "min": round(min(r), 3), "max": round(max(r), 3)}
     roofline = None
     if kerns:
         roofline = roofline_of(best, *kerns[best], N)

@@ -238,6 +238,22 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
 int sph_tuning_get(const sph_state *st, int32_t *sizes);
 int sph_tuning_set(sph_state *st, const int32_t *sizes);
 
+/* Candidate sweep of the f32c form's neighbour-list builds (no counterpart in
+ * the reference, whose _nl_build! sweeps the 3x3 cells of width 2h(1+skin),
+ * src/neighbors.jl:192-245; its README lists h-wide cells with a 5x5 sweep as
+ * a possible change, README.md:429,795-796).  sweep 5: from the next rebuild
+ * on, each cell's slots are ordered by quadrant (half cell in x, then in y),
+ * then id, and the mask list is built from the 5x5 half cells around each
+ * slot's quadrant -- about 52 candidates instead of 75, the same list (every
+ * pair closer than the cell width lies in the half-cell block).  The pair
+ * passes walk the same masks; sums within a cell follow the quadrant order, so
+ * the f32c results change at rounding level.  sph_cell_list_get and
+ * sph_neighbors_get still report the reference's order.  Checkpoint
+ * snapshots keep the order of the layout they hold.  One-scene f32c states
+ * with N < 2^29; sweep 3 (the default) restores the (cell, id) order at the
+ * next rebuild. */
+int sph_state_set_sweep(sph_state *st, int sweep);
+
 /* --- binomial checkpointing of the time loop.  No counterpart in the
  * reference, whose Tape keeps every step (src/driver.jl:38-65) and whose README
  * recommends recompute checkpointing for long horizons.  The horizon is cut

@@ -7,8 +7,8 @@ with Julia's ``!`` written as a trailing underscore.
 from .api import (AdjointWorkspace, BatchField, BatchState, BatchWorkspace, CellList,
                   Checkpointer, DeviceArray, Field, MI355XBackend, NeighborList, SPHParams,
                   State, Tape, autotune_, backward_, batch_objective, build_, build_neighbors_,
-                  capacity, check_neighbor_flags, ckpt_plan, reset_, simulate_, simulate_graph_,
-                  step_, target_objective, tuning)
+                  capacity, check_neighbor_flags, ckpt_plan, reset_, set_sweep_, simulate_,
+                  simulate_graph_, step_, target_objective, tuning)
 from .design import DesignChain, Optimizer, Sensitivity
 
 __all__ = ["SPHParams", "State", "CellList", "NeighborList", "AdjointWorkspace", "Tape",
@@ -16,4 +16,4 @@ __all__ = ["SPHParams", "State", "CellList", "NeighborList", "AdjointWorkspace",
            "reset_", "capacity", "check_neighbor_flags", "MI355XBackend", "Field", "Checkpointer",
            "ckpt_plan", "DeviceArray", "BatchState", "BatchWorkspace", "BatchField",
            "batch_objective", "simulate_graph_", "autotune_",
-           "tuning", "DesignChain", "Optimizer", "Sensitivity"]
+           "tuning", "set_sweep_", "DesignChain", "Optimizer", "Sensitivity"]

@@ -89,6 +89,7 @@ SIGNATURES = {
     "sph_autotune": (C.c_int, [_vp, _vp, _i64, _vp]),
     "sph_tuning_get": (C.c_int, [_vp, _vp]),
     "sph_tuning_set": (C.c_int, [_vp, _vp]),
+    "sph_state_set_sweep": (C.c_int, [_vp, C.c_int]),
     "sph_ckpt_create": (C.c_int, [_vp, _PP, C.c_int, _i64, _i64, _i64, _i64, _d, _i64, _i64,
                                   _pp]),
     "sph_ckpt_destroy": (C.c_int, [_vp]),

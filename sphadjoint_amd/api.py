@@ -468,6 +468,17 @@ def tuning(st: State, sizes=None):
     return dict(zip(names, out.tolist()))
 
 
+def set_sweep_(st, sweep):
+    """The candidate sweep of an f32c state's neighbour-list builds
+    (sph_state_set_sweep): 5 orders each cell's slots by quadrant and builds
+    the list from the 5x5 half cells around each slot, 3 (the default) is the
+    reference's 3x3 sweep.  Takes a State or a Checkpointer (its work state);
+    it applies from the next rebuild."""
+    if isinstance(st, Checkpointer):
+        st = st.state
+    L.call("sph_state_set_sweep", st._h, int(sweep))
+
+
 class AdjointWorkspace:
     """AdjointWorkspace (src/driver.jl:117-133): gX, gV, gtheta after backward_."""
 

@@ -6,6 +6,7 @@
 // everything per particle runs in the kernels of sph_cells.hip / sph_f64.hip.
 #include "sphadj.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -481,6 +482,14 @@ struct sph_state {
   // the next step's density pass builds it (set by the checkpointer, which
   // rebuilds before a snapshot and restores the rebuilt layout)
   bool pending_list = false;
+  // f32c form, one scene: sweep 5 has each rebuild order every cell's slots by
+  // quadrant (cell_quadrant) and build the mask list by the 5x5 half-cell
+  // sweep; sweep 3 (the default) keeps the reference's (cell, id) order and
+  // the 3x3 sweep.  layout_quad: the current layout is quadrant-ordered, and
+  // qs holds its quadrant starts (4 ncell + 1 entries).
+  int sweep = 3;
+  bool layout_quad = false;
+  int *qs = nullptr;
   // current layout; for the f32c form scell holds the binning cell of each
   // slot, for the FP64 form bcell does (scell is the sweep cell there)
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr, *bcell = nullptr;
@@ -595,7 +604,7 @@ void state_free(sph_state *st) {
                   &st->cursor, &st->tmp_id, &st->tmp_slot, &st->perm, &st->new_ids, &st->bsum,
                   &st->bigcells,
                   &st->flags, &st->nl_idx, &st->nl_cnt, &st->nl, &st->nbcnt, &st->lane,
-                  &st->bcell};
+                  &st->bcell, &st->qs};
   for (auto pp : ints) dfree(*pp);
   double2 **d2[] = {&st->X, &st->V, &st->Xn, &st->Vn, &st->Xref, &st->Xref2, &st->PI, &st->a_id,
                     &st->pi_id, &st->a_slot, &st->tmp2, &st->seed_slot};
@@ -613,7 +622,9 @@ void state_free(sph_state *st) {
 }
 
 // build! (src/neighbors.jl:97-114): cell list of the current X into the cl buffers.
-void do_build(sph_state *st) {
+// build! (src/neighbors.jl:97-114); with quad_keys each cell's entries are
+// ranked by (quadrant, id) instead of id, for a quadrant-ordered layout
+void do_build(sph_state *st, bool quad_keys = false) {
   hipStream_t s = st->ctx->stream;
   const int N = (int)st->N;
   sph_ctx *c = st->ctx;
@@ -628,11 +639,14 @@ void do_build(sph_state *st) {
                      st->bigcells);
   });
   timed(c, K_FILL, [&] {
-    launch_fill(s, st->cellof, st->ids, N, st->cursor, st->tmp_id, st->tmp_slot);
+    if (quad_keys)
+      launch_fill(s, st->cellof, st->ids, N, st->cursor, st->tmp_id, st->tmp_slot, st->X, st->g,
+                  st->kc.cs);
+    else launch_fill(s, st->cellof, st->ids, N, st->cursor, st->tmp_id, st->tmp_slot);
   });
   timed(c, K_RANK, [&] {
     launch_rank(s, st->cellof, st->cl_cstart, st->tmp_id, st->tmp_slot, N, st->bigcells,
-                st->perm, st->new_ids);
+                st->perm, st->new_ids, quad_keys ? st->ids : nullptr);
   });
   st->cl_view = 1;  // the cell list of this build is (cl_cstart, new_ids)
 }
@@ -670,7 +684,8 @@ void materialise_aux(sph_state *st) {
 // form) the mask list is left to the step's fused density pass.
 void do_build_neighbors(sph_state *st, bool defer_list = false) {
   hipStream_t s = st->ctx->stream;
-  do_build(st);
+  const bool quad = st->sweep == 5;
+  do_build(st, quad);
   int check = (st->age != kAgeNever) ? 1 : 0;
   if (st->precision == SPH_F32C) {
     timed(st->ctx, K_PERMUTE, [&] {
@@ -690,12 +705,14 @@ void do_build_neighbors(sph_state *st, bool defer_list = false) {
   std::swap(st->ids, st->new_ids);
   std::swap(st->cstart, st->cl_cstart);
   st->cl_view = 2;  // the cell list of this build is now (cstart, ids) of the layout
+  st->layout_quad = quad;
+  if (quad) launch_quad_starts(s, st->X, st->cstart, st->g, st->kc.cs, st->qs);
   timed(st->ctx, K_NLBUILD, [&] {
     if (st->precision == SPH_F32C) {
       // (the permute above has dealt the new thread order)
       if (!defer_list)
         launch_nm_build_f32c(s, st->xr, st->scell, st->cstart, st->bs.at(TK_NLBUILD), st->nm,
-                             st->flags);
+                             st->flags, quad ? st->qs : nullptr);
     } else
       launch_nl_build_f64(s, st->X, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb, st->nl,
                           st->nbcnt, st->flags);
@@ -781,7 +798,8 @@ void do_step(sph_state *st, const double *theta, sph_tape *tp) {
     timed(st->ctx, K_DENSITY, [&] {
       if (fused)
         launch_density_build_f32c(s, st->xr, st->scell, st->cstart, st->lane,
-                                  st->bs.at(TK_DENSITY), PI32, st->nm, em, st->flags);
+                                  st->bs.at(TK_DENSITY), PI32, st->nm, em, st->flags,
+                                  st->layout_quad ? st->qs : nullptr);
       else
         launch_density_f32c(s, st->xr, st->scell, st->nm, st->cstart, st->lane,
                             st->bs.at(TK_DENSITY), PI32, nullptr, em);
@@ -1488,6 +1506,10 @@ int sph_cell_list_get(sph_state *st, int32_t *cellof, int32_t *counts, int32_t *
     HIPC(hipMemcpyAsync(ord.data(), dorder, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     for (size_t c = 0; c < nc; ++c) cnt[c] = stt[c + 1] - stt[c];
+    // a quadrant-ordered layout holds each cell's ids by quadrant: the
+    // reference's order is ascending id
+    if (st->cl_view == 2 && st->layout_quad)
+      for (size_t c = 0; c < nc; ++c) std::sort(ord.begin() + stt[c], ord.begin() + stt[c + 1]);
     if (counts) std::memcpy(counts, cnt.data(), sizeof(int) * nc);
     if (starts) std::memcpy(starts, stt.data(), sizeof(int) * nc);
     if (order)
@@ -1512,10 +1534,35 @@ int sph_neighbors_get(sph_state *st, int32_t *indices, int32_t *counts) {
     HIPC(hipMemsetAsync(st->nl_idx, 0, sizeof(int) * n * st->maxnb, s));
     launch_materialise(s, st->Xref, st->ids, st->scell, st->cstart, st->g, st->kc, (int)st->maxnb,
                        st->sm, st->si, st->nl_idx, st->nl_cnt, st->flags);
+    std::vector<int> cnt_h;
     if (indices)
       HIPC(hipMemcpyAsync(indices, st->nl_idx, sizeof(int) * n * st->maxnb, hipMemcpyDeviceToHost, s));
-    if (counts) HIPC(hipMemcpyAsync(counts, st->nl_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    if (counts || (indices && st->layout_quad)) {
+      cnt_h.resize(n);
+      HIPC(hipMemcpyAsync(cnt_h.data(), st->nl_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    }
     HIPC(hipStreamSynchronize(s));
+    if (counts) std::memcpy(counts, cnt_h.data(), sizeof(int) * n);
+    if (indices && st->layout_quad) {
+      // a quadrant-ordered layout lists each cell's entries by quadrant; the
+      // reference's sweep visits cells in ascending index and ids ascending in
+      // each, i.e. the order of (cell, id).  (When a list overflows maxnb the
+      // entries kept are the first maxnb of the quadrant order.)
+      std::vector<int> sc(n), idv(n), cell_of(n);
+      HIPC(hipMemcpy(sc.data(), st->scell, sizeof(int) * n, hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(idv.data(), st->ids, sizeof(int) * n, hipMemcpyDeviceToHost));
+      for (size_t t = 0; t < n; ++t) cell_of[idv[t]] = sc[t];
+      std::vector<int> e;
+      for (size_t i = 0; i < n; ++i) {
+        e.resize(cnt_h[i]);
+        for (int m = 0; m < cnt_h[i]; ++m) e[m] = indices[m * st->sm + (int64_t)i * st->si];
+        std::sort(e.begin(), e.end(), [&](int a, int b) {
+          const int ca = cell_of[a - 1], cb = cell_of[b - 1];
+          return ca != cb ? ca < cb : a < b;
+        });
+        for (int m = 0; m < cnt_h[i]; ++m) indices[m * st->sm + (int64_t)i * st->si] = e[m];
+      }
+    }
   });
 }
 
@@ -2536,6 +2583,7 @@ struct sph_ckpt {
     float2 *V32 = nullptr;
     int *ids = nullptr;
     int *scell = nullptr, *cstart = nullptr, *lane = nullptr;   // f32c form: the layout
+    bool quad = false;                                           // ... quadrant-ordered
   };
   std::vector<Snap> snaps;              // slot 0: the initial state; 1..nsnap: schedule
   std::vector<int> slot_of;             // epoch -> slot (-1: none)
@@ -2584,6 +2632,7 @@ void ck_copy(sph_ckpt *ck, bool to_snap, int slot) {
       launch_snapshot_f32c(s, true, st->bs.B, st->X, st->V32, st->ids, st->scell, st->lane,
                            st->cstart, st->Xref, st->xr, q.X, q.V32, q.ids, q.scell, q.lane,
                            q.cstart);
+      q.quad = st->layout_quad;
     } else {
       cp(q.X, st->X, sizeof(double2) * n);
       cp(q.V, st->V, sizeof(double2) * n);
@@ -2596,6 +2645,8 @@ void ck_copy(sph_ckpt *ck, bool to_snap, int slot) {
       launch_snapshot_f32c(s, false, st->bs.B, st->X, st->V32, st->ids, st->scell, st->lane,
                            st->cstart, st->Xref, st->xr, q.X, q.V32, q.ids, q.scell, q.lane,
                            q.cstart);
+      st->layout_quad = q.quad;
+      if (q.quad) launch_quad_starts(s, st->X, st->cstart, st->g, st->kc.cs, st->qs);
       st->age = 0;
       st->pending_list = true;
       st->cl_view = 2;
@@ -2721,6 +2772,20 @@ int sph_tuning_set(sph_state *st, const int32_t *sizes) {
         fail(SPH_E_ARG, "launch configuration %d is not +-64, +-128 or +-256", (int)sizes[k]);
     }
     for (int k = 0; k < TK_COUNT; ++k) st->bs.cfg[k] = v[k];
+  });
+}
+
+int sph_state_set_sweep(sph_state *st, int sweep) {
+  return guarded([&] {
+    if (!st) fail(SPH_E_ARG, "NULL argument");
+    if (sweep != 3 && sweep != 5) fail(SPH_E_ARG, "sweep must be 3 or 5, not %d", sweep);
+    if (sweep == 5) {
+      if (st->precision != SPH_F32C) fail(SPH_E_ARG, "the 5x5 half-cell sweep is an f32c build");
+      if (is_batched(st)) fail(SPH_E_ARG, "the 5x5 half-cell sweep takes one scene");
+      if (st->N >= (1LL << 29)) fail(SPH_E_ARG, "the 5x5 half-cell sweep takes N < 2^29");
+      if (!st->qs) st->qs = dalloc<int>(4 * (size_t)st->g.ncell + 1);
+    }
+    st->sweep = sweep;
   });
 }
 

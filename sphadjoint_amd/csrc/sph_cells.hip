@@ -134,7 +134,11 @@ __global__ void k_scan_final(int *__restrict__ in, int n, const int *__restrict_
   }
 }
 
+// Entries of each cell at their cell's cursor: the rank key (the particle id,
+// or with qX the quadrant-major key quadrant * N + id of the half-cell sweep)
+// and the slot.
 __global__ void k_fill(const int *__restrict__ cellof, const int *__restrict__ ids, int N,
+                       const double2 *__restrict__ qX, Grid g, double cs,
                        int *__restrict__ cursor, int *__restrict__ tmp_id,
                        int *__restrict__ tmp_slot) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,23 +151,55 @@ __global__ void k_fill(const int *__restrict__ cellof, const int *__restrict__ i
   base = __shfl(base, head_lane, 64);
   if (c >= 0) {
     int pos = base + (lane - head_lane);
-    tmp_id[pos] = ids[s];
+    int key = ids[s];
+    if (qX) {
+      const int cy = c / g.nx;
+      key += cell_quadrant(qX[s], cs, c - cy * g.nx, cy) * N;
+    }
+    tmp_id[pos] = key;
     tmp_slot[pos] = s;
   }
 }
 
-// Rank of one big cell's entries [b, e) by particle id (ids are distinct): a
-// workgroup sets one bit per id in an LDS bitmap of a window of kRankWin ids,
-// prefixes the popcounts of its words, and places each entry at b + (ids
-// below it); windows cover the cell's id span [lo, hi] in turn.  O(entries +
-// span / 32) per window instead of O(entries^2); the same ranks, hence the
-// same order, as the per-entry count.
+// First slot of each quadrant of each cell, for a layout whose cells hold
+// their slots in quadrant order: qs[4c + k] (k = 0..3; qs[4c] = cstart[c]),
+// qs[4 ncell] = N.  Thread c binary-searches cell c's slots by the quadrant
+// of their position X (the positions the layout was sorted by).
+__global__ void k_quad_starts(const double2 *__restrict__ X, const int *__restrict__ cstart,
+                              Grid g, double cs, int *__restrict__ qs) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > g.ncell) return;
+  if (c == g.ncell) {
+    qs[4 * c] = g.N;
+    return;
+  }
+  const int b = cstart[c], e = cstart[c + 1];
+  const int cy = c / g.nx, cx = c - cy * g.nx;
+  qs[4 * c] = b;
+  for (int k = 1; k < 4; ++k) {
+    int lo = b, hi = e;
+    while (lo < hi) {
+      const int mid = lo + ((hi - lo) >> 1);
+      if (cell_quadrant(X[mid], cs, cx, cy) < k) lo = mid + 1;
+      else hi = mid;
+    }
+    qs[4 * c + k] = lo;
+  }
+}
+
+// Rank of one big cell's entries [b, e) by key (distinct: ids, or
+// quadrant-major keys): a workgroup sets one bit per key in an LDS bitmap of a
+// window of kRankWin keys, prefixes the popcounts of its words, and places
+// each entry at b + (keys below it); windows cover the cell's key span
+// [lo, hi] in turn.  O(entries + span / 32) per window instead of
+// O(entries^2); the same ranks, hence the same order, as the per-entry count.
 constexpr int kRankWords = 2048;
 constexpr int kRankWin = 32 * kRankWords;
 constexpr int kRankWords_t = kRankWords / kBlock;   // words per thread in the prefix
 __device__ void rank_big_cell(int b, int e, const int *__restrict__ tmp_id,
-                              const int *__restrict__ tmp_slot, int *__restrict__ perm,
-                              int *__restrict__ new_ids, unsigned *bits, int *pre) {
+                              const int *__restrict__ tmp_slot, const int *__restrict__ ids,
+                              int *__restrict__ perm, int *__restrict__ new_ids, unsigned *bits,
+                              int *pre) {
   __shared__ int red[2 * (kBlock / 64)];
   const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
   int lo = INT_MAX, hi = INT_MIN;
@@ -214,8 +250,9 @@ __device__ void rank_big_cell(int b, int e, const int *__restrict__ tmp_id,
       if (d >= 0 && d < kRankWin) {
         const int wd = (int)(d >> 5);
         const int t = b + base + pre[wd] + __popc(bits[wd] & ((1u << (d & 31)) - 1u));
-        perm[t] = tmp_slot[q];
-        new_ids[t] = id;
+        const int so = tmp_slot[q];
+        perm[t] = so;
+        new_ids[t] = ids ? ids[so] : id;
       }
     }
     base += total;
@@ -223,14 +260,17 @@ __device__ void rank_big_cell(int b, int e, const int *__restrict__ tmp_id,
   }
 }
 
-// Rank inside the cell by particle id: entry p of a cell with at most
-// kRankSmall entries counts the smaller ids of its cell; the last kRankBig
-// workgroups take the big cells listed by the scan (big), one at a time.
+// Rank inside the cell by key (the particle id, or a quadrant-major key):
+// entry p of a cell with at most kRankSmall entries counts the smaller keys of
+// its cell; the last kRankBig workgroups take the big cells listed by the scan
+// (big), one at a time.  new_ids receives the ids: the keys themselves, or
+// ids[slot] when the keys are not ids.
 constexpr int kRankBig = 64;
 __global__ void __launch_bounds__(kBlock)
 k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
        const int *__restrict__ tmp_id, const int *__restrict__ tmp_slot, int N,
-       const int *__restrict__ big, int *__restrict__ perm, int *__restrict__ new_ids) {
+       const int *__restrict__ big, const int *__restrict__ ids, int *__restrict__ perm,
+       int *__restrict__ new_ids) {
   __shared__ unsigned bits[kRankWords];
   __shared__ int pre[kRankWords];
   const int nb = (N + kBlock - 1) / kBlock;
@@ -238,7 +278,7 @@ k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
     const int nbig = big[0];
     for (int k = (int)blockIdx.x - nb; k < nbig; k += kRankBig) {
       const int c = big[1 + k];
-      rank_big_cell(cstart[c], cstart[c + 1], tmp_id, tmp_slot, perm, new_ids, bits, pre);
+      rank_big_cell(cstart[c], cstart[c + 1], tmp_id, tmp_slot, ids, perm, new_ids, bits, pre);
     }
     return;
   }
@@ -253,7 +293,7 @@ k_rank(const int *__restrict__ cellof, const int *__restrict__ cstart,
   for (int q = b; q < e; ++q) rank += (tmp_id[q] < id) ? 1 : 0;
   int t = b + rank;
   perm[t] = so;
-  new_ids[t] = id;
+  new_ids[t] = ids ? ids[so] : id;
 }
 
 __global__ void k_permute(const int *__restrict__ perm, const int *__restrict__ cellof,
@@ -361,14 +401,20 @@ void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int 
   k_scan_final<<<nb, kBlock, 0, s>>>(in, n, bsum, out, cursor, big);
 }
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
-                 int *tmp_id, int *tmp_slot) {
-  if (N > 0) k_fill<<<nblocks(N), kBlock, 0, s>>>(cellof, ids, N, cursor, tmp_id, tmp_slot);
+                 int *tmp_id, int *tmp_slot, const double2 *qX, Grid g, double cs) {
+  if (N > 0)
+    k_fill<<<nblocks(N), kBlock, 0, s>>>(cellof, ids, N, qX, g, cs, cursor, tmp_id, tmp_slot);
 }
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
-                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids) {
+                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids,
+                 const int *ids) {
   if (N > 0)
     k_rank<<<nblocks(N) + kRankBig, kBlock, 0, s>>>(cellof, cstart, tmp_id, tmp_slot, N, big,
-                                                    perm, new_ids);
+                                                    ids, perm, new_ids);
+}
+void launch_quad_starts(hipStream_t s, const double2 *X, const int *cstart, Grid g, double cs,
+                        int *qs) {
+  k_quad_starts<<<nblocks(g.ncell + 1), kBlock, 0, s>>>(X, cstart, g, cs, qs);
 }
 void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                     const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,

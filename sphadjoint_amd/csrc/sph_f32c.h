@@ -114,14 +114,16 @@ constexpr long long kMaxSlotsF32C = 0xFFFFFFFFLL / (3 * kMaskWords * 4);
 // (flags[2] = 1 when a position is not finite)
 void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *cellof,
                         int *counts, int *flags);
+// mask-list build; with qs (the quadrant starts of a quadrant-ordered
+// single-scene layout, launch_quad_starts) the 5x5 half-cell sweep
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          const Batch &B, uint32_t *nm, int *flags);
+                          const Batch &B, uint32_t *nm, int *flags, const int *qs = nullptr);
 // first density pass after a rebuild, fused with the mask-list build: writes
 // the mask list nm, the exact list em and PI (flags is not written: no pair is
-// ever left out, see kMaskRun)
+// ever left out, see kMaskRun); qs as for launch_nm_build_f32c
 void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
                                const int *cstart, const int *lane, const Batch &B, float2 *PI,
-                               uint32_t *nm, uint32_t *em, int *flags);
+                               uint32_t *nm, uint32_t *em, int *flags, const int *qs = nullptr);
 // density + EOS over the mask list nm; when em is given, the exact list of
 // these positions (pairs with r2 < (2h)^2) is written there for the force pass
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,

@@ -220,6 +220,54 @@ __device__ __forceinline__ RowRun row_run(Rsrc cst, int cx, int jy, const Grid &
   return r;
 }
 
+// The candidates of row r (cell row jy, 0 below the slot's row, 2 above) in
+// the 5x5 half-cell sweep of a list build over a quadrant-ordered layout (qs:
+// the quadrant starts, sph_cells.hip): cand(b, xs) for each offset b < len of
+// the row run in the slot's half-cell block (cell_quadrant), ascending, with
+// xs the slot's x in the frame of b's cell.  (qx, qy): the slot's quadrant.
+// A row whose both halves are in the block is one range of the run (less the
+// half column away from the slot); the other row is the halves next to the
+// slot of its two or three cells.
+template <class Cand>
+__device__ __forceinline__ void half_cell_row(Rsrc qsr, const RowRun &run, int len, int cx,
+                                              int jy, int r, int qx, int qy, const Grid &g,
+                                              float xm, float xi, float xp, Cand &&cand) {
+  const int nx = g.nx;
+  if (r == 1 || (r == 0) == (qy == 0)) {
+    int a = 0, e = len;
+    if (qx && cx > 0) a = (int)ldu(qsr, (uint32_t)(4 * (jy * nx + cx - 1) + 2) * 4u) - run.rb;
+    if (!qx && cx < nx - 1) {
+      const int t = (int)ldu(qsr, (uint32_t)(4 * (jy * nx + cx + 1) + 2) * 4u) - run.rb;
+      e = t < e ? t : e;
+    }
+    for (int b = a; b < e; ++b) cand(b, b < run.s0 ? xm : (b < run.s1 ? xi : xp));
+  } else {
+    const int qyr = r == 0 ? 1 : 0;   // the half of row r next to the slot
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < nx - 1 ? cx + 1 : nx - 1;
+    for (int cc = x0; cc <= x1; ++cc) {
+      const int d = cc - cx;
+      const int cmask = d < 0 ? (qx ? 2 : 3) : (d > 0 ? (qx ? 3 : 1) : 3);   // halves in x
+      const float xs = d < 0 ? xm : (d > 0 ? xp : xi);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!((cmask >> h) & 1)) continue;
+        const uint32_t qi = (uint32_t)(4 * (jy * nx + cc) + 2 * h + qyr);
+        const int a = (int)ldu(qsr, qi * 4u) - run.rb;
+        int e = (int)ldu(qsr, (qi + 1u) * 4u) - run.rb;
+        e = e < len ? e : len;
+        for (int b = a; b < e; ++b) cand(b, xs);
+      }
+    }
+  }
+}
+
+// quadrant of slot s in its cell c from the quadrant starts
+__device__ __forceinline__ int slot_quadrant(Rsrc qsr, int s, int c) {
+  const uint32_t q = (uint32_t)(4 * c);
+  return (s >= (int)ldu(qsr, (q + 1u) * 4u) ? 1 : 0) + (s >= (int)ldu(qsr, (q + 2u) * 4u) ? 1 : 0) +
+         (s >= (int)ldu(qsr, (q + 3u) * 4u) ? 1 : 0);
+}
+
 // Bitmask list: word w (32 bits) of row r of slot s sits at
 // nm[(w*3 + r)*N + s]; words beyond ceil(len/32) are neither written nor read.
 // The masks of a run longer than kMaskRun slots cover its first kMaskRun
@@ -366,10 +414,14 @@ struct NbXG {
   float gr;
 };
 
+// Mask-list build: the 3x3 sweep over every slot of the three row runs, or
+// with qs (kSub) the 5x5 half-cell sweep over a quadrant-ordered layout (the
+// same list: every list neighbour lies in the half-cell block).
+template <bool kSub>
 __global__ void __launch_bounds__(kBlock)
 k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-             const int *__restrict__ cstart, Batch B, uint32_t *__restrict__ nm,
-             int *__restrict__ flags) {
+             const int *__restrict__ cstart, const int *__restrict__ qs, Batch B,
+             uint32_t *__restrict__ nm, int *__restrict__ flags) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   const int blk = xcd_block();
@@ -393,6 +445,8 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int bc = bcell[s];
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
+  const Rsrc qsr = rsrc(qs, kSub ? (uint32_t)(4 * B.ncell + 1) * 4u : 0u);
+  const int q = kSub ? slot_quadrant(qsr, s, bc) : 0;
   auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -403,19 +457,40 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
       // masks cover the first kMaskRun slots of the run; the passes test the rest
       const int len = run.len < kMaskRun ? run.len : kMaskRun;
       const int rb = run.rb + ro.o[r];
-      for (int w = 0; w * 32 < len; ++w) {
+      if constexpr (kSub) {
         uint32_t m = 0;
-        const int e = len - 32 * w < 32 ? len - 32 * w : 32;
-        for (int k = 0; k < e; ++k) {
-          const int b = 32 * w + k;
-          const float2 xj = ld(rb + b);
-          const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
-          const float dx = xs - xj.x;
-          const float dy = yi - xj.y;
-          const float r2 = fmaf(dx, dx, dy * dy);
-          if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
+        int wc = 0;   // the word being formed; candidates come in ascending order
+        half_cell_row(qsr, run, len, c.x, jy, r, q >> 1, q & 1, g, xm, xi.x, xp,
+                      [&](int b, float xs) __attribute__((always_inline)) {
+                        for (; wc < (b >> 5); ++wc) {
+                          nm[(size_t)(wc * 3 + r) * B.N + s] = m;
+                          m = 0;
+                        }
+                        const float2 xj = ld(rb + b);
+                        const float dx = xs - xj.x;
+                        const float dy = yi - xj.y;
+                        const float r2 = fmaf(dx, dx, dy * dy);
+                        if (r2 < rc2 && r2 > P.r2min) m |= 1u << (b & 31);
+                      });
+        for (; wc * 32 < len; ++wc) {
+          nm[(size_t)(wc * 3 + r) * B.N + s] = m;
+          m = 0;
         }
-        nm[(size_t)(w * 3 + r) * B.N + s] = m;
+      } else {
+        for (int w = 0; w * 32 < len; ++w) {
+          uint32_t m = 0;
+          const int e = len - 32 * w < 32 ? len - 32 * w : 32;
+          for (int k = 0; k < e; ++k) {
+            const int b = 32 * w + k;
+            const float2 xj = ld(rb + b);
+            const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+            const float dx = xs - xj.x;
+            const float dy = yi - xj.y;
+            const float r2 = fmaf(dx, dx, dy * dy);
+            if (r2 < rc2 && r2 > P.r2min) m |= 1u << k;
+          }
+          nm[(size_t)(w * 3 + r) * B.N + s] = m;
+        }
       }
     }
   };
@@ -550,17 +625,22 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
 // list neighbour (r2min < r2 < rc2, as k_nm_build32), and for those within
 // (2h)^2 adds the density term and sets the exact-list bit, in the order in
 // which k_density32 would visit them -- the same pairs, the same arithmetic,
-// the same sums.  Writes nm, em, PI and the overflow flag.
+// the same sums.  Writes nm, em, PI and the overflow flag.  With qs (kSub)
+// the sweep is the 5x5 half-cell one of k_nm_build32<true>, which visits the
+// list neighbours in the same ascending order.
+template <bool kSub>
 __global__ void __launch_bounds__(kBlock)
 k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
-                  const int *__restrict__ cstart, const int *__restrict__ lane, Batch B,
-                  float2 *__restrict__ PI, uint32_t *__restrict__ nm,
-                  uint32_t *__restrict__ em, int *__restrict__ flags) {
+                  const int *__restrict__ cstart, const int *__restrict__ qs,
+                  const int *__restrict__ lane, Batch B, float2 *__restrict__ PI,
+                  uint32_t *__restrict__ nm, uint32_t *__restrict__ em,
+                  int *__restrict__ flags) {
   __shared__ StageTab T;
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
+  const Rsrc qsr = rsrc(qs, kSub ? (uint32_t)(4 * B.ncell + 1) * 4u : 0u);
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
@@ -571,6 +651,7 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int bc = bcell[s];
   const int2 c = cell_xy(bc, g);
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
+  const int q = kSub ? slot_quadrant(qsr, s, bc) : 0;
   FSum acc(1.f);
   auto rows = [&](const RowOff &ro, auto &&ld) __attribute__((always_inline)) {
 #pragma unroll
@@ -581,27 +662,57 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
       const float yi = xi.y - (float)(r - 1) * P.cs;
       const int len = run.len < kMaskRun ? run.len : kMaskRun;
       const int rb = run.rb + ro.o[r];
-      for (int w = 0; w * 32 < len; ++w) {
+      if constexpr (kSub) {
         uint32_t m = 0, e = 0;
-        const int n = len - 32 * w < 32 ? len - 32 * w : 32;
-        for (int k = 0; k < n; ++k) {
-          const int b = 32 * w + k;
-          const float2 xj = ld(rb + b);
-          const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
-          const float dx = xs - xj.x;
-          const float dy = yi - xj.y;
-          const float r2 = fmaf(dx, dx, dy * dy);
-          if (r2 < rc2 && r2 > P.r2min) {
-            m |= 1u << k;
-            if (r2 < P.r2max) {
-              acc.add(density_term(r2, P));
-              e |= 1u << k;
+        int wc = 0;   // the word being formed; candidates come in ascending order
+        auto flush = [&](int upto) __attribute__((always_inline)) {
+          for (; wc < upto; ++wc) {
+            const size_t widx = (size_t)(wc * 3 + r) * B.N + s;
+            nm[widx] = m;
+            em[widx] = e;
+            m = 0;
+            e = 0;
+          }
+        };
+        half_cell_row(qsr, run, len, c.x, jy, r, q >> 1, q & 1, g, xm, xi.x, xp,
+                      [&](int b, float xs) __attribute__((always_inline)) {
+                        flush(b >> 5);
+                        const float2 xj = ld(rb + b);
+                        const float dx = xs - xj.x;
+                        const float dy = yi - xj.y;
+                        const float r2 = fmaf(dx, dx, dy * dy);
+                        if (r2 < rc2 && r2 > P.r2min) {
+                          m |= 1u << (b & 31);
+                          if (r2 < P.r2max) {
+                            acc.add(density_term(r2, P));
+                            e |= 1u << (b & 31);
+                          }
+                        }
+                      });
+        flush((len + 31) >> 5);
+      } else {
+        for (int w = 0; w * 32 < len; ++w) {
+          uint32_t m = 0, e = 0;
+          const int n = len - 32 * w < 32 ? len - 32 * w : 32;
+          for (int k = 0; k < n; ++k) {
+            const int b = 32 * w + k;
+            const float2 xj = ld(rb + b);
+            const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+            const float dx = xs - xj.x;
+            const float dy = yi - xj.y;
+            const float r2 = fmaf(dx, dx, dy * dy);
+            if (r2 < rc2 && r2 > P.r2min) {
+              m |= 1u << k;
+              if (r2 < P.r2max) {
+                acc.add(density_term(r2, P));
+                e |= 1u << k;
+              }
             }
           }
+          const size_t widx = (size_t)(w * 3 + r) * B.N + s;
+          nm[widx] = m;
+          em[widx] = e;
         }
-        const size_t widx = (size_t)(w * 3 + r) * B.N + s;
-        nm[widx] = m;
-        em[widx] = e;
       }
       // the run's slots past the masks, as the passes over these lists take them
       for (int b = kMaskRun; b < run.len; ++b) {
@@ -1365,14 +1476,21 @@ void launch_assign_f32c(hipStream_t s, const double2 *X, const Batch &B, int *ce
   if (B.N > 0) k_assign32<<<B.nblk, B.bs, 0, s>>>(X, B, cellof, counts, flags);
 }
 void launch_nm_build_f32c(hipStream_t s, const float2 *xr, const int *bcell, const int *cstart,
-                          const Batch &B, uint32_t *nm, int *flags) {
-  if (B.N > 0) k_nm_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, B, nm, flags);
+                          const Batch &B, uint32_t *nm, int *flags, const int *qs) {
+  if (B.N <= 0) return;
+  if (qs) k_nm_build32<true><<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, qs, B, nm, flags);
+  else k_nm_build32<false><<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, nullptr, B, nm, flags);
 }
 void launch_density_build_f32c(hipStream_t s, const float2 *xr, const int *bcell,
                                const int *cstart, const int *lane, const Batch &B, float2 *PI,
-                               uint32_t *nm, uint32_t *em, int *flags) {
-  if (B.N > 0)
-    k_density_build32<<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, lane, B, PI, nm, em, flags);
+                               uint32_t *nm, uint32_t *em, int *flags, const int *qs) {
+  if (B.N <= 0) return;
+  if (qs)
+    k_density_build32<true><<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, qs, lane, B, PI, nm, em,
+                                                    flags);
+  else
+    k_density_build32<false><<<B.nblk, B.bs, 0, s>>>(xr, bcell, cstart, nullptr, lane, B, PI, nm,
+                                                     em, flags);
 }
 void launch_density_f32c(hipStream_t s, const float2 *xr, const int *bcell, const uint32_t *nm,
                          const int *cstart, const int *lane, const Batch &B, float2 *PI,

@@ -94,6 +94,24 @@ __device__ __forceinline__ int cell_coord_i32(double v, int n) {
   return (int)(c - 1);
 }
 
+// Quadrant of position x in its 0-based binning cell (cx, cy), which spans
+// [(cx-1) cs, cx cs) x [(cy-1) cs, cy cs): 2*qx + qy, qx = 1 in the right
+// half, qy = 1 in the upper half (v = x/cs as the binning divides, so a
+// position left of a clamped edge cell counts as left, right of one as
+// right).  The 5x5 half-cell sweep of a list build orders each cell's slots by
+// quadrant.  A slot in the left half (qx = 0) of cell column cx has every
+// position within cs of it in columns cx-1 and cx or in the left half of
+// column cx+1; one in the right half, in the right half of cx-1 or in columns
+// cx and cx+1; rows likewise.  So the sweep scans 5x5 half cells, about 69 %
+// of the 3x3 block's candidates, and misses no pair closer than cs (up to the
+// rounding of v at a half-cell edge, which only concerns pairs at a distance
+// of cs within that rounding).
+__host__ __device__ inline int cell_quadrant(double2 x, double cs, int cx, int cy) {
+  const int qx = (x.x / cs - (double)(cx - 1)) >= 0.5 ? 1 : 0;
+  const int qy = (x.y / cs - (double)(cy - 1)) >= 0.5 ? 1 : 0;
+  return 2 * qx + qy;
+}
+
 // Sweep level of one reverse step's gtheta gather: 0 tight ranges (a quarter
 // cell, f32c form), 1 a one-cell margin, m >= 2 an m-cell margin (the SPH
 // cells holding a theta cell's positions, widened by m cells on each side:
@@ -163,10 +181,21 @@ void launch_assign(hipStream_t s, const double2 *X, Grid g, double cs, int *cell
 // the cells with more than 128 entries in big (big[0] = count, n + 1 entries)
 void launch_scan_excl(hipStream_t s, int *in, int n, int *out, int *cursor, int *bsum,
                       int *big);
+// entries at their cells' cursors with their rank keys: the ids, or with qX
+// (the positions; g, cs the single grid of the state) quadrant * N + id, which
+// orders each cell by quadrant, then id (N < 2^29)
 void launch_fill(hipStream_t s, const int *cellof, const int *ids, int N, int *cursor,
-                 int *tmp_id, int *tmp_slot);
+                 int *tmp_id, int *tmp_slot, const double2 *qX = nullptr, Grid g = Grid{},
+                 double cs = 0.0);
+// rank by key inside each cell; with ids (keys that are not ids), new_ids
+// takes ids[slot] instead of the key
 void launch_rank(hipStream_t s, const int *cellof, const int *cstart, const int *tmp_id,
-                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids);
+                 const int *tmp_slot, int N, const int *big, int *perm, int *new_ids,
+                 const int *ids = nullptr);
+// first slot of each quadrant of each cell of a quadrant-ordered layout:
+// qs[4c + k], k = 0..3, and qs[4 ncell] = N (4 ncell + 1 entries)
+void launch_quad_starts(hipStream_t s, const double2 *X, const int *cstart, Grid g, double cs,
+                        int *qs);
 // also writes bcell[slot] = the slot's binning cell (cellof, the division rule)
 void launch_permute(hipStream_t s, const int *perm, const int *cellof, const double2 *X,
                     const double2 *V, const double2 *Xref, int check, Grid g, CellConstF64 k,
