@@ -744,7 +744,9 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           float2 *__restrict__ tapeV, double2 *__restrict__ a_out,
           double *__restrict__ alpha_out) {
   __shared__ StageTab T;
-  __shared__ float2 sx[kStageCap], spi[kStageCap], sv[kStageCap];
+  // a pair reads (x, pi) as one 16-byte record and v beside it
+  __shared__ float4 sxp[kStageCap];
+  __shared__ float2 sv[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
@@ -757,8 +759,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
                  return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)};
                },
                [&](int i, const NbXPV &d) {
-                 sx[i] = d.x;
-                 spi[i] = d.pi;
+                 sxp[i] = make_float4(d.x.x, d.x.y, d.pi.x, d.pi.y);
                  sv[i] = d.v;
                });
   if (!valid_) return;
@@ -784,7 +785,10 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   // nm is the exact list of the density pass over these positions
   if (T.ok) {
     for_each_nb<false>(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
-                [&](int i) __attribute__((always_inline)) { return NbXPV{sx[i], spi[i], sv[i]}; },
+                [&](int i) __attribute__((always_inline)) {
+                  const float4 q = sxp[i];
+                  return NbXPV{make_float2(q.x, q.y), make_float2(q.z, q.w), sv[i]};
+                },
                 pair);
   } else {
     for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
@@ -1277,7 +1281,8 @@ __global__ void __launch_bounds__(kBlock) k_theta_strip_keyed(ThetaKeyed A, Batc
 // tested and the exact list is written to em for P2.
 struct P1Lds {
   StageTab T;
-  float2 sx[kStageCapP1], spi[kStageCapP1], sab[kStageCapP1], sv[kStageCapP1];
+  // a pair reads (x, pi) and (abar, v) as two 16-byte records
+  float4 sxp[kStageCapP1], sav[kStageCapP1];
 };
 #define P1_PARAMS                                                                           \
   const double2 *__restrict__ X, const float2 *__restrict__ xr,                             \
@@ -1293,7 +1298,7 @@ struct P1Lds {
 template <bool kExact>
 __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   StageTab &T = L.T;
-  float2 *const sx = L.sx, *const spi = L.spi, *const sab = L.sab, *const sv = L.sv;
+  float4 *const sxp = L.sxp, *const sav = L.sav;
   SCENE_WIN_AT(B, lane, blk);
   NB_RSRC;
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapP1);
@@ -1307,10 +1312,8 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
                  return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
                },
                [&](int i, const NbXPAV &d) {
-                 sx[i] = d.x;
-                 spi[i] = d.pi;
-                 sab[i] = d.ab;
-                 sv[i] = d.v;
+                 sxp[i] = make_float4(d.x.x, d.x.y, d.pi.x, d.pi.y);
+                 sav[i] = make_float4(d.ab.x, d.ab.y, d.v.x, d.v.y);
                });
   if (!valid_) return;
   const double2 x64 = X[s];
@@ -1355,7 +1358,9 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   if (T.ok) {
     for_each_nb<false>(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) {
-                  return NbXPAV{sx[i], spi[i], sab[i], sv[i]};
+                  const float4 q = sxp[i], r = sav[i];
+                  return NbXPAV{make_float2(q.x, q.y), make_float2(q.z, q.w),
+                                make_float2(r.x, r.y), make_float2(r.z, r.w)};
                 },
                 pair);
   } else {
