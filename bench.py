@@ -318,16 +318,23 @@ def run_batched(nscenes, steps, seed=0):
     return out
 
 
-def run_long_horizon(R, steps, snapshots, runs=2):
+def run_long_horizon(R, steps, snapshots, runs=2, tape_frac=0.0):
     """The long horizon at N = 1M: `steps` steps of the f32c form forward +
     adjoint under a `snapshots` budget (one-epoch segments); p-s/s, device
-    memory of the checkpointer, recomputed segments, J over repeated runs."""
+    memory of the checkpointer, recomputed segments, J over repeated runs.
+    With tape_frac > 0 the checkpointer's tape may take that share of the free
+    device memory (80 B per particle-step and a layout record per epoch): the
+    last segments that fit are taped in the forward, not recomputed."""
     S, be = R.S, R.be
     gc.collect()
     be.synchronize()
     free0, total = be.mem_info()
+    tape_steps = None
+    if tape_frac > 0:
+        per_step = 80 * R.N + (12 * R.N + 4 * 300000) // R.args.interval
+        tape_steps = max(1, min(steps, int(tape_frac * free0 / per_step)))
     ck = S.Checkpointer(be, R.N, steps, R.p, snapshots=snapshots, seg_epochs=1,
-                        interval=R.args.interval, precision="f32c")
+                        interval=R.args.interval, precision="f32c", tape_steps=tape_steps)
     ws = S.AdjointWorkspace(be, R.N, R.p, precision="f32c")
     be.synchronize()
     free1, _ = be.mem_info()
@@ -345,6 +352,7 @@ def run_long_horizon(R, steps, snapshots, runs=2):
     stats = ck.stats()
     same = all(j == Js[0] for j in Js) and all(np.array_equal(g, gts[0]) for g in gts)
     out = {"steps": steps, "N": R.N, "snapshots": snapshots, "seg_epochs": 1,
+           "tape_steps": tape_steps,
            "fwd_adj_particle_steps_per_s": round(R.N * steps / min(ts), 1),
            "seconds": [round(t, 3) for t in ts],
            "checkpointer_device_bytes": int(free0 - free1),
@@ -438,6 +446,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--long-steps", type=int, default=5333,
                     help="steps of the long-horizon leg at N=1M (0: skip)")
+    ap.add_argument("--long-tape-frac", type=float, default=0.5,
+                    help="share of free device memory the long-horizon checkpointer may tape "
+                         "(a second long-horizon run; 0: skip)")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the per-kernel and roofline passes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -522,11 +533,15 @@ def main():
     if kerns:
         roofline = roofline_of(best, *kerns[best], N)
 
-    long_h = None
+    long_h = long_t = None
     if args.long_steps > 0 and best == "f32c_ckpt":
         log(f"long horizon: {args.long_steps} steps, f32c, {args.snapshots} snapshots")
         long_h = run_long_horizon(R, args.long_steps, args.snapshots)
         log(f"long horizon: {long_h}")
+        if args.long_tape_frac > 0:
+            long_t = run_long_horizon(R, args.long_steps, args.snapshots,
+                                      tape_frac=args.long_tape_frac)
+            log(f"long horizon with a tape budget: {long_t}")
     del R
     gc.collect()
 
@@ -572,6 +587,7 @@ def main():
         "kernels_us": ({k: round(v["avg_us"], 3) for k, v in kerns[best][0].items()}
                        if kerns else None),
         "long_horizon": long_h,
+        "long_horizon_tape_budget": long_t,
         "batched_scenes_per_s": batched,
     }
     print(json.dumps(line), flush=True)

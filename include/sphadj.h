@@ -284,6 +284,17 @@ int sph_ckpt_backward(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
 /* the same with seeds in device memory */
 int sph_ckpt_backward_dev(sph_ckpt *ck, sph_workspace *ws, const sph_field *theta,
                           const double *seedX, const double *seedV);
+/* Tape budget of a checkpointer (no counterpart in the reference): let its
+ * segment tape hold up to tape_steps steps (whole segments, at least one).
+ * The last segments of the horizon that fit, after the one the binomial
+ * schedule tapes first, are then recorded in the forward and never
+ * recomputed; the schedule covers the segments before them with the same
+ * snapshot budget.  Gradients are the same bits as without the budget.
+ * Device memory of the tape per particle-step: f32c 80 B (X, V, pterm/invrho,
+ * exact masks), FP64 48 B, plus one layout record per epoch.  The default is
+ * one segment.  Reallocates the tape; a forward run before it needs a new
+ * forward. */
+int sph_ckpt_set_tape_steps(sph_ckpt *ck, int64_t tape_steps);
 /* {segments, snapshots, segments run by the last forward, segments run by the
  *  last backward (recomputation included), segments taped in the last backward} */
 int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5);

@@ -97,6 +97,7 @@ SIGNATURES = {
     "sph_ckpt_state": (C.c_int, [_vp, _pp]),
     "sph_ckpt_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "sph_ckpt_stats": (C.c_int, [_vp, _vp]),
+    "sph_ckpt_set_tape_steps": (C.c_int, [_vp, _i64]),
     "sph_ckpt_plan": (C.c_int, [_i64, _i64, _vp]),
     "sph_design_create": (C.c_int, [_vp, _i64, _i64, _i64, _d, _d, _pp]),
     "sph_design_destroy": (C.c_int, [_vp]),

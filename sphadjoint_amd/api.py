@@ -545,10 +545,16 @@ class Checkpointer:
     ``forward(X0, V0, theta)`` runs the whole horizon and returns the final
     state (owned by the checkpointer; ``backward`` reuses its buffers, so read
     the final state before calling ``backward``).
+
+    ``tape_steps`` lets the segment tape hold that many steps: the last
+    segments that fit are recorded in the forward and never recomputed, and
+    the schedule covers the segments before them (same gradients).  The tape
+    costs 80 B (f32c) or 48 B (FP64) of device memory per particle-step.
     """
 
     def __init__(self, backend: MI355XBackend, N, nsteps, p: SPHParams, *, snapshots=64,
-                 seg_epochs=1, skin=0.2, interval=4, maxnb=64, precision="f64"):
+                 seg_epochs=1, skin=0.2, interval=4, maxnb=64, precision="f64",
+                 tape_steps=None):
         self.backend, self.p, self.N = backend, p, int(N)
         self.nsteps, self.skin, self.interval = int(nsteps), float(skin), int(interval)
         self.seg_epochs = int(seg_epochs)
@@ -560,6 +566,10 @@ class Checkpointer:
         self._h = h
         self._stref = None
         self._theta = None
+        if tape_steps is not None:
+            # the last segments that fit on a tape of tape_steps steps are
+            # recorded in the forward, not recomputed (sph_ckpt_set_tape_steps)
+            L.call("sph_ckpt_set_tape_steps", h, int(tape_steps))
 
     @property
     def state(self):

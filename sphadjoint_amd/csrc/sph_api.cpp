@@ -2575,6 +2575,10 @@ struct sph_ckpt {
   int precision = SPH_F64;
   int64_t N = 0, nsteps = 0, nsnap = 0, interval = 4, maxnb = 64, E = 0;
   int64_t seg = 4;                      // steps per schedule unit (whole epochs)
+  // tape budget: the last `tail` units of the horizon are taped in the
+  // forward, together with the schedule's last unit, and never recomputed;
+  // the binomial schedule covers the first Es = E - tail units
+  int64_t tail = 0, Es = 0;
   double skin = 0.2;
   sph_state *work = nullptr;
   sph_tape *mini = nullptr;
@@ -2600,9 +2604,11 @@ struct sph_ckpt {
 
 namespace {
 
-// steps of schedule unit e (the last one may be short)
+// steps of schedule unit e (the last one may be short; the schedule's last
+// unit runs to the end of the horizon, through the taped tail)
 int64_t ck_steps(const sph_ckpt *ck, int64_t e) {
   int64_t rest = ck->nsteps - e * ck->seg;
+  if (e == ck->Es - 1) return rest;
   return rest < ck->seg ? rest : ck->seg;
 }
 
@@ -2697,10 +2703,13 @@ void ck_bind(sph_ckpt *ck) {
     if (ck->at != e) fail(SPH_E_STATE, "taped epoch %lld, state at %lld", (long long)e,
                           (long long)ck->at);
     sph_tape *tp = ck->mini;
+    const int64_t ns = ck_steps(ck, e);
+    if (ns > tp->cap)
+      fail(SPH_E_STATE, "segment of %lld steps on a tape of %lld", (long long)ns,
+           (long long)tp->cap);
     for (auto &r : tp->epochs) tp->pool.push_back(r);
     tp->epochs.clear();
     tp->n = 0;
-    const int64_t ns = ck_steps(ck, e);
     for (int64_t k = 0; k < ns; ++k) do_step(ck->work, ck->theta->d, tp);
     hipStream_t s = ck->ctx->stream;
     const size_t off = (size_t)tp->n * tp->N;
@@ -2903,6 +2912,7 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
       ck->skin = skin;
       ck->seg = seg_epochs * interval;
       ck->E = (nsteps + ck->seg - 1) / ck->seg;
+      ck->Es = ck->E;
       ck->nsnap = nsnap < ck->E ? nsnap : ck->E;
       std::vector<double> zeros((size_t)N * 2, 0.0);
       int rc = sph_state_create(ctx, p, precision, N, zeros.data(), zeros.data(), skin, interval,
@@ -2936,6 +2946,34 @@ int sph_ckpt_create(sph_ctx *ctx, const sph_params *p, int precision, int64_t N,
     }
     ctx_retain(ctx);
     *out = ck;
+  });
+}
+
+int sph_ckpt_set_tape_steps(sph_ckpt *ck, int64_t tape_steps) {
+  return guarded([&] {
+    if (!ck) fail(SPH_E_ARG, "NULL argument");
+    if (tape_steps < 1) fail(SPH_E_ARG, "tape_steps must be >= 1");
+    // units the tape holds: the schedule's last unit and tail units after it
+    int64_t units = tape_steps / ck->seg;
+    units = units < 1 ? 1 : (units > ck->E ? ck->E : units);
+    const int64_t tail = units - 1;
+    const int64_t Es = ck->E - tail;
+    // the tape holds any one unit, and the schedule's last unit with the tail
+    const int64_t one = ck->seg < ck->nsteps ? ck->seg : ck->nsteps;
+    const int64_t last = ck->nsteps - (Es - 1) * ck->seg;
+    const int64_t cap = last > one ? last : one;
+    sph_tape *tp = nullptr;
+    const int keep = SPH_TAPE_KEEP_PI | (ck->precision == SPH_F32C ? SPH_TAPE_KEEP_LISTS : 0);
+    (void)hipStreamSynchronize(ck->ctx->stream);
+    int rc = sph_tape_create_keep(ck->ctx, ck->N, cap, ck->precision, keep, &tp);
+    if (rc != SPH_OK) throw Error(rc, g_err);
+    if (ck->mini) sph_tape_destroy(ck->mini);
+    ck->mini = tp;
+    ck->tail = tail;
+    ck->Es = Es;
+    ck->sched.E = Es;
+    ck->sched.nsnap = ck->nsnap < Es ? ck->nsnap : Es;
+    ck->forward_done = false;   // a forward under the old budget has no backward
   });
 }
 
@@ -2994,7 +3032,7 @@ int ckpt_forward_impl(sph_ckpt *ck, const double *X0, const double *V0, const sp
     ck->theta = theta;
     ck->slot_of.assign((size_t)ck->E + 1, -1);
     ck->free_slots.clear();
-    for (int k = (int)ck->nsnap; k >= 1; --k) ck->free_slots.push_back(k);
+    for (int k = (int)ck->sched.nsnap; k >= 1; --k) ck->free_slots.push_back(k);
     ck->at = 0;
     ck->on_tape = -1;
     ck_copy(ck, true, 0);
@@ -3069,7 +3107,7 @@ int sph_ckpt_stats(const sph_ckpt *ck, int64_t *out5) {
     if (!ck || !out5) fail(SPH_E_ARG, "NULL argument");
     out5[0] = ck->E;
     out5[1] = ck->nsnap;
-    out5[2] = ck->fwd_advanced + ck->fwd_taped;
+    out5[2] = ck->fwd_advanced + ck->fwd_taped + ck->tail;
     out5[3] = ck->sched.advanced + ck->sched.taped;
     out5[4] = ck->sched.taped;
   });

@@ -269,3 +269,48 @@ def test_long_horizon_5333_steps(be):
     print("   f32c vs FP64: J %.3e  gX %.3e  gV %.3e  gtheta %.3e" % (eJ, *e))
     assert eJ < 1e-4
     assert e[2] < 1e-2
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+@pytest.mark.parametrize("nsteps,interval,snapshots,seg,tape_steps", [
+    (37, 4, 3, 1, 12),     # two tail segments after the schedule's last
+    (37, 4, 0, 1, 40),     # the whole horizon on the tape: no recomputation
+    (40, 2, 1, 4, 16),     # multi-epoch segments, one tail segment
+    (25, 1, 2, 1, 7),      # one step per epoch
+    (37, 4, 2, 3, 5),      # a budget below one segment: one segment
+])
+def test_tape_budget_equals_full_tape(be, precision, nsteps, interval, snapshots, seg,
+                                      tape_steps):
+    """With a tape budget the last segments of the horizon are taped in the
+    forward and never recomputed; the gradients are the full tape's bit for
+    bit, and the reverse sweep recomputes fewer segments."""
+    say("ckpt tape budget vs full tape", precision, nsteps, interval, snapshots, seg, tape_steps)
+    sc = scenes.brinkman_56x37(nsteps=nsteps)
+    rng = np.random.default_rng(nsteps * 13 + tape_steps)
+    cX = rng.standard_normal((2, sc.N))
+    cV = rng.standard_normal((2, sc.N))
+    want = full_tape(be, sc, nsteps, interval, cX, cV, precision)
+    pg = S.SPHParams(**sc.params)
+    runs = {}
+    for budget in (None, tape_steps):
+        ck = S.Checkpointer(be, sc.N, nsteps, pg, snapshots=snapshots, seg_epochs=seg,
+                            interval=interval, precision=precision, tape_steps=budget)
+        ws = S.AdjointWorkspace(be, sc.N, pg, precision=precision)
+        st = ck.forward(sc.X0, sc.V0, sc.theta)
+        X, V = st.X, st.V
+        ck.backward(ws, seedX=cX, seedV=cV)
+        runs[budget] = ((X, V, ws.gX, ws.gV, ws.gtheta), ck.stats())
+    got, stats = runs[tape_steps]
+    for a, b, name in zip(got, want, ("X", "V", "gX", "gV", "gtheta")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    E = -(-nsteps // (interval * seg))
+    units = min(max(tape_steps // (interval * seg), 1), E)
+    base = runs[None][1]
+    print(f"   E={E}, taped units {units}: backward recomputed {stats['backward_forward_epochs']} "
+          f"segments (no budget: {base['backward_forward_epochs']})")
+    assert stats["epochs"] == E
+    assert stats["forward_epochs"] == base["forward_epochs"] or units > 1
+    if units > 1:
+        assert stats["backward_forward_epochs"] < base["backward_forward_epochs"]
+    if units == E:
+        assert stats["backward_forward_epochs"] == 0
