@@ -61,8 +61,8 @@ struct Batch {
 constexpr int kSlotBlocks[3] = {64, 128, 256};
 // kernels whose workgroup size is tuned (sph_autotune)
 enum TunedKernel { TK_NLBUILD, TK_DENSITY, TK_FORCE, TK_ADJ_R, TK_ADJ_P1, TK_ADJ_P2, TK_COUNT };
-constexpr int kThetaLanes = 8;   // lanes that share one theta cell in the gtheta gather
-constexpr int kThetaStrip = 32;  // theta cells of one theta row per gtheta-gather workgroup
+constexpr int kThetaLanes = 4;   // lanes that share one theta cell in the gtheta gather
+constexpr int kThetaStrip = 64;  // theta cells of one theta row per gtheta-gather workgroup
 // SPH cell range [c0, c1] holding slots whose interp cell has index i along
 // an axis of nt theta cells of width ht: the cells of the theta cell's
 // interval (the division rule of cell_index) widened by one cell on each side

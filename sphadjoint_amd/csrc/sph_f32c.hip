@@ -1062,10 +1062,14 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
 // than kThetaCols - 1 SPH cells per row, reads keys and cell starts in global
 // memory (same order, same sums).
 // Workgroups are dealt to the scenes of a batched state in order
-// (Scene32::tboff).
-constexpr int kThetaStage = 2048;
+// (Scene32::tboff).  The gather is bound by its dependent rounds (stage,
+// compare, read the matches), so fewer, wider workgroups pay: 64 cells of 4
+// lanes run P1 + gather in 70.6 us on the 1M block against 75.6 us for 32
+// cells of 8 lanes (128 cells of 2 lanes need more LDS than P1 and drop its
+// occupancy: 82 us).
+constexpr int kThetaStage = 4096;
 constexpr int kThetaRows = 12;
-constexpr int kThetaCols = 64;
+constexpr int kThetaCols = 128;
 constexpr int kMatchCap = 8;
 constexpr int kMatchBatch = 4;
 struct ThetaStripLds {
