@@ -276,6 +276,32 @@ __device__ __forceinline__ int slot_quadrant(Rsrc qsr, int s, int c) {
 // cells) leaves no pair out.
 constexpr int kMaskRun = 32 * kMaskWords;
 
+// What the walk of slot s reads from memory before its first pair: the three
+// rows' runs and (kMasks) each row's first mask word, all independent of one
+// another.  A pass loads them in one batch -- before its staging, so that
+// their latency overlaps the staging's, or where registers are short right
+// after it -- instead of paying two round trips per row in every wavefront.
+// A row outside the grid has len = 0; its mask word is loaded (it lies in the
+// list) but never used.
+struct NbHead {
+  RowRun run[3];
+  uint32_t m0[3];
+  int2 c;
+};
+template <bool kMasks = true>
+__device__ __forceinline__ NbHead nb_head(Rsrc nmr, Rsrc cst, int s, int bc, const Grid &g,
+                                          int coff, uint32_t Ntot) {
+  NbHead h;
+  h.c = cell_xy(bc, g);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int jy = h.c.y + r - 1;
+    h.run[r] = (jy >= 0 && jy < g.ny) ? row_run(cst, h.c.x, jy, g, coff) : RowRun{0, 0, 0, 0};
+    h.m0[r] = kMasks ? ldu(nmr, ((uint32_t)r * Ntot + (uint32_t)s) * 4u) : 0u;
+  }
+  return h;
+}
+
 // Loader index of a neighbour: slot j of row r is passed to the loader as
 // j + ro.o[r].  Global loads use ro = 0 (the index is the slot); staged
 // passes use the LDS position of the row's staged segment minus its first
@@ -285,7 +311,7 @@ struct RowOff {
 };
 
 // Calls fn(d, dx, dy, r2) for every neighbour flagged in the mask list m_in
-// of slot s (own-frame position xi, binning cell bc), in the reference's sweep
+// of slot s (own-frame position xi, rows h from nb_head), in the reference's sweep
 // order, where d = ld(i) holds the neighbour's per-slot data (d.x must be its
 // xr), i being its loader index (RowOff).  fn returns whether the pair is a
 // true neighbour (r2 < (2h)^2); when em is given, those pairs form a mask list
@@ -310,24 +336,21 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
 // (for loaders from global memory); loaders from LDS walk one pair at a time,
 // the wavefronts sharing a SIMD hiding the LDS latency.
 template <bool kPipe = true, class Ld, class Fn>
-__device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, Rsrc cst, int s,
-                                            int bc, float2 xi, const Grid &g, int coff,
-                                            uint32_t Ntot, const PhysF32 &P, const RowOff &ro,
-                                            Ld &&ld, Fn &&fn) {
+__device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, const NbHead &h,
+                                            int s, float2 xi, uint32_t Ntot, const PhysF32 &P,
+                                            const RowOff &ro, Ld &&ld, Fn &&fn) {
   const float cs = P.cs;
-  const int2 c = cell_xy(bc, g);
   const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const int jy = c.y + r - 1;
-    if (jy < 0 || jy >= g.ny) continue;
-    const RowRun run = row_run(cst, c.x, jy, g, coff);
+    const RowRun run = h.run[r];
+    if (run.len == 0) continue;
     const float yi = xi.y - (float)(r - 1) * cs;
     int nw = (run.len + 31) >> 5;
     nw = nw < kMaskWords ? nw : kMaskWords;
     for (int w = 0; w < nw; ++w) {
       const uint32_t widx = (uint32_t)(w * 3 + r) * Ntot + (uint32_t)s;
-      uint32_t m = ldu(nmr, widx * 4u);
+      uint32_t m = w == 0 ? h.m0[r] : ldu(nmr, widx * 4u);
       uint32_t hit = 0;
       if (m != 0) {
         const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
@@ -435,15 +458,18 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const int s = sl < S.g.N ? S.off + sl : w0;
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
+  // the slot's own loads before the staging plan, its row runs after it: both
+  // arrive while the window is copied
+  const float2 xi = xr[s];
+  const int bc = bcell[s];
   stage_plan(T, bcell, cst, w0, w1, g, S.coff, B.nostage ? 0 : kStageCap);
+  const NbHead h = nb_head<false>(cst, cst, s, bc, g, S.coff, B.N);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
   if (sl >= S.g.N) return;
   const float rc2 = (float)(S.cs64 * S.cs64);
-  const float2 xi = xr[s];
-  const int bc = bcell[s];
-  const int2 c = cell_xy(bc, g);
+  const int2 c = h.c;
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
   const Rsrc qsr = rsrc(qs, kSub ? (uint32_t)(4 * B.ncell + 1) * 4u : 0u);
   const int q = kSub ? slot_quadrant(qsr, s, bc) : 0;
@@ -452,7 +478,7 @@ k_nm_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
     for (int r = 0; r < 3; ++r) {
       const int jy = c.y + r - 1;
       if (jy < 0 || jy >= g.ny) continue;
-      const RowRun run = row_run(cst, c.x, jy, g, S.coff);
+      const RowRun run = h.run[r];
       const float yi = xi.y - (float)(r - 1) * P.cs;
       // masks cover the first kMaskRun slots of the run; the passes test the rest
       const int len = run.len < kMaskRun ? run.len : kMaskRun;
@@ -562,14 +588,14 @@ __device__ __forceinline__ float2 density_finish(const FSum &acc, const PhysF32 
 }
 
 template <bool kPipe, class Ld>
-__device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst, int s, int bc,
-                                               float2 xi, const Grid &g, int coff, uint32_t Ntot,
-                                               const PhysF32 &P, const RowOff &ro, Ld &&ld) {
+__device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, const NbHead &h, int s,
+                                               float2 xi, uint32_t Ntot, const PhysF32 &P,
+                                               const RowOff &ro, Ld &&ld) {
   // sum of W/(m A) = u^4 (2q + 1) by Fast2Sum from the self term 1, the
   // largest term of the sum: every later term is positive and no larger, so
   // each step's error term is exact
   FSum acc(1.f);
-  for_each_nb<kPipe>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, ro, ld,
+  for_each_nb<kPipe>(nmr, em, h, s, xi, Ntot, P, ro, ld,
               [&, P](const NbX &, float, float, float r2) __attribute__((always_inline)) {
     if (r2 < P.r2max) {
       acc.add(density_term(r2, P));
@@ -581,14 +607,14 @@ __device__ __forceinline__ float2 density_pair(Rsrc nmr, uint32_t *em, Rsrc cst,
 }
 
 // density over the staged rows when the window fits, else over global loads
-__device__ __forceinline__ float2 density_slot(Rsrc nmr, uint32_t *em, Rsrc cst, Rsrc xrr,
+__device__ __forceinline__ float2 density_slot(Rsrc nmr, uint32_t *em, const NbHead &h, Rsrc xrr,
                                                const StageTab &T, const float2 *sx, int s, int bc,
-                                               float2 xi, const Grid &g, int coff, uint32_t Ntot,
+                                               float2 xi, const Grid &g, uint32_t Ntot,
                                                const PhysF32 &P) {
   if (T.ok)
-    return density_pair<false>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, stage_rowoff(T, bc, g),
+    return density_pair<false>(nmr, em, h, s, xi, Ntot, P, stage_rowoff(T, bc, g),
                         [&](int i) __attribute__((always_inline)) { return NbX{sx[i]}; });
-  return density_pair<true>(nmr, em, cst, s, bc, xi, g, coff, Ntot, P, RowOff{},
+  return density_pair<true>(nmr, em, h, s, xi, Ntot, P, RowOff{},
                       [&](int i) __attribute__((always_inline)) {
                         return NbX{ld2(xrr, (uint32_t)i * 8u)};
                       });
@@ -609,13 +635,17 @@ k_density32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
+  // the slot's own loads before the staging plan, its walk's row heads after
+  // it: both arrive while the window is copied
+  const int bc = bcell[s];
+  const float2 xi = xr[s];
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
+  const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
-  const int bc = bcell[s];
-  float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
+  float2 rd = density_slot(nmr, em, h, xrr, T, sx, s, bc, xi, g, B.N, P);
   PI[s] = eos(rd, P);
   if (rho_out) rho_out[s] = (double)rd.x;
 }
@@ -641,15 +671,18 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
   const Rsrc cst = rsrc(cstart, (uint32_t)(B.ncell + 1) * 4u);
   const Rsrc xrr = rsrc(xr, (uint32_t)B.N * 8u);
   const Rsrc qsr = rsrc(qs, kSub ? (uint32_t)(4 * B.ncell + 1) * 4u : 0u);
+  // the slot's own loads before the staging plan, its row runs after it: both
+  // arrive while the window is copied
+  const float2 xi = xr[s];
+  const int bc = bcell[s];
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
+  const NbHead h = nb_head<false>(cst, cst, s, bc, g, S.coff, B.N);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
   const float rc2 = (float)(S.cs64 * S.cs64);
-  const float2 xi = xr[s];
-  const int bc = bcell[s];
-  const int2 c = cell_xy(bc, g);
+  const int2 c = h.c;
   const float xm = xi.x + P.cs, xp = xi.x - P.cs;
   const int q = kSub ? slot_quadrant(qsr, s, bc) : 0;
   FSum acc(1.f);
@@ -658,7 +691,7 @@ k_density_build32(const float2 *__restrict__ xr, const int *__restrict__ bcell,
     for (int r = 0; r < 3; ++r) {
       const int jy = c.y + r - 1;
       if (jy < 0 || jy >= g.ny) continue;
-      const RowRun run = row_run(cst, c.x, jy, g, S.coff);
+      const RowRun run = h.run[r];
       const float yi = xi.y - (float)(r - 1) * P.cs;
       const int len = run.len < kMaskRun ? run.len : kMaskRun;
       const int rb = run.rb + ro.o[r];
@@ -749,6 +782,12 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sv[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
+  // the slot's own loads and its drag coefficient first: they arrive while
+  // the window is staged (the row heads, loaded after it, in one round trip)
+  const double2 x64 = X[s];
+  const float2 xi = xr[s], vi = V[s], pii = PI[s];
+  const int bc = bcell[s];
+  const float al = interp_alpha(theta + S.toff, interp_cell(x64.x, x64.y, P), P, nullptr, nullptr);
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
@@ -763,9 +802,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
                  sv[i] = d.v;
                });
   if (!valid_) return;
-  const double2 x64 = X[s];
-  const float2 xi = xr[s], vi = V[s], pii = PI[s];
-  const int bc = bcell[s];
+  const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   // pair term F (C (v_i - v_j) - m (pt_i + pt_j) d), F = Fc u^3, with Fc and
   // m folded into the own-particle factors
   const float piF = P.mFc * pii.x;
@@ -784,22 +821,20 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   };
   // nm is the exact list of the density pass over these positions
   if (T.ok) {
-    for_each_nb<false>(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, nullptr, h, s, xi, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) {
                   const float4 q = sxp[i];
                   return NbXPV{make_float2(q.x, q.y), make_float2(q.z, q.w), sv[i]};
                 },
                 pair);
   } else {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
+    for_each_nb(nmr, nullptr, h, s, xi, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
                   return NbXPV{ld2(xrr, o8), ld2(pir, o8), ld2(vr, o8)};
                 },
                 pair);
   }
-  InterpF ic = interp_cell(x64.x, x64.y, P);
-  float al = interp_alpha(theta + S.toff, ic, P, nullptr, nullptr);
   float axv = ax.value() + (P.gx + wall_accel((float)x64.x, P.Lx, P.kw));
   float ayv = ay.value() + (P.gy + wall_accel((float)x64.y, P.Ly, P.kw));
   if (a_out) {
@@ -973,14 +1008,17 @@ k_adj_r32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   __shared__ float2 sx[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
+  // own loads before the staging plan, row heads after it (as k_density32)
+  const double2 x64 = X[s];
+  const int bc = bcell[s];
+  const float2 xi = xr[s];
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
+  const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   if (T.ok)
     stage_copy(T, [&](int j) { return ld2(xrr, (uint32_t)j * 8u); },
                [&](int i, float2 v) { sx[i] = v; });
   if (!valid_) return;
-  const double2 x64 = X[s];
-  const int bc = bcell[s];
-  float2 rd = density_slot(nmr, em, cst, xrr, T, sx, s, bc, xr[s], g, S.coff, B.N, P);
+  float2 rd = density_slot(nmr, em, h, xrr, T, sx, s, bc, xi, g, B.N, P);
   PI[s] = eos(rd, P);
   InterpF ic = interp_cell(x64.x, x64.y, P);
   tkey[s] = S.tcoff + ic.l00 - ic.l00 / P.ngy;   // scene's theta cell j0 + i0*(ngy-1)
@@ -1305,6 +1343,11 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   float4 *const sxp = L.sxp, *const sav = L.sav;
   SCENE_WIN_AT(B, lane, blk);
   NB_RSRC;
+  // the slot's own loads first: they arrive while the window is staged
+  const double2 x64 = X[s];
+  const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
+  const int bc = bcell[s];
+  const float ga = galpha[s];
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapP1);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
@@ -1320,11 +1363,9 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
                  sav[i] = make_float4(d.ab.x, d.ab.y, d.v.x, d.v.y);
                });
   if (!valid_) return;
-  const double2 x64 = X[s];
-  const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
+  const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   const float pti = pii.x, iri = pii.y;
   const float mpti = P.m * pti, tmi = P.twomumass * iri;
-  const int bc = bcell[s];
   ASum gx, gy, gvx, gvy, grh, sumS;
   auto pair = [&, P](const NbXPAV &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     if (kExact || r2 < P.r2max) {
@@ -1360,7 +1401,7 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   };
   uint32_t *emw = kExact ? nullptr : em;
   if (T.ok) {
-    for_each_nb<false>(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, emw, h, s, xi, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) {
                   const float4 q = sxp[i], r = sav[i];
                   return NbXPAV{make_float2(q.x, q.y), make_float2(q.z, q.w),
@@ -1368,7 +1409,7 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
                 },
                 pair);
   } else {
-    for_each_nb(nmr, emw, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
+    for_each_nb(nmr, emw, h, s, xi, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   const uint32_t o8 = (uint32_t)i * 8u;
                   return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
@@ -1381,10 +1422,8 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
               (P.c2 * P.m * iri * iri) * Ssum;
   float gxv = gx.value() + ab.x * wall_daccel((float)x64.x, P.Lx, P.kw);
   float gyv = gy.value() + ab.y * wall_daccel((float)x64.y, P.Ly, P.kw);
-  float ga = galpha[s];
-  InterpF ic = interp_cell(x64.x, x64.y, P);
   float dadx, dady;
-  interp_alpha(theta + S.toff, ic, P, &dadx, &dady);
+  interp_alpha(theta + S.toff, interp_cell(x64.x, x64.y, P), P, &dadx, &dady);
   gxv += ga * dadx;
   gyv += ga * dady;
   gXs[s] = make_float2(gxv, gyv);
@@ -1434,7 +1473,17 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   __shared__ float sgr[kStageCap];
   SCENE_WIN(B, lane);
   NB_RSRC;
+  // the slot's own loads before the staging plan, its walk's row heads after
+  // it: both arrive while the window is copied
+  RpwIn rin{};
+  if (kFuseR) rin = rpw_load(R, s);
+  const float2 xi = xr[s];
+  const float gri = grho[s];
+  const int bc = bcell[s];
+  const float2 gs = gXs[s];
+  const float2 go = gX[s];
   stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
+  const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   const Rsrc grr = rsrc(grho, (uint32_t)B.N * 4u);
   if (T.ok)
     stage_copy(T,
@@ -1446,11 +1495,6 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
                  sgr[i] = d.gr;
                });
   if (!valid_) return;
-  RpwIn rin{};
-  if (kFuseR) rin = rpw_load(R, s);
-  const float2 xi = xr[s];
-  const float gri = grho[s];
-  const int bc = bcell[s];
   ASum gx, gy;
   auto pair = [&, P](const NbXG &d, float dx, float dy, float r2) __attribute__((always_inline)) {
     const float u = fmaf(__builtin_amdgcn_sqrtf(r2), P.mhh, 1.f);
@@ -1461,17 +1505,15 @@ k_adj_p232(const float2 *__restrict__ xr, const float *__restrict__ grho,
   };
   // nm is the exact list of P1 over these positions
   if (T.ok) {
-    for_each_nb<false>(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, stage_rowoff(T, bc, g),
+    for_each_nb<false>(nmr, nullptr, h, s, xi, B.N, P, stage_rowoff(T, bc, g),
                 [&](int i) __attribute__((always_inline)) { return NbXG{sx[i], sgr[i]}; }, pair);
   } else {
-    for_each_nb(nmr, nullptr, cst, s, bc, xi, g, S.coff, B.N, P, RowOff{},
+    for_each_nb(nmr, nullptr, h, s, xi, B.N, P, RowOff{},
                 [&](int i) __attribute__((always_inline)) {
                   return NbXG{ld2(xrr, (uint32_t)i * 8u), ld1(grr, (uint32_t)i * 4u)};
                 },
                 pair);
   }
-  float2 gs = gXs[s];
-  float2 go = gX[s];
   const float2 gxn = make_float2(go.x + (gs.x + gx.value()), go.y + (gs.y + gy.value()));
   gX[s] = gxn;
   // the R pass of the previous step (same slot order) on this step's final gX
