@@ -401,9 +401,17 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
 }
 
 // ---- LDS staging of the neighbour rows of a window (sph_stage.h) -------------
-// Capacities in slots: kStageCap, and kStageCapP1 for adjoint P1, whose 32
-// bytes per slot would otherwise leave room for only four workgroups per CU.
-constexpr int kStageCap = 1024;
+// Capacities in slots.  A 256-slot window in a cell row two lattice rows high
+// between rows three high stages ~1100 slots on the 1M block (every eighth
+// cell row there, 5 % of the windows); the others stage 800-900.
+// kStageCap, for the passes of 8 or 12 bytes a slot, holds them all: those
+// passes' occupancy is bound by registers, not LDS.  Force (24 bytes) and
+// adjoint P1 (32 bytes) keep the capacities of six and five workgroups per
+// CU (kStageCapF, kStageCapP1) and read the few larger windows from global
+// memory: measured, capacities of 1120 that hold them cost a workgroup per CU
+// and more time than they save (force 41 -> 43 us, P1 68 -> 75 us).
+constexpr int kStageCap = 1280;
+constexpr int kStageCapF = 1024;
 constexpr int kStageCapP1 = 960;
 
 // window plan over the cell starts of a buffer descriptor
@@ -778,8 +786,8 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
           double *__restrict__ alpha_out) {
   __shared__ StageTab T;
   // a pair reads (x, pi) as one 16-byte record and v beside it
-  __shared__ float4 sxp[kStageCap];
-  __shared__ float2 sv[kStageCap];
+  __shared__ float4 sxp[kStageCapF];
+  __shared__ float2 sv[kStageCapF];
   SCENE_WIN(B, lane);
   NB_RSRC;
   // the slot's own loads and its drag coefficient first: they arrive while
@@ -788,7 +796,7 @@ k_force32(const double2 *__restrict__ X, const float2 *__restrict__ xr,
   const float2 xi = xr[s], vi = V[s], pii = PI[s];
   const int bc = bcell[s];
   const float al = interp_alpha(theta + S.toff, interp_cell(x64.x, x64.y, P), P, nullptr, nullptr);
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCap);
+  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapF);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   if (T.ok)
