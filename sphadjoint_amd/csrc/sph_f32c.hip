@@ -336,68 +336,74 @@ __device__ __forceinline__ void nb_pair(const D &d, int b, int s0, int s1, float
 // (for loaders from global memory); loaders from LDS walk one pair at a time,
 // the wavefronts sharing a SIMD hiding the LDS latency.
 template <bool kPipe = true, class Ld, class Fn>
+__device__ __forceinline__ void nb_row(int r, Rsrc nmr, uint32_t *__restrict__ em, const NbHead &h,
+                                       int s, float2 xi, uint32_t Ntot, const PhysF32 &P,
+                                       const RowOff &ro, Ld &&ld, Fn &&fn) {
+  const RowRun run = h.run[r];
+  if (run.len == 0) return;
+  const float cs = P.cs;
+  const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
+  const float yi = xi.y - (float)(r - 1) * cs;
+  int nw = (run.len + 31) >> 5;
+  nw = nw < kMaskWords ? nw : kMaskWords;
+  for (int w = 0; w < nw; ++w) {
+    const uint32_t widx = (uint32_t)(w * 3 + r) * Ntot + (uint32_t)s;
+    uint32_t m = w == 0 ? h.m0[r] : ldu(nmr, widx * 4u);
+    uint32_t hit = 0;
+    if (m != 0) {
+      const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
+      const int wb = run.rb + 32 * w + ro.o[r];
+      if constexpr (!kPipe) {
+        do {
+          const int b = __builtin_ctz(m);
+          m &= m - 1;
+          nb_pair(ld(wb + b), b, s0, s1, xm, xi.x, xp, yi, hit, fn);
+        } while (m != 0);
+      } else {
+      int b0 = __builtin_ctz(m);
+      m &= m - 1;
+      auto d0 = ld(wb + b0);
+      for (;;) {
+        const bool more1 = m != 0;
+        const int b1 = more1 ? __builtin_ctz(m) : b0;
+        m &= m - 1;
+        auto d1 = ld(wb + b1);
+        nb_pair(d0, b0, s0, s1, xm, xi.x, xp, yi, hit, fn);
+        if (!more1) break;
+        const bool more0 = m != 0;
+        b0 = more0 ? __builtin_ctz(m) : b1;
+        m &= m - 1;
+        d0 = ld(wb + b0);
+        nb_pair(d1, b1, s0, s1, xm, xi.x, xp, yi, hit, fn);
+        if (!more0) break;
+      }
+      }
+    }
+    if (em) em[widx] = hit;
+  }
+  // slots of a row run beyond the kMaskRun slots its masks cover (a pile-up,
+  // e.g. particles beyond a wall in a clamped edge cell): tested directly,
+  // in slot order after the masked ones, so no pair is left out
+  if (run.len > kMaskRun) {
+    const int rb = run.rb + ro.o[r];
+    for (int b = kMaskRun; b < run.len; ++b) {
+      const auto d = ld(rb + b);
+      const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
+      const float dx = xs - d.x.x;
+      const float dy = yi - d.x.y;
+      const float r2 = fmaf(dx, dx, dy * dy);
+      if (r2 > P.r2min && r2 < P.r2max) fn(d, dx, dy, r2);
+    }
+  }
+}
+
+// all three rows in order (row r = 0 below the slot's cell row, 2 above)
+template <bool kPipe = true, class Ld, class Fn>
 __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em, const NbHead &h,
                                             int s, float2 xi, uint32_t Ntot, const PhysF32 &P,
                                             const RowOff &ro, Ld &&ld, Fn &&fn) {
-  const float cs = P.cs;
-  const float xm = xi.x + cs, xp = xi.x - cs;   // own x in the frames of cells cx-1, cx+1
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const RowRun run = h.run[r];
-    if (run.len == 0) continue;
-    const float yi = xi.y - (float)(r - 1) * cs;
-    int nw = (run.len + 31) >> 5;
-    nw = nw < kMaskWords ? nw : kMaskWords;
-    for (int w = 0; w < nw; ++w) {
-      const uint32_t widx = (uint32_t)(w * 3 + r) * Ntot + (uint32_t)s;
-      uint32_t m = w == 0 ? h.m0[r] : ldu(nmr, widx * 4u);
-      uint32_t hit = 0;
-      if (m != 0) {
-        const int s0 = run.s0 - 32 * w, s1 = run.s1 - 32 * w;   // word-local cell bounds
-        const int wb = run.rb + 32 * w + ro.o[r];
-        if constexpr (!kPipe) {
-          do {
-            const int b = __builtin_ctz(m);
-            m &= m - 1;
-            nb_pair(ld(wb + b), b, s0, s1, xm, xi.x, xp, yi, hit, fn);
-          } while (m != 0);
-        } else {
-        int b0 = __builtin_ctz(m);
-        m &= m - 1;
-        auto d0 = ld(wb + b0);
-        for (;;) {
-          const bool more1 = m != 0;
-          const int b1 = more1 ? __builtin_ctz(m) : b0;
-          m &= m - 1;
-          auto d1 = ld(wb + b1);
-          nb_pair(d0, b0, s0, s1, xm, xi.x, xp, yi, hit, fn);
-          if (!more1) break;
-          const bool more0 = m != 0;
-          b0 = more0 ? __builtin_ctz(m) : b1;
-          m &= m - 1;
-          d0 = ld(wb + b0);
-          nb_pair(d1, b1, s0, s1, xm, xi.x, xp, yi, hit, fn);
-          if (!more0) break;
-        }
-        }
-      }
-      if (em) em[widx] = hit;
-    }
-    // slots of a row run beyond the kMaskRun slots its masks cover (a pile-up,
-    // e.g. particles beyond a wall in a clamped edge cell): tested directly,
-    // in slot order after the masked ones, so no pair is left out
-    if (run.len > kMaskRun) {
-      const int rb = run.rb + ro.o[r];
-      for (int b = kMaskRun; b < run.len; ++b) {
-        const auto d = ld(rb + b);
-        const float xs = b < run.s0 ? xm : (b < run.s1 ? xi.x : xp);
-        const float dx = xs - d.x.x;
-        const float dy = yi - d.x.y;
-        const float r2 = fmaf(dx, dx, dy * dy);
-        if (r2 > P.r2min && r2 < P.r2max) fn(d, dx, dy, r2);
-      }
-    }
-  }
+  for (int r = 0; r < 3; ++r) nb_row<kPipe>(r, nmr, em, h, s, xi, Ntot, P, ro, ld, fn);
 }
 
 // ---- LDS staging of the neighbour rows of a window (sph_stage.h) -------------
@@ -405,20 +411,22 @@ __device__ __forceinline__ void for_each_nb(Rsrc nmr, uint32_t *__restrict__ em,
 // between rows three high stages ~1100 slots on the 1M block (every eighth
 // cell row there, 5 % of the windows); the others stage 800-900.
 // kStageCap, for the passes of 8 or 12 bytes a slot, holds them all: those
-// passes' occupancy is bound by registers, not LDS.  Force (24 bytes) and
-// adjoint P1 (32 bytes) keep the capacities of six and five workgroups per
-// CU (kStageCapF, kStageCapP1) and read the few larger windows from global
-// memory: measured, capacities of 1120 that hold them cost a workgroup per CU
-// and more time than they save (force 41 -> 43 us, P1 68 -> 75 us).
+// passes' occupancy is bound by registers, not LDS.  Force (24 bytes) keeps
+// the capacity of six workgroups per CU (kStageCapF) and reads the few larger
+// windows from global memory: measured, a capacity of 1120 that holds them
+// costs a workgroup per CU and more time than it saves (41 -> 43 us).  Force
+// staging one row at a time as P1 does (below) is slower too: 40.8 -> 44.8
+// us, its registers then spill at seven waves per SIMD.  Adjoint P1 (32
+// bytes) stages one sweep row at a time (kStageCapP1, at adj_p1).
 constexpr int kStageCap = 1280;
 constexpr int kStageCapF = 1024;
-constexpr int kStageCapP1 = 960;
 
 // window plan over the cell starts of a buffer descriptor
+template <bool kPhased = false>
 __device__ __forceinline__ void stage_plan(StageTab &T, const int *__restrict__ bcell, Rsrc cst,
                                            int w0, int w1, const Grid &g, int coff, int cap) {
-  stage_plan_at(T, bcell, [&](int i) { return (int)ldu(cst, (uint32_t)i * 4u); }, w0, w1, g,
-                coff, cap);
+  stage_plan_at<kPhased>(T, bcell, [&](int i) { return (int)ldu(cst, (uint32_t)i * 4u); }, w0,
+                         w1, g, coff, cap);
 }
 
 // loader offsets of slot s (binning cell bc) when its window is staged
@@ -1091,57 +1099,33 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
 
 // ---- gtheta gather over the keys of the pointwise pass -----------------------
 
-// The same gather over per-slot keys written by the f32c adjoint's pointwise
-// pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1) of the slot's
-// interp cell, so that a candidate costs one key compare; a match's FP64
-// fractions (fx, fy) are formed from its position exactly as the f32c
-// interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
+// The gather of sph_theta.hip over per-slot keys written by the f32c adjoint's
+// pointwise pass: key[p] = (the scene's first theta cell) + j0 + i0*(ngy-1)
+// of the slot's interp cell, so that a candidate costs one key compare; a
+// match's FP64 fractions (fx, fy) are formed from its position exactly as the
+// f32c interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
 // contributions and their order are those of k_theta_cells.
-// A workgroup takes a strip of kThetaStrip theta cells of one theta row
-// (kThetaLanes lanes per cell) and first copies, for each SPH cell row the
-// strip sweeps, the keys of the slot run that covers the SPH cell ranges of
-// all its cells into LDS (coalesced); each cell then compares keys there and
-// reads positions and galpha only for its own slots, kMatchCap matches at a
-// time; the cell starts of the strip's rows are staged too, so that a cell's
-// sweep issues no dependent global loads before its matches.  A strip whose
-// runs exceed kThetaStage keys, that sweeps more than kThetaRows rows or more
-// than kThetaCols - 1 SPH cells per row, reads keys and cell starts in global
-// memory (same order, same sums).
-// Workgroups are dealt to the scenes of a batched state in order
-// (Scene32::tboff).  The gather is bound by its dependent rounds (stage,
-// compare, read the matches), so fewer, wider workgroups pay: 64 cells of 4
-// lanes run P1 + gather in 70.6 us on the 1M block against 75.6 us for 32
-// cells of 8 lanes (128 cells of 2 lanes need more LDS than P1 and drop its
-// occupancy: 82 us).
-constexpr int kThetaStage = 4096;
-constexpr int kThetaRows = 12;
-constexpr int kThetaCols = 128;
-constexpr int kMatchCap = 8;
-constexpr int kMatchBatch = 4;
-struct ThetaStripLds {
-  int skey[kThetaStage];
-  int slo[kThetaRows], soff[kThetaRows + 1];
-  int scs[kThetaRows][kThetaCols];   // cstart of cells X0 .. X1+1 of each row
-  int sok, sx0;
-  int smatch[kMatchCap][kBlock];     // each lane's matched slots, in order
-};
-
-// workgroup blk of the gather; FP64 without contraction, so that each
-// contribution is k_theta_cells' product bit for bit
-__device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const ThetaKeyed &A,
-                                            const Batch &B) {
+// A workgroup takes a strip of kThetaStrip theta cells of one theta row,
+// kThetaLanes lanes per cell; lane l of a cell sweeps every kThetaLanes-th
+// slot (from the l-th) of each SPH cell row of the cell's range, the rows in
+// order, and a butterfly adds the lanes' sums.  A lane reads the cell starts
+// of all its rows at once (kLeanRows rows a round), then its candidates' keys
+// kLeanBatch at a time, and the positions and galpha of the matches among
+// them together.  It holds no LDS, so a launch that runs it beside P1 keeps
+// P1's occupancy.  Workgroups are dealt to the scenes of a batched state in
+// order (Scene32::tboff).
+// (Measured on the 1M block: alone 21 us against 34 us for a version that
+// first staged the strip's keys and cell starts in 31 KB of LDS, and inside
+// P1's launch 60.5 against 68.7 us, the staged version's LDS having held P1
+// at five waves per SIMD.)
+constexpr int kLeanBatch = 4;
+constexpr int kLeanRows = 4;
+__device__ __forceinline__ void theta_strip(int blk, const ThetaKeyed &A, const Batch &B) {
 #pragma clang fp contract(off)
-  static_assert(kThetaStrip * kThetaLanes == kBlock, "one strip per workgroup");
-  int *const skey = L.skey, *const slo = L.slo, *const soff = L.soff;
-  int (*const scs)[kThetaCols] = L.scs;
-  int (*const smatch)[kBlock] = L.smatch;
-  int &sok = L.sok, &sx0 = L.sx0;
   const int *__restrict__ key = A.key;
   const double2 *__restrict__ X = A.X;
   const float *__restrict__ galpha = A.galpha;
   const int *__restrict__ cstart = A.cstart;
-  const int tstamp = A.tstamp, first = A.first;
-  double4 *__restrict__ part = A.part;
   const Scene32 &Sc = B.sc[B.tb2s[blk]];
   const Grid g = Sc.g;
   const ThetaGrid &T = Sc.T;
@@ -1154,96 +1138,17 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
   const int ilast = min(ifirst + kThetaStrip, ncx) - 1;
   const int lane = threadIdx.x & (kThetaLanes - 1);
   const int i0 = ifirst + (int)threadIdx.x / kThetaLanes;
+  if (i0 > ilast) return;          // the kThetaLanes lanes of a group leave together
   // SPH cell ranges holding particles with interp cell (i, j0), tabulated per
   // scene: a quarter cell of margin (theta_cell_range_tight) unless the R pass
   // found a slot further out this step (level 1: one cell, theta_cell_range;
   // level m >= 2: the one-cell range widened by m - 1 cells on each side)
-  const int lv = level_of(A.tlev, tstamp);
+  const int lv = level_of(A.tlev, A.tstamp);
   const int2 *tr = (lv >= 1 ? B.trange : B.trange_tight) + Sc.troff;
   const int wid = lv > 1 ? lv - 1 : 0;
-  auto widen = [&](int2 r, int n) {
-    return make_int2(max(r.x - wid, 0), min(r.y + wid, n - 1));
-  };
-  auto xrange = [&](int i, int &cx0, int &cx1) {
-    const int2 r = widen(tr[i], g.nx);
-    cx0 = r.x;
-    cx1 = r.y;
-  };
-  const int2 ry = widen(tr[ncx + j0], g.ny);
-  const int cy0 = ry.x, cy1 = ry.y;
-  const int nrow = cy1 - cy0 + 1;
-  // plan: row k of the strip stages slots [lo_k, lo_k + len_k) at soff[k]
-  if (threadIdx.x < 64) {
-    const int k = (int)threadIdx.x;
-    int X0, X1, dummy;
-    xrange(ifirst, X0, dummy);
-    xrange(ilast, dummy, X1);
-    int lo = 0, len = 0;
-    if (k < nrow && k < kThetaRows) {
-      const int base = coff + (cy0 + k) * g.nx;
-      lo = cstart[base + X0];
-      len = cstart[base + X1 + 1] - lo;
-    }
-    int inc = len;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-      const int t = __shfl_up(inc, d, 64);
-      if (k >= d) inc += t;
-    }
-    if (k < kThetaRows) {
-      slo[k] = lo;
-      soff[k] = inc - len;
-    }
-    const int total = __shfl(inc, kThetaRows - 1, 64);
-    if (k == 0) {
-      soff[kThetaRows] = total;
-      sx0 = X0;
-      sok = (nrow <= kThetaRows && total <= kThetaStage && X1 - X0 + 2 <= kThetaCols) ? 1 : 0;
-    }
-  }
-  __syncthreads();
-  const bool staged = sok != 0;
-  if (staged) {
-    // cell starts of the rows, then the keys; each thread issues the loads of
-    // four elements before storing them
-    const int X0 = sx0;
-    const int ncol = g.nx - X0 + 1 < kThetaCols ? g.nx - X0 + 1 : kThetaCols;
-    const int ncs = nrow * kThetaCols, total = soff[kThetaRows];
-    int off[kThetaRows];
-#pragma unroll
-    for (int q = 0; q < kThetaRows; ++q) off[q] = soff[q];
-    for (int i0 = (int)threadIdx.x; i0 < ncs + total; i0 += 4 * kBlock) {
-      int v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
-        v[u] = 0;
-        if (i < ncs) {
-          const int k = i / kThetaCols, c = i % kThetaCols;
-          if (c < ncol) v[u] = cstart[coff + (cy0 + k) * g.nx + X0 + c];
-        } else if (i < ncs + total) {
-          const int t = i - ncs;
-          // the run holding t; only the strip's nrow rows hold keys (nrow
-          // is the same in every lane, so the search stops there for all)
-          int k = 0;
-#pragma unroll
-          for (int q = 1; q < kThetaRows; ++q) {
-            if (q >= nrow) break;
-            k += (t >= off[q]) ? 1 : 0;
-          }
-          v[u] = key[slo[k] + (t - soff[k])];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * kBlock;
-        if (i < ncs) (&scs[0][0])[i] = v[u];
-        else if (i < ncs + total) skey[i - ncs] = v[u];
-      }
-    }
-    __syncthreads();
-  }
-  if (i0 > ilast) return;          // the kThetaLanes lanes of a group leave together
+  const int2 rxw = tr[i0], ryw = tr[ncx + j0];
+  const int cx0 = max(rxw.x - wid, 0), cx1 = min(rxw.y + wid, g.nx - 1);
+  const int cy0 = max(ryw.x - wid, 0), cy1 = min(ryw.y + wid, g.ny - 1);
   const int mykey = tcoff + j0 + i0 * ncy;
   const PhysF32 &P = Sc.P;
   auto frac = [&](double2 x) {
@@ -1252,8 +1157,6 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
     gy = (gy > P.gyclamp) ? P.gyclamp : ((gy < 0.0) ? 0.0 : gy);
     return make_double2(gx - (double)i0, gy - (double)j0);
   };
-  int cx0, cx1;
-  xrange(i0, cx0, cx1);
   double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
   auto add = [&](double2 f, double ga) {
     s00 += (ga * (1.0 - f.x)) * (1.0 - f.y);
@@ -1261,46 +1164,43 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
     s01 += (ga * (1.0 - f.x)) * f.y;
     s11 += (ga * f.x) * f.y;
   };
-  // the lane's matches in sweep order, collected up to kMatchCap at a time
-  // (their positions, in LDS), then their data read in batches of
-  // kMatchBatch independent loads; the sweep resumes where it stopped
-  const int tid = (int)threadIdx.x;
-  int cy = cy0, p = 0, e = 0, ro = 0;
-  auto row = [&](int y) {   // start the sweep of SPH cell row y
-    const int k = y - cy0;
-    const int b = staged ? scs[k][cx0 - sx0] : cstart[coff + y * g.nx + cx0];
-    e = staged ? scs[k][cx1 + 1 - sx0] : cstart[coff + y * g.nx + cx1 + 1];
-    ro = staged ? soff[k] - slo[k] : 0;
-    p = b + lane;
-  };
-  row(cy);
-  for (;;) {
-    int n = 0;
-    while (n < kMatchCap) {
-      if (p >= e) {
-        if (++cy > cy1) break;
-        row(cy);
-        continue;
-      }
-      if ((staged ? skey[p + ro] : key[p]) == mykey) smatch[n++][tid] = p;
-      p += kThetaLanes;
-    }
-    for (int m0 = 0; m0 < n; m0 += kMatchBatch) {
-      double2 f[kMatchBatch];
-      double ga[kMatchBatch];
+  for (int ya = cy0; ya <= cy1; ya += kLeanRows) {
+    int rb[kLeanRows], re[kLeanRows];
 #pragma unroll
-      for (int q = 0; q < kMatchBatch; ++q) {
-        if (m0 + q < n) {
-          const int pm = smatch[m0 + q][tid];
-          f[q] = X[pm];
-          ga[q] = (double)galpha[pm];
+    for (int q = 0; q < kLeanRows; ++q) {
+      const int y = ya + q;
+      rb[q] = re[q] = 0;
+      if (y <= cy1) {
+        const int base = coff + y * g.nx;
+        rb[q] = cstart[base + cx0];
+        re[q] = cstart[base + cx1 + 1];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kLeanRows; ++q) {
+      const int e = re[q];
+      for (int p0 = rb[q] + lane; p0 < e; p0 += kThetaLanes * kLeanBatch) {
+        bool hit[kLeanBatch];
+#pragma unroll
+        for (int u = 0; u < kLeanBatch; ++u) {
+          const int p = p0 + u * kThetaLanes;
+          hit[u] = p < e && key[p] == mykey;
         }
-      }
+        double2 f[kLeanBatch];
+        double ga[kLeanBatch];
 #pragma unroll
-      for (int q = 0; q < kMatchBatch; ++q)
-        if (m0 + q < n) add(frac(f[q]), ga[q]);
+        for (int u = 0; u < kLeanBatch; ++u) {
+          if (hit[u]) {
+            const int p = p0 + u * kThetaLanes;
+            f[u] = X[p];
+            ga[u] = (double)galpha[p];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kLeanBatch; ++u)
+          if (hit[u]) add(frac(f[u]), ga[u]);
+      }
     }
-    if (n < kMatchCap) break;   // the sweep has ended
   }
 #pragma unroll
   for (int m = kThetaLanes / 2; m > 0; m >>= 1) {
@@ -1310,8 +1210,8 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
     s11 += __shfl_xor(s11, m, kThetaLanes);
   }
   if (lane == 0) {
-    double4 &pt = part[tcoff + j0 + i0 * ncy];
-    if (first) {
+    double4 &pt = A.part[tcoff + j0 + i0 * ncy];
+    if (A.first) {
       pt = make_double4(s00, s10, s01, s11);
     } else {
       const double4 o = pt;
@@ -1320,18 +1220,22 @@ __device__ __forceinline__ void theta_strip(ThetaStripLds &L, int blk, const The
   }
 }
 
-
 __global__ void __launch_bounds__(kBlock) k_theta_strip_keyed(ThetaKeyed A, Batch B) {
-  __shared__ ThetaStripLds L;
-  theta_strip(L, (int)blockIdx.x, A, B);
+  theta_strip((int)blockIdx.x, A, B);
 }
 
 // adjoint P1: adj_pass1_kernel! + design term + gV += gVs (src/adjoint.jl:30-122,166-185).
 // kExact: nm is already the exact list of these positions; otherwise pairs are
 // tested and the exact list is written to em for P2.
+// The window's three sweep rows are staged and walked one after the other
+// (stage_copy_row), each slot walking its rows in the same order as
+// for_each_nb: a third of the LDS of staging the whole window at once, so
+// that registers, not LDS, bound the occupancy (7 waves per SIMD instead of
+// 5: P1 49.8 -> 46.0 us on the 1M block, the same bits).  A pair reads
+// (x, pi) and (abar, v) as two 16-byte records.
+constexpr int kStageCapP1 = 448;
 struct P1Lds {
   StageTab T;
-  // a pair reads (x, pi) and (abar, v) as two 16-byte records
   float4 sxp[kStageCapP1], sav[kStageCapP1];
 };
 #define P1_PARAMS                                                                           \
@@ -1356,21 +1260,25 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   const float2 xi = xr[s], vi = V[s], ab = abar[s], pii = PI[s];
   const int bc = bcell[s];
   const float ga = galpha[s];
-  stage_plan(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapP1);
+  stage_plan<true>(T, bcell, cst, w0_, w1_, g, S.coff, B.nostage ? 0 : kStageCapP1);
   const Rsrc pir = rsrc(PI, (uint32_t)B.N * 8u);
   const Rsrc vr = rsrc(V, (uint32_t)B.N * 8u);
   const Rsrc abr = rsrc(abar, (uint32_t)B.N * 8u);
-  if (T.ok)
-    stage_copy(T,
-               [&](int j) {
-                 const uint32_t o8 = (uint32_t)j * 8u;   // one offset for the four arrays
-                 return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
-               },
-               [&](int i, const NbXPAV &d) {
-                 sxp[i] = make_float4(d.x.x, d.x.y, d.pi.x, d.pi.y);
-                 sav[i] = make_float4(d.ab.x, d.ab.y, d.v.x, d.v.y);
-               });
-  if (!valid_) return;
+  auto gld = [&](int j) __attribute__((always_inline)) {
+    const uint32_t o8 = (uint32_t)j * 8u;   // one offset for the four arrays
+    return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
+  };
+  auto sst = [&](int i, const NbXPAV &d) __attribute__((always_inline)) {
+    sxp[i] = make_float4(d.x.x, d.x.y, d.pi.x, d.pi.y);
+    sav[i] = make_float4(d.ab.x, d.ab.y, d.v.x, d.v.y);
+  };
+  auto sld = [&](int i) __attribute__((always_inline)) {
+    const float4 q = sxp[i], r = sav[i];
+    return NbXPAV{make_float2(q.x, q.y), make_float2(q.z, q.w), make_float2(r.x, r.y),
+                  make_float2(r.z, r.w)};
+  };
+  // (threads past the scene's end take part in the row copies and barriers;
+  // they load the head of slot w0_, a slot of the scene, and never walk it)
   const NbHead h = nb_head(nmr, cst, s, bc, g, S.coff, B.N);
   const float pti = pii.x, iri = pii.y;
   const float mpti = P.m * pti, tmi = P.twomumass * iri;
@@ -1409,20 +1317,19 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   };
   uint32_t *emw = kExact ? nullptr : em;
   if (T.ok) {
-    for_each_nb<false>(nmr, emw, h, s, xi, B.N, P, stage_rowoff(T, bc, g),
-                [&](int i) __attribute__((always_inline)) {
-                  const float4 q = sxp[i], r = sav[i];
-                  return NbXPAV{make_float2(q.x, q.y), make_float2(q.z, q.w),
-                                make_float2(r.x, r.y), make_float2(r.z, r.w)};
-                },
-                pair);
+    const RowOff ro = stage_rowoff(T, bc, g);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (r) __syncthreads();   // every walk of row r-1 is done with the buffer
+      // two elements in flight per thread: the copy's registers are live
+      // beside the sums, and a row is ~1.1-1.6 elements per thread
+      stage_copy_row<2>(T, r, gld, sst);
+      if (valid_) nb_row<false>(r, nmr, emw, h, s, xi, B.N, P, ro, sld, pair);
+    }
+    if (!valid_) return;
   } else {
-    for_each_nb(nmr, emw, h, s, xi, B.N, P, RowOff{},
-                [&](int i) __attribute__((always_inline)) {
-                  const uint32_t o8 = (uint32_t)i * 8u;
-                  return NbXPAV{ld2(xrr, o8), ld2(pir, o8), ld2(abr, o8), ld2(vr, o8)};
-                },
-                pair);
+    if (!valid_) return;
+    for_each_nb(nmr, emw, h, s, xi, B.N, P, RowOff{}, gld, pair);
   }
   float Ssum = sumS.value();
   // the pair sum of -C ad_v ir_i, with ir_i taken out of the sum
@@ -1440,8 +1347,11 @@ __device__ __forceinline__ void adj_p1(P1Lds &L, int blk, P1_PARAMS) {
   gV[s] = make_float2(gvo.x + gvx.value(), gvo.y + gvy.value());
 }
 
+// The exact-list instance fits 72 VGPRs, 7 waves per SIMD, without spills
+// when asked to; the other (P1 writing the exact list) stays at its 5.
+#define SPH_P1_WAVES __attribute__((amdgpu_waves_per_eu(kExact ? 7 : 5, 8)))
 template <bool kExact>
-__global__ void __launch_bounds__(kBlock) k_adj_p132(P1_PARAMS) {
+__global__ void __launch_bounds__(kBlock) SPH_P1_WAVES k_adj_p132(P1_PARAMS) {
   __shared__ P1Lds L;
   adj_p1<kExact>(L, xcd_block(), P1_ARGS);
 }
@@ -1450,23 +1360,20 @@ __global__ void __launch_bounds__(kBlock) k_adj_p132(P1_PARAMS) {
 // what the pointwise pass and the tape hold, and neither reads what the other
 // writes.  Gather workgroups are spread evenly among the P1 workgroups (block
 // b is a gather block when floor((b+1) nt / n) > floor(b nt / n)), so that
-// they fill the SIMD time P1 leaves while its waves wait on LDS and memory,
-// and the two share one LDS allocation.  Each workgroup runs the body it
-// would run in its own kernel, so every result is unchanged.
-union P1ThetaLds {
-  P1Lds p1;
-  ThetaStripLds th;
-};
+// they fill the SIMD time P1 leaves while its waves wait on LDS and memory;
+// the gather holds no LDS, so the launch has P1's occupancy.  Each workgroup
+// runs the body it would run in its own kernel, so every result is
+// unchanged.
 template <bool kExact>
-__global__ void __launch_bounds__(kBlock) k_adj_p1_theta32(P1_PARAMS, ThetaKeyed A) {
-  __shared__ P1ThetaLds L;
+__global__ void __launch_bounds__(kBlock) SPH_P1_WAVES k_adj_p1_theta32(P1_PARAMS, ThetaKeyed A) {
+  __shared__ P1Lds L;
   const int lb = xcd_block();
   const long long nt = B.ntblk, n = gridDim.x;
   const int tb = (int)((long long)lb * nt / n);
   if ((int)((long long)(lb + 1) * nt / n) > tb)
-    theta_strip(L.th, tb, A, B);
+    theta_strip(tb, A, B);
   else
-    adj_p1<kExact>(L.p1, lb - tb, P1_ARGS);
+    adj_p1<kExact>(L, lb - tb, P1_ARGS);
 }
 
 // adjoint P2: adj_pass2_kernel! and gX += gXs (src/adjoint.jl:126-156,187-191)

@@ -34,6 +34,7 @@ struct StageTab {
   int off[kStageSegs + 1];    // LDS position of each segment; off[kStageSegs] = total
   int delta[kStageSegs];      // lo - off: slot = LDS position + delta of its segment
   int ya, ok, nseg;           // nseg: segments in use (3 per own cell row)
+  int ptot[3];                // phased plans: slots of the segments of row r (k % 3 == r)
 };
 
 // Plans the staging of the window of slots [w0, w1) (w1 > w0); every thread
@@ -41,7 +42,10 @@ struct StageTab {
 // bcell[slot] is the slot's binning cell (ascending over the slots), and
 // cst_at(i) returns cstart[i] (the first slot of cell i - coff).  A segment
 // of a row outside the grid is empty and has lo = INT_MAX.
-template <class CstAt>
+// kPhased: the segments of sweep row r (k % 3 == r) share one LDS buffer of
+// cap slots, copied one row after the other (stage_copy_row): the positions
+// of a row's segments are its own prefix, and T.ok needs each row to fit.
+template <bool kPhased = false, class CstAt>
 __device__ __forceinline__ void stage_plan_at(StageTab &T, const int *__restrict__ bcell,
                                               CstAt &&cst_at, int w0, int w1, const Grid &g,
                                               int coff, int cap) {
@@ -60,22 +64,40 @@ __device__ __forceinline__ void stage_plan_at(StageTab &T, const int *__restrict
         len = cst_at(base + xhi + 1) - lo;
       }
     }
-    int inc = len;
+    int inc = len, total = 0;
+    bool fits;
+    if constexpr (kPhased) {
 #pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-      const int t = __shfl_up(inc, d, 64);
-      if (k >= d) inc += t;
+      for (int d = 3; d < kStageSegs; d += 3) {
+        const int t = __shfl_up(len, d, 64);
+        if (k >= d) inc += t;
+      }
+      const int t0 = __shfl(inc, kStageSegs - 3, 64), t1 = __shfl(inc, kStageSegs - 2, 64),
+                t2 = __shfl(inc, kStageSegs - 1, 64);
+      if (k == 0) {
+        T.ptot[0] = t0;
+        T.ptot[1] = t1;
+        T.ptot[2] = t2;
+      }
+      fits = t0 <= cap && t1 <= cap && t2 <= cap;
+    } else {
+#pragma unroll
+      for (int d = 1; d < 16; d <<= 1) {
+        const int t = __shfl_up(inc, d, 64);
+        if (k >= d) inc += t;
+      }
+      total = __shfl(inc, kStageSegs - 1, 64);
+      fits = total <= cap;
     }
-    const int total = __shfl(inc, kStageSegs - 1, 64);
     if (k < kStageSegs) {
       T.lo[k] = lo;
       T.off[k] = inc - len;
       T.delta[k] = lo - (inc - len);
     }
     if (k == 0) {
-      T.off[kStageSegs] = total;
+      T.off[kStageSegs] = total;   // unused by phased plans (T.ptot)
       T.ya = a.y;
-      T.ok = (nrow <= kStageOwnRows && total <= cap) ? 1 : 0;
+      T.ok = (nrow <= kStageOwnRows && fits) ? 1 : 0;
       T.nseg = 3 * (nrow < kStageOwnRows ? nrow : kStageOwnRows);
     }
   }
@@ -115,6 +137,40 @@ __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) 
     }
 #pragma unroll
     for (int u = 0; u < kStageUnroll; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < total) st(i, d[u]);
+    }
+  }
+  __syncthreads();
+}
+
+// Copies the segments of sweep row r of a phased plan (stage_plan_at<true>)
+// to LDS positions 0 .. T.ptot[r]-1, as stage_copy does for a whole plan.
+template <int kUnroll = kStageUnroll, class Ld, class St>
+__device__ __forceinline__ void stage_copy_row(const StageTab &T, int r, Ld &&ld, St &&st) {
+  const int total = T.ptot[r];
+  const int nseg = T.nseg;
+  int off[kStageOwnRows];
+#pragma unroll
+  for (int q = 0; q < kStageOwnRows; ++q) off[q] = T.off[3 * q + r];
+  for (int i0 = (int)threadIdx.x; i0 < total; i0 += kUnroll * (int)blockDim.x) {
+    decltype(ld(0)) d[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < total) {
+        // segment of i: the last of row r in use starting at or before i
+        int q = 0;
+#pragma unroll
+        for (int p = 1; p < kStageOwnRows; ++p) {
+          if (3 * p + r >= nseg) break;
+          q += (i >= off[p]) ? 1 : 0;
+        }
+        d[u] = ld(i + T.delta[3 * q + r]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < total) st(i, d[u]);
     }
