@@ -1108,16 +1108,19 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
 // A workgroup takes a strip of kThetaStrip theta cells of one theta row,
 // kThetaLanes lanes per cell; lane l of a cell sweeps every kThetaLanes-th
 // slot (from the l-th) of each SPH cell row of the cell's range, the rows in
-// order, and a butterfly adds the lanes' sums.  A lane reads the cell starts
-// of all its rows at once (kLeanRows rows a round), then its candidates' keys
-// kLeanBatch at a time, and the positions and galpha of the matches among
-// them together.  It holds no LDS, so a launch that runs it beside P1 keeps
-// P1's occupancy.  Workgroups are dealt to the scenes of a batched state in
-// order (Scene32::tboff).
-// (Measured on the 1M block: alone 21 us against 34 us for a version that
-// first staged the strip's keys and cell starts in 31 KB of LDS, and inside
-// P1's launch 60.5 against 68.7 us, the staged version's LDS having held P1
-// at five waves per SIMD.)
+// order, and a butterfly adds the lanes' sums.  The gather is bound by its
+// dependent memory rounds, so a lane reads the cell starts of all its rows at
+// once (kLeanRows rows a round), then the keys of up to kLeanKeys of its
+// candidates over those rows together, then the positions and galpha of the
+// matches among them kLeanBatch at a time.  It holds no LDS, so a launch that
+// runs it beside P1 keeps P1's occupancy.  Workgroups are dealt to the scenes
+// of a batched state in order (Scene32::tboff).
+// (Measured on the 1M block, P1 + gather in one launch: 68.7 us for a version
+// that first staged the strip's keys and cell starts in 31 KB of LDS, which
+// held P1 at five waves per SIMD; 60.5 us reading keys from global memory
+// row by row, four at a time; 57.9 us with the keys of all rows in one
+// batch.  Loading the range tables beside the sweep level: no change.)
+constexpr int kLeanKeys = 16;
 constexpr int kLeanBatch = 4;
 constexpr int kLeanRows = 4;
 __device__ __forceinline__ void theta_strip(int blk, const ThetaKeyed &A, const Batch &B) {
@@ -1164,41 +1167,55 @@ __device__ __forceinline__ void theta_strip(int blk, const ThetaKeyed &A, const 
     s01 += (ga * (1.0 - f.x)) * f.y;
     s11 += (ga * f.x) * f.y;
   };
+  // the lane's candidates of kLeanRows rows, in sweep order: candidate t of
+  // row q is slot rb[q] + kThetaLanes t (t < nq[q]), numbered k = pre[q] + t
   for (int ya = cy0; ya <= cy1; ya += kLeanRows) {
-    int rb[kLeanRows], re[kLeanRows];
+    int rb[kLeanRows], pre[kLeanRows];
+    int total = 0;
 #pragma unroll
     for (int q = 0; q < kLeanRows; ++q) {
       const int y = ya + q;
-      rb[q] = re[q] = 0;
+      int b = 0, e = 0;
       if (y <= cy1) {
         const int base = coff + y * g.nx;
-        rb[q] = cstart[base + cx0];
-        re[q] = cstart[base + cx1 + 1];
+        b = cstart[base + cx0];
+        e = cstart[base + cx1 + 1];
       }
+      rb[q] = b + lane;
+      pre[q] = total;
+      total += e > rb[q] ? (e - rb[q] + kThetaLanes - 1) / kThetaLanes : 0;
     }
+    auto slot_of = [&](int k) {   // slot of candidate k < total
+      int q = 0;
 #pragma unroll
-    for (int q = 0; q < kLeanRows; ++q) {
-      const int e = re[q];
-      for (int p0 = rb[q] + lane; p0 < e; p0 += kThetaLanes * kLeanBatch) {
-        bool hit[kLeanBatch];
+      for (int r = 1; r < kLeanRows; ++r) q += (k >= pre[r]) ? 1 : 0;
+      return rb[q] + kThetaLanes * (k - pre[q]);
+    };
+    // kLeanKeys keys at a time, then the matches among them kLeanBatch at a time
+    for (int k0 = 0; k0 < total; k0 += kLeanKeys) {
+      int kv[kLeanKeys];
 #pragma unroll
-        for (int u = 0; u < kLeanBatch; ++u) {
-          const int p = p0 + u * kThetaLanes;
-          hit[u] = p < e && key[p] == mykey;
-        }
+      for (int u = 0; u < kLeanKeys; ++u) kv[u] = k0 + u < total ? key[slot_of(k0 + u)] : -1;
+      uint32_t hits = 0;
+#pragma unroll
+      for (int u = 0; u < kLeanKeys; ++u) hits |= (kv[u] == mykey ? 1u : 0u) << u;
+      while (hits != 0) {
+        bool ok[kLeanBatch];
         double2 f[kLeanBatch];
         double ga[kLeanBatch];
 #pragma unroll
         for (int u = 0; u < kLeanBatch; ++u) {
-          if (hit[u]) {
-            const int p = p0 + u * kThetaLanes;
+          ok[u] = hits != 0;
+          if (ok[u]) {
+            const int p = slot_of(k0 + __builtin_ctz(hits));
+            hits &= hits - 1;
             f[u] = X[p];
             ga[u] = (double)galpha[p];
           }
         }
 #pragma unroll
         for (int u = 0; u < kLeanBatch; ++u)
-          if (hit[u]) add(frac(f[u]), ga[u]);
+          if (ok[u]) add(frac(f[u]), ga[u]);
       }
     }
   }
