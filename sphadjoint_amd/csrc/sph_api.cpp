@@ -535,6 +535,7 @@ struct TapeEpoch {
   int64_t first = 0;  // first tape entry of the epoch
   int *ids = nullptr, *scell = nullptr, *cstart = nullptr;
   int *bcell = nullptr;  // FP64 form: binning cells (the f32c form's scell is its binning cell)
+  int *lane = nullptr;   // f32c form: the pair passes' thread order of the epoch
 };
 
 struct sph_tape {
@@ -747,19 +748,18 @@ TapeEpoch &tape_epoch_for(sph_tape *tp, const sph_state *st) {
     e.ids = dalloc<int>(tp->N);
     e.scell = dalloc<int>(tp->N);
     e.cstart = dalloc<int>(tp->ncell + 1);
-    if (st->bcell) e.bcell = dalloc<int>(tp->N);
   }
+  if (st->bcell && !e.bcell) e.bcell = dalloc<int>(tp->N);
+  if (st->precision == SPH_F32C && !e.lane) e.lane = dalloc<int>(tp->N);
   e.owner = st;
   e.serial = st->epoch;
   e.first = tp->n + 1;
-  hipStream_t s = st->ctx->stream;
-  HIPC(hipMemcpyAsync(e.ids, st->ids, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
-  HIPC(hipMemcpyAsync(e.scell, st->scell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
-  if (st->bcell) {
-    if (!e.bcell) e.bcell = dalloc<int>(tp->N);
-    HIPC(hipMemcpyAsync(e.bcell, st->bcell, sizeof(int) * tp->N, hipMemcpyDeviceToDevice, s));
-  }
-  HIPC(hipMemcpyAsync(e.cstart, st->cstart, sizeof(int) * (tp->ncell + 1), hipMemcpyDeviceToDevice, s));
+  const bool f32c = st->precision == SPH_F32C;
+  launch_epoch_record(st->ctx->stream,
+                      EpochLayout{st->ids, st->scell, st->bcell, f32c ? st->lane : nullptr,
+                                  st->cstart},
+                      (int)tp->N, tp->ncell + 1,
+                      EpochLayoutOut{e.ids, e.scell, e.bcell, e.lane, e.cstart});
   tp->epochs.push_back(e);
   return tp->epochs.back();
 }
@@ -1757,6 +1757,7 @@ int sph_tape_destroy(sph_tape *tp) {
         dfree(e.ids);
         dfree(e.scell);
         dfree(e.bcell);
+        dfree(e.lane);
         dfree(e.cstart);
       }
     dfree(tp->X);
@@ -2097,13 +2098,16 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       timed(c, K_NLBUILD, [&] {
         const double2 *X0 = tape->X + (size_t)(E.first - 1) * n;
         if (f32c) {
-          // the thread order of the epoch with its relative coordinates; a
-          // tape with each step's exact masks and (pterm, invrho) needs no mask
-          // list (without the latter the R pass recomputes the density over it)
-          launch_xrel_f32c(s, X0, E.scell, B, ws->xr32, ws->lane);
-          if (!(tape->EM && tape->keep_pi))
+          // a tape with each step's exact masks and (pterm, invrho) needs no
+          // mask list and takes the epoch's thread order from its record;
+          // otherwise the thread order with the relative coordinates of the
+          // epoch's first entry, and the mask list over them (without stored
+          // (pterm, invrho) the R pass recomputes the density over it)
+          if (!(tape->EM && tape->keep_pi)) {
+            launch_xrel_f32c(s, X0, E.scell, B, ws->xr32, ws->lane);
             launch_nm_build_f32c(s, ws->xr32, E.scell, E.cstart, BS.at(TK_NLBUILD), ws->nm,
                                  ws->flags);
+          }
         } else {
           launch_nl_build_f64(s, X0, E.scell, E.cstart, g, tape->kc, maxnb, ws->nl, ws->nbcnt,
                               ws->flags);
@@ -2113,27 +2117,43 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       nl = ws->nl;
       nbcnt = ws->nbcnt;
       nm = ws->nm;
-      lane = ws->lane;
+      lane = f32c && tape->EM && tape->keep_pi ? E.lane : ws->lane;
     }
     if (f32c) {
       const float2 *Vk = tape->V32 + off;
       const float2 *Vnext = tape->V32 + (size_t)k * n;
+      const float2 *PIk = ws->PI32;
+      bool remap_r = false;   // this step's R pass ran with the remap
       if (e != cur) {
         // entry k+1 was stored in the next epoch's slot order: move the
-        // adjoint state and V_{k+1} into this epoch's order
+        // adjoint state and V_{k+1} into this epoch's order (gXs32 is free
+        // between steps: P1 writes it before P2 reads it); with stored
+        // (pterm, invrho) the step's pointwise R pass runs in the same launch
+        // (an epoch's last step never had its R pass fused into a P2)
+        remap_r = tape->keep_pi;
+        if (remap_r) {
+          ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
+          PIk = tape->PI32 + off;
+        }
         timed(c, K_REMAP, [&] {
-          // (gXs32 is free between steps: P1 writes it before P2 reads it)
           launch_inverse(s, tape->epochs[cur].ids, N, ws->inv);
-          launch_gather_remap3f(s, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids, N, ws->t2f,
-                                ws->gXs32, ws->Vtmp32);
+          if (remap_r) {
+            launch_remap_rpw_f32c(s, Xk, E.scell, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids,
+                                  theta->d, BS.at(TK_ADJ_R), ws->t2f, ws->gXs32, ws->xr32,
+                                  ws->abar32, ws->galpha32, ws->tkey, ws->tlev, ws->tstamp);
+          } else {
+            launch_gather_remap3f(s, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids, N, ws->t2f,
+                                  ws->gXs32, ws->Vtmp32);
+          }
           std::swap(ws->gX32, ws->t2f);
           std::swap(ws->gV32, ws->gXs32);
         });
-        Vnext = ws->Vtmp32;
+        if (!remap_r) Vnext = ws->Vtmp32;
         cur = e;
       }
-      const float2 *PIk = ws->PI32;
-      if (r_done) {
+      if (remap_r) {
+        // done above
+      } else if (r_done) {
         // this step's R pass ran in the previous step's P2 launch
         ws->tstamp = r_stamp;
         PIk = tape->PI32 + off;

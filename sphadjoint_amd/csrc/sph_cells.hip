@@ -356,6 +356,20 @@ __global__ void k_scatter_i32(const int *__restrict__ src, const int *__restrict
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < N) dst[ids[s]] = src[s];
 }
+// A tape's record of an epoch's layout, in one launch: thread i copies slot i
+// of the per-slot arrays given (ids, sweep cells, binning cells, thread order;
+// a null source is skipped) and entry i of the cell starts.
+__global__ void k_epoch_record(EpochLayout src, int N, int ncs, EpochLayoutOut dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) {
+    dst.ids[i] = src.ids[i];
+    dst.scell[i] = src.scell[i];
+    if (src.bcell) dst.bcell[i] = src.bcell[i];
+    if (src.lane) dst.lane[i] = src.lane[i];
+  }
+  if (i < ncs) dst.cstart[i] = src.cstart[i];
+}
+
 __global__ void k_inverse(const int *__restrict__ ids, int N, int *__restrict__ inv) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < N) inv[ids[s]] = s;
@@ -433,6 +447,11 @@ void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, cons
 }
 void launch_scatter_i32(hipStream_t s, const int *src, const int *ids, int N, int *dst) {
   if (N > 0) k_scatter_i32<<<nblocks(N), kBlock, 0, s>>>(src, ids, N, dst);
+}
+void launch_epoch_record(hipStream_t s, const EpochLayout &src, int N, int ncs,
+                         const EpochLayoutOut &dst) {
+  const int n = N > ncs ? N : ncs;
+  if (n > 0) k_epoch_record<<<nblocks(n), kBlock, 0, s>>>(src, N, ncs, dst);
 }
 void launch_inverse(hipStream_t s, const int *ids, int N, int *inv) {
   if (N > 0) k_inverse<<<nblocks(N), kBlock, 0, s>>>(ids, N, inv);

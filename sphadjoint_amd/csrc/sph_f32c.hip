@@ -1097,6 +1097,30 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
   rpw_slot(R, s, bcell[s], gX[s], S, B);
 }
 
+// The pointwise R pass of a reverse step that opens an epoch, fused with the
+// gather that brings the adjoint state and V_{k+1} from the next epoch's slot
+// order into this one's: slot s takes gX, gV and V_{k+1} from slot
+// inv[ids[s]] of the next epoch (inv: that epoch's inverse of its ids), keeps
+// gX in gXn and runs rpw_finish on them, R.gV receiving this order's gV (a
+// buffer other than gVo).  The same values as the remap gather followed by
+// k_adj_rpw32.
+__global__ void __launch_bounds__(kBlock)
+k_remap_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__ gXo,
+              const float2 *__restrict__ gVo, const float2 *__restrict__ Vno,
+              const int *__restrict__ inv, const int *__restrict__ ids, Batch B,
+              float2 *__restrict__ gXn) {
+  const int blk = xcd_block();
+  const int scn = B.b2s[blk];
+  const Scene32 S = B.sc[scn];
+  const int sl = (blk - B.boff[scn]) * B.bs + (int)threadIdx.x;
+  if (sl >= S.g.N) return;
+  const int s = S.off + sl;
+  const int o = inv[ids[s]];
+  const float2 gx = gXo[o];
+  gXn[s] = gx;
+  rpw_finish(R, RpwIn{R.X[s], gVo[o], Vno[o]}, s, bcell[s], gx, S, B);
+}
+
 // ---- gtheta gather over the keys of the pointwise pass -----------------------
 
 // The gather of sph_theta.hip over per-slot keys written by the f32c adjoint's
@@ -1528,6 +1552,15 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
                          int tstamp) {
   const RpwArgs R{X, Vnext, theta, xr, gV, abar, galpha, tkey, tlev, tstamp};
   if (B.N > 0) k_adj_rpw32<<<B.nblk, B.bs, 0, s>>>(R, bcell, gX, B);
+}
+void launch_remap_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell,
+                           const float2 *gXo, const float2 *gVo, const float2 *Vno,
+                           const int *inv, const int *ids, const double *theta, const Batch &B,
+                           float2 *gXn, float2 *gVn, float2 *xr, float2 *abar, float *galpha,
+                           int *tkey, unsigned long long *tlev, int tstamp) {
+  if (gVn == gVo) throw std::logic_error("remapped R pass writes gV over its source");
+  const RpwArgs R{X, nullptr, theta, xr, gVn, abar, galpha, tkey, tlev, tstamp};
+  if (B.N > 0) k_remap_rpw32<<<B.nblk, B.bs, 0, s>>>(R, bcell, gXo, gVo, Vno, inv, ids, B, gXn);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,

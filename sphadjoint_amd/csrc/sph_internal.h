@@ -205,6 +205,16 @@ void launch_materialise(hipStream_t s, const double2 *Xref, const int *ids, cons
                         const int *cstart, Grid g, CellConstF64 k, int maxnb, long long sm,
                         long long si, int *indices, int *nbcount, int *flags);
 void launch_scatter_i32(hipStream_t s, const int *src, const int *ids, int N, int *dst);
+// an epoch's layout: ids, sweep cells and cell starts; binning cells (FP64
+// form) and the pair passes' thread order (f32c form) where present
+struct EpochLayout {
+  const int *ids, *scell, *bcell, *lane, *cstart;
+};
+struct EpochLayoutOut {
+  int *ids, *scell, *bcell, *lane, *cstart;
+};
+void launch_epoch_record(hipStream_t s, const EpochLayout &src, int N, int ncs,
+                         const EpochLayoutOut &dst);
 void launch_inverse(hipStream_t s, const int *ids, int N, int *inv);
 void launch_gather_remap2(hipStream_t s, const double2 *src, const int *inv_src,
                           const int *ids_dst, int N, double2 *dst);
