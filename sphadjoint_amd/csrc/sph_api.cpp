@@ -2061,10 +2061,26 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       else launch_theta_nodes_f64(s, ws->tpart, T, ws->gtheta);
     });
   };
-  // the adjoint state into the slot order of the final entry
+  // the adjoint state into the slot order of the final entry; with stored
+  // (pterm, invrho) and exact masks (no list to rebuild over the epoch's first
+  // positions, which would overwrite the relative coordinates) the final
+  // step's pointwise R pass runs in the same launch
+  bool r_done = false;   // f32c form: the R pass of step k already ran (in P2 of step k+1)
+  int r_stamp = 0;
   timed(c, K_REMAP, [&] {
     launch_inverse(s, ids_in, N, ws->inv);
-    if (f32c) {
+    if (f32c && tape->keep_pi && tape->EM && tape->n >= 1) {
+      const TapeEpoch &E = tape->epochs[cur];
+      const size_t off = (size_t)(tape->n - 1) * n;
+      r_stamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
+      launch_remap_rpw_f32c(s, tape->X + off, E.scell, ws->gX32, ws->gV32,
+                            tape->V32 + (size_t)tape->n * n, ws->inv, E.ids, theta->d,
+                            BS.at(TK_ADJ_R), ws->t2f, ws->gXs32, ws->xr32, ws->abar32,
+                            ws->galpha32, ws->tkey, ws->tlev, r_stamp, false);
+      std::swap(ws->gX32, ws->t2f);
+      std::swap(ws->gV32, ws->gXs32);
+      r_done = true;
+    } else if (f32c) {
       launch_gather_remap2f(s, ws->gX32, ws->inv, tape->epochs[cur].ids, N, ws->t2f);
       std::swap(ws->gX32, ws->t2f);
       launch_gather_remap2f(s, ws->gV32, ws->inv, tape->epochs[cur].ids, N, ws->t2f);
@@ -2083,8 +2099,6 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const uint32_t *nm = use_given ? lnm : ws->nm;
   const int *lane = use_given ? llane : ws->lane;
   int listed = use_given ? (int)tape->epochs.size() - 1 : -1;  // epoch whose list `nl` holds
-  bool r_done = false;   // f32c form: the R pass of step k already ran (in P2 of step k+1)
-  int r_stamp = 0;
   for (int64_t k = tape->n; k >= 1; --k) {
     const int e = tape->entry_epoch[k];
     const TapeEpoch &E = tape->epochs[e];
@@ -2140,7 +2154,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
           if (remap_r) {
             launch_remap_rpw_f32c(s, Xk, E.scell, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids,
                                   theta->d, BS.at(TK_ADJ_R), ws->t2f, ws->gXs32, ws->xr32,
-                                  ws->abar32, ws->galpha32, ws->tkey, ws->tlev, ws->tstamp);
+                                  ws->abar32, ws->galpha32, ws->tkey, ws->tlev, ws->tstamp,
+                                  true);
           } else {
             launch_gather_remap3f(s, ws->gX32, ws->gV32, Vnext, ws->inv, E.ids, N, ws->t2f,
                                   ws->gXs32, ws->Vtmp32);

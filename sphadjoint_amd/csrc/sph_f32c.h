@@ -166,13 +166,14 @@ void launch_adj_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell, cons
                          const double *theta, const Batch &B, float2 *xr, float2 *gX,
                          float2 *gV, float2 *abar, float *galpha, int *tkey,
                          unsigned long long *tlev, int tstamp);
-// the same R pass on the adjoint state and V_{k+1} of the next epoch's slot
-// order (gXo, gVo, Vno), gathered through inv[ids[s]]: gX into gXn, gV into gVn
+// the same R pass on the adjoint state (gXo, gVo) of another slot order,
+// gathered through inv[ids[s]] (V_{k+1}, Vno, too when remap_v, else it is in
+// this order already): gX into gXn, gV into gVn
 void launch_remap_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell,
                            const float2 *gXo, const float2 *gVo, const float2 *Vno,
                            const int *inv, const int *ids, const double *theta, const Batch &B,
                            float2 *gXn, float2 *gVn, float2 *xr, float2 *abar, float *galpha,
-                           int *tkey, unsigned long long *tlev, int tstamp);
+                           int *tkey, unsigned long long *tlev, int tstamp, bool remap_v);
 // P1 over nm: the exact list when `exact`, else the mask list, whose exact
 // subset is then written to em for P2
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,

@@ -1098,12 +1098,13 @@ k_adj_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__
 }
 
 // The pointwise R pass of a reverse step that opens an epoch, fused with the
-// gather that brings the adjoint state and V_{k+1} from the next epoch's slot
-// order into this one's: slot s takes gX, gV and V_{k+1} from slot
-// inv[ids[s]] of the next epoch (inv: that epoch's inverse of its ids), keeps
-// gX in gXn and runs rpw_finish on them, R.gV receiving this order's gV (a
-// buffer other than gVo).  The same values as the remap gather followed by
+// gather that brings the adjoint state (and with kRemapV V_{k+1}) from
+// another slot order into the epoch's: slot s takes gX, gV (and V_{k+1}) from
+// slot inv[ids[s]] of that order (inv: the inverse of its ids), keeps gX in
+// gXn and runs rpw_finish on them, R.gV receiving this order's gV (a buffer
+// other than gVo).  The same values as the remap gathers followed by
 // k_adj_rpw32.
+template <bool kRemapV>
 __global__ void __launch_bounds__(kBlock)
 k_remap_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict__ gXo,
               const float2 *__restrict__ gVo, const float2 *__restrict__ Vno,
@@ -1118,7 +1119,7 @@ k_remap_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict
   const int o = inv[ids[s]];
   const float2 gx = gXo[o];
   gXn[s] = gx;
-  rpw_finish(R, RpwIn{R.X[s], gVo[o], Vno[o]}, s, bcell[s], gx, S, B);
+  rpw_finish(R, RpwIn{R.X[s], gVo[o], Vno[kRemapV ? o : s]}, s, bcell[s], gx, S, B);
 }
 
 // ---- gtheta gather over the keys of the pointwise pass -----------------------
@@ -1557,10 +1558,14 @@ void launch_remap_rpw_f32c(hipStream_t s, const double2 *X, const int *bcell,
                            const float2 *gXo, const float2 *gVo, const float2 *Vno,
                            const int *inv, const int *ids, const double *theta, const Batch &B,
                            float2 *gXn, float2 *gVn, float2 *xr, float2 *abar, float *galpha,
-                           int *tkey, unsigned long long *tlev, int tstamp) {
+                           int *tkey, unsigned long long *tlev, int tstamp, bool remap_v) {
   if (gVn == gVo) throw std::logic_error("remapped R pass writes gV over its source");
   const RpwArgs R{X, nullptr, theta, xr, gVn, abar, galpha, tkey, tlev, tstamp};
-  if (B.N > 0) k_remap_rpw32<<<B.nblk, B.bs, 0, s>>>(R, bcell, gXo, gVo, Vno, inv, ids, B, gXn);
+  if (B.N <= 0) return;
+  if (remap_v)
+    k_remap_rpw32<true><<<B.nblk, B.bs, 0, s>>>(R, bcell, gXo, gVo, Vno, inv, ids, B, gXn);
+  else
+    k_remap_rpw32<false><<<B.nblk, B.bs, 0, s>>>(R, bcell, gXo, gVo, Vno, inv, ids, B, gXn);
 }
 void launch_adj_p1_f32c(hipStream_t s, const double2 *X, const float2 *xr, const float2 *V,
                         const float2 *PI, const float2 *abar, const float *galpha,
