@@ -70,8 +70,11 @@ ALG_STEP_BYTES = {"f64": {"fwd": 96, "fwd_adj": 304}, "f32c": {"fwd": 72, "fwd_a
 IMPL_BYTES = {
     "f64": {"density": 16 + 16, "force_integrate": 48 + 32, "adjoint_r": 64 + 56,
             "adjoint_p1": 88 + 40, "adjoint_p2": 56 + 16},
-    # f32c: X double2; V, xr, PI, abar, gX, gV float2; galpha, grho float; cell int
-    "f32c": {"density": 8 + 8, "force_integrate": 44 + 32, "adjoint_r": 44 + 28,
+    # f32c: X double2; V, xr, PI, abar, gX, gV float2; galpha, grho float; cell int.
+    # Its R pass runs inside P2, or at an epoch's last step inside the remap
+    # gather (ids, inverse, gathered gX, gV, V'; X; writes gX, gV, xr, abar,
+    # galpha, key): the latter's arrays
+    "f32c": {"density": 8 + 8, "force_integrate": 44 + 32, "adjoint_r": 48 + 40,
              "adjoint_p1": 64 + 20, "adjoint_p2": 32 + 8},
 }
 # measured HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
@@ -80,7 +83,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_block_1000x1000.json"
 # VALU instructions per wave of each kernel (rocprofv3 SQ_INSTS_VALU / SQ_WAVES)
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_busy_block_1000x1000.json")
 SYMBOL_F32C = {"density": "k_density32", "force_integrate": "k_force32",
-               "adjoint_r": "k_adj_rpw32", "adjoint_p1": "k_adj_p1_theta32<true>",
+               "adjoint_r": "k_remap_rpw32<true>", "adjoint_p1": "k_adj_p1_theta32<true>",
                "adjoint_p2": "k_adj_p232<true>"}
 SYMBOL = {"f64": {"density": "k_density", "force_integrate": "k_force", "adjoint_r": "k_adj_r",
                   "adjoint_p1": "k_adj_p1", "adjoint_p2": "k_adj_p2"},
