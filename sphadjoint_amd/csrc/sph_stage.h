@@ -116,9 +116,11 @@ template <class Ld, class St>
 __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) {
   const int total = T.off[kStageSegs];
   const int nseg = T.nseg;
-  int off[kStageSegs];
-#pragma unroll
-  for (int q = 0; q < kStageSegs; ++q) off[q] = T.off[q];
+  // a window inside one cell row (the common case) has three segments: two
+  // compares against offsets held in scalar registers; a window over more
+  // rows compares against the offsets in LDS
+  const int o1 = __builtin_amdgcn_readfirstlane(T.off[1]);
+  const int o2 = __builtin_amdgcn_readfirstlane(T.off[2]);
   for (int i0 = (int)threadIdx.x; i0 < total; i0 += kStageUnroll * (int)blockDim.x) {
     decltype(ld(0)) d[kStageUnroll];
 #pragma unroll
@@ -126,12 +128,8 @@ __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) 
       const int i = i0 + u * (int)blockDim.x;
       if (i < total) {
         // segment of i: the last one in use starting at or before i (off ascends)
-        int k = 0;
-#pragma unroll
-        for (int q = 1; q < kStageSegs; ++q) {
-          if (q >= nseg) break;
-          k += (i >= off[q]) ? 1 : 0;
-        }
+        int k = (i >= o1 ? 1 : 0) + (i >= o2 ? 1 : 0);
+        for (int q = 3; q < nseg; ++q) k += (i >= T.off[q]) ? 1 : 0;
         d[u] = ld(i + T.delta[k]);
       }
     }
