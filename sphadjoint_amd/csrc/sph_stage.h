@@ -108,9 +108,12 @@ __device__ __forceinline__ void stage_plan_at(StageTab &T, const int *__restrict
 // index i.  Called by every thread of the workgroup when T.ok.  Each thread
 // issues the loads of kStageUnroll elements before storing any of them, so a
 // window's copy costs one load latency per kStageUnroll*blockDim elements.
-// The segment of an element is found among the T.nseg segments in use (the
-// count is the same in every lane, so the search is a scalar loop) and its
-// slot offset read from T.delta.
+// The segment of an element is the last of the T.nseg segments in use that
+// starts at or before it: two compares for a window inside one cell row
+// (three segments), one more per further segment, whose offset is read from
+// LDS (a loop over an array of offsets in registers, indexed at run time,
+// compiles to s_set_gpr_idx per segment and element); its slot offset is
+// read from T.delta.
 constexpr int kStageUnroll = 4;
 template <class Ld, class St>
 __device__ __forceinline__ void stage_copy(const StageTab &T, Ld &&ld, St &&st) {
