@@ -2887,6 +2887,9 @@ int sph_autotune(sph_state *st, const sph_field *theta, int64_t steps, int32_t *
       }
       for (int k = 0; k < TK_COUNT; ++k) {
         const int64_t n = c->launches[kid[k]];
+        // (on this tape the R pass runs inside P2 and the epoch remaps, timed
+        // with them: adjoint_r, whose own launch serves other tapes, keeps its
+        // default size)
         if (n == 0) continue;
         // P1 at 256 threads carries the gtheta gather in its launch: compare
         // P1 + gather at every size
