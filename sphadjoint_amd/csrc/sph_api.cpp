@@ -579,7 +579,7 @@ struct sph_workspace {
   uint32_t *nm = nullptr;                                  // f32c form: row bitmasks
   uint32_t *em = nullptr;                                  // f32c form: exact list of a step
   int *lane = nullptr;                                     // f32c form: pair-pass thread order
-  int *tkey = nullptr;        // f32c form: interp-cell key of each slot (keyed gtheta gather)
+  int *tkey = nullptr;        // interp-cell key of each slot (keyed gtheta gather)
   unsigned long long *tlev = nullptr;  // gtheta sweep level of the reverse steps (raise_level)
   int tstamp = 0;             // stamp of the current reverse step (1, 2, ...)
   // FP64 form: the one-cell gtheta sweep ranges (theta columns, then rows) of
@@ -2008,11 +2008,11 @@ void ensure_ws_list(sph_workspace *ws, int64_t maxnb) {
     ws->tlev = dalloc<unsigned long long>(1);
     HIPC(hipMemsetAsync(ws->tlev, 0, sizeof(unsigned long long), ws->ctx->stream));
   }
+  if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);   // the gtheta gather's keys
   if (ws->precision == SPH_F32C) {
     if (!ws->nm) ws->nm = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->em) ws->em = dalloc<uint32_t>((size_t)ws->N * 3 * kMaskWords);
     if (!ws->lane) ws->lane = dalloc<int>((size_t)ws->N);
-    if (!ws->tkey) ws->tkey = dalloc<int>((size_t)ws->N);
     return;
   }
   if (ws->nl_cap < maxnb) {
@@ -2039,10 +2039,16 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
   const BatchStore &BS = tape->bs;
   const ThetaGrid T = make_theta_grid(ws->p, tape->rc);
   if (!f32c && (ws->tr_nx != g.nx || ws->tr_ny != g.ny || ws->tr_cs != T.cs)) {
-    // the FP64 R passes compare each slot's binning cell with these ranges
+    // the FP64 R passes compare each slot's binning cell with these ranges,
+    // and the gtheta gather sweeps them: the one-cell ranges of the theta
+    // columns, then of the theta rows, then the quarter-cell ranges likewise
     std::vector<int2> tr;
-    for (int i = 0; i < T.ngx - 1; ++i) tr.push_back(theta_cell_range(i, T.ngx - 1, T.hx, T.cs, g.nx));
-    for (int j = 0; j < T.ngy - 1; ++j) tr.push_back(theta_cell_range(j, T.ngy - 1, T.hy, T.cs, g.ny));
+    const int ncx = T.ngx - 1, ncy = T.ngy - 1;
+    const double ex = 0.25 * T.cs;
+    for (int i = 0; i < ncx; ++i) tr.push_back(theta_cell_range(i, ncx, T.hx, T.cs, g.nx));
+    for (int j = 0; j < ncy; ++j) tr.push_back(theta_cell_range(j, ncy, T.hy, T.cs, g.ny));
+    for (int i = 0; i < ncx; ++i) tr.push_back(theta_cell_range_tight(i, ncx, T.hx, T.cs, g.nx, ex));
+    for (int j = 0; j < ncy; ++j) tr.push_back(theta_cell_range_tight(j, ncy, T.hy, T.cs, g.ny, ex));
     dfree(ws->trange64);
     ws->trange64 = dalloc<int2>(tr.size());
     HIPC(hipMemcpyAsync(ws->trange64, tr.data(), sizeof(int2) * tr.size(), hipMemcpyHostToDevice, s));
@@ -2246,7 +2252,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
       }
       const double2 *PIk = ws->PI;
       ws->tstamp = ws->tstamp % 2000000000 + 1;   // never the initial 0
-      const LevelArgs LA{E.bcell, ws->trange64, ws->tlev, ws->tstamp, T.ngx - 1, T.ngy};
+      const LevelArgs LA{E.bcell, ws->trange64, ws->tlev, ws->tstamp, T.ngx - 1, T.ngy,
+                         ws->tkey};
       const EpochCells EC{E.bcell, E.scell, E.cstart};
       timed(c, K_ADJ_R, [&] {
         if (tape->keep_pi) {
@@ -2259,8 +2266,8 @@ void bw_run(sph_workspace *ws, sph_tape *tape, const sph_field *theta, const int
         }
       });
       timed(c, K_THETA, [&] {
-        launch_theta_cells_f64(s, Xk, ws->galpha, E.cstart, g, T, tfirst, ws->tpart, ws->tlev,
-                               ws->tstamp);
+        launch_theta_cells_keyed_f64(s, Xk, ws->galpha, ws->tkey, E.cstart, ws->trange64, g, T,
+                                     tfirst, ws->tpart, ws->tlev, ws->tstamp);
       });
       timed(c, K_ADJ_P1, [&] {
         launch_adj_p1_f64(s, Xk, Vk, PIk, ws->abar, ws->galpha, theta->d, nl, nbcnt, g, P, EC,

@@ -1129,7 +1129,7 @@ k_remap_rpw32(RpwArgs R, const int *__restrict__ bcell, const float2 *__restrict
 // of the slot's interp cell, so that a candidate costs one key compare; a
 // match's FP64 fractions (fx, fy) are formed from its position exactly as the
 // f32c interp_cell forms them (x * 1/hx, clamped, minus the cell index); the
-// contributions and their order are those of k_theta_cells.
+// contributions and their order are those of k_theta_cells_keyed.
 // A workgroup takes a strip of kThetaStrip theta cells of one theta row,
 // kThetaLanes lanes per cell; lane l of a cell sweeps every kThetaLanes-th
 // slot (from the l-th) of each SPH cell row of the cell's range, the rows in

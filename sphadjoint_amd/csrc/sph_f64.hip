@@ -359,16 +359,25 @@ k_force(const double2 *__restrict__ X, const double2 *__restrict__ V,
   if (tapeV) tapeV[s] = vi;
 }
 
-// The sweep level of the step's gtheta gather (k_theta_cells, a one-cell
-// margin unless raised) must reach slot s, whose binning cell at the epoch's
-// rebuild is L.bcell[s] and whose interp cell is l00: raise it when a larger
-// margin is needed (a slot that moved more than a cell since the rebuild).
+// The step's gtheta gather (k_theta_cells_keyed) sweeps the quarter-cell SPH
+// ranges of each theta cell at level 0, the one-cell ranges at level 1 and
+// those widened by m - 1 cells at level m >= 2.  Its level must reach slot s,
+// whose binning cell at the epoch's rebuild is L.bcell[s] and whose interp
+// cell is l00: raise it when s lies beyond the quarter-cell range (a slot that
+// moved more than a quarter cell past its binning cell since the rebuild).
+// The interp cell is also stored as the slot's key.
 __device__ __forceinline__ void sweep_level(const LevelArgs &L, const Grid &g, int s, int l00) {
+  L.key[s] = l00;
   const int i0 = l00 / L.ngy, j0 = l00 - i0 * L.ngy;
   const int c = L.bcell[s];
   const int cy = c / g.nx, cx = c - cy * g.nx;
-  const int m = max(margin_beyond(cx, L.trange[i0]), margin_beyond(cy, L.trange[L.ncx + j0]));
-  if (m >= 2) raise_level(L.lev, L.stamp, m);
+  const int ncy = L.ngy - 1;
+  const int2 *tt = L.trange + (L.ncx + ncy);
+  const int2 rx = tt[i0], ry = tt[L.ncx + j0];
+  if (cx < rx.x || cx > rx.y || cy < ry.x || cy > ry.y) {
+    const int m = max(margin_beyond(cx, L.trange[i0]), margin_beyond(cy, L.trange[L.ncx + j0]));
+    raise_level(L.lev, L.stamp, m > 1 ? m : 1);
+  }
 }
 
 // Adjoint pass R: density recompute, _adj_integrate! and _adj_drag!

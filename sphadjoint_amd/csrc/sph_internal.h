@@ -244,15 +244,23 @@ void launch_force_f64(hipStream_t s, const double2 *X, const double2 *V, const d
 // step's stamp, and the theta grid the gather sweeps for
 struct LevelArgs {
   const int *bcell;
-  const int2 *trange;   // one-cell-margin ranges: ncx theta columns, then the theta rows
+  const int2 *trange;   // one-cell-margin ranges: ncx theta columns, then the theta
+                        // rows; then the quarter-cell ranges in the same order
   unsigned long long *lev;
   int stamp, ncx, ngy;
+  int *key;             // out: each slot's interp cell l00, for the keyed gather
 };
 // the margin (>= 2) a sweep needs to reach binning cell c when c lies beyond
 // its one-cell-margin range w (theta_cell_range, which clamps to the axis), else 0
 __host__ __device__ inline int margin_beyond(int c, int2 w) {
   return c < w.x ? w.x + 1 - c : (c > w.y ? c - w.y + 1 : 0);
 }
+// the FP64 gtheta gather over the keys (LevelArgs::key) of the step's R pass
+// (trange: the table of LevelArgs::trange)
+void launch_theta_cells_keyed_f64(hipStream_t s, const double2 *X, const double *galpha,
+                                  const int *key, const int *cstart, const int2 *trange, Grid g,
+                                  ThetaGrid T, bool first, double4 *part,
+                                  const unsigned long long *lev, int stamp);
 void launch_adj_r_f64(hipStream_t s, const double2 *X, const double2 *Vnext, const double *theta,
                       const int *nl, const int *nbcnt, Grid g, PhysF64 P, const EpochCells &ec,
                       double2 *PI, double2 *gX, double2 *gV, double2 *abar, double *galpha,
@@ -275,13 +283,8 @@ void launch_objective_f64(hipStream_t s, const double2 *X, int N, double xc, dou
                           double2 *seed, const int *ids = nullptr);
 int objective_blocks(int N);
 
-// deterministic gtheta gather (sph_theta.hip): the theta-cell partials of one
-// step, written (first step of a neighbour-list epoch) or added to part; the
-// node sums of part are added to gtheta once per epoch (launch_theta_nodes_f64)
-// (the step's sweep level, raised by its R pass: the margin is max(1, level) cells)
-void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
-                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part,
-                            const unsigned long long *lev, int stamp);
+// node sums of the theta-cell partials (sph_theta.hip), added to gtheta once
+// per neighbour-list epoch
 void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta);
 
 // design-field chain of the topology optimiser (sph_design.hip): an

@@ -7,9 +7,11 @@
 // summation order changes from run to run.  Here the scatter becomes two
 // gathers in a fixed order:
 //   A  eight lanes per theta cell (i0, j0): sweep the SPH cells that can hold
-//      particles whose interp_cell is (i0, j0) (plus one cell of margin for
-//      the displacement since the last rebuild), recompute interp_cell exactly
-//      as the reference does, and sum the four weighted contributions
+//      particles whose interp_cell is (i0, j0) (plus a margin for the
+//      displacement since the last rebuild), keep the candidates whose
+//      interp_cell (stored by the R pass) is (i0, j0), recompute their
+//      fractions exactly as the reference does, and sum the four weighted
+//      contributions
 //      (ga*(1-fx))*(1-fy), (ga*fx)*(1-fy), (ga*(1-fx))*fy, (ga*fx)*fy;
 //      the lanes take candidates round-robin and combine by a fixed butterfly;
 //      The partials of the steps of one neighbour-list epoch add up in
@@ -27,18 +29,30 @@ namespace {
 
 inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// SPH cell (0-based, clamped) of coordinate v for the division rule.
-__device__ __forceinline__ int cell_of(double v, double cs, int n) {
-  return cell_coord_div(v / cs, n);
-}
-
 constexpr int kLanes = 8;   // lanes that share one theta cell
 
-template <class GA>
+// Gather A.  Each slot's key is its interp cell l00 = j0 + i0*ngy, stored by
+// the step's R pass (k_adj_r / k_adj_rpw, interp_cell of src/params.jl:53-68),
+// so that a candidate costs one key compare; a match's fractions are
+// recomputed from its position by the same rule (x / hx, clamped, minus the
+// cell index).  The gather is bound by dependent memory rounds, so a lane
+// reads the cell starts of kKeyRows rows at once, then the keys of up to
+// kKeyBatch of its candidates over those rows together, then the positions
+// and galpha of the matches among them kHitBatch at a time, adding them in
+// candidate order.  (Measured on the 1M block, FP64 form: 32.6 us a step for
+// a version that loaded every candidate's position and recomputed its interp
+// cell over the one-cell ranges; 30.1 us over the keys, with bit-identical
+// sums; 19.8 us over the quarter-cell ranges.  4 lanes a cell: 21.5 us,
+// 16 lanes: 23.5 us.)
+constexpr int kKeyRows = 4;
+constexpr int kKeyBatch = 16;
+constexpr int kHitBatch = 4;
 __global__ void __launch_bounds__(kBlock)
-k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
-              const int *__restrict__ cstart, Grid g, ThetaGrid T, int first,
-              double4 *__restrict__ part, const unsigned long long *__restrict__ lev, int stamp) {
+k_theta_cells_keyed(const double2 *__restrict__ X, const double *__restrict__ galpha,
+                    const int *__restrict__ key, const int *__restrict__ cstart,
+                    const int2 *__restrict__ trange, Grid g, ThetaGrid T, int first,
+                    double4 *__restrict__ part, const unsigned long long *__restrict__ lev,
+                    int stamp) {
   const int ncx = T.ngx - 1, ncy = T.ngy - 1;
   const int lane = threadIdx.x & (kLanes - 1);
   // theta cells in x-fastest order, so that neighbouring groups read
@@ -47,52 +61,78 @@ k_theta_cells(const double2 *__restrict__ X, const GA *__restrict__ galpha,
   if (t >= ncx * ncy) return;      // the kLanes lanes of a group leave together
   const int i0 = t % ncx;
   const int j0 = t / ncx;
-  // SPH cell range holding particles with interp cell (i0, j0), widened by a
-  // one-cell margin, or by the step's larger sweep level (raised by its R pass
-  // for a slot that moved further since the rebuild)
+  // SPH cell range holding particles with interp cell (i0, j0): a quarter cell
+  // of margin unless the step's R pass found a slot further out (level 1: one
+  // cell; level m >= 2: the one-cell range widened by m - 1 cells a side)
   const int lv = level_of(lev, stamp);
-  const int mg = lv > 1 ? lv : 1;
-  int cx0 = (i0 == 0) ? 0 : cell_of((double)i0 * T.hx, T.cs, g.nx) - mg;
-  int cx1 = (i0 == ncx - 1) ? g.nx - 1 : cell_of((double)(i0 + 1) * T.hx, T.cs, g.nx) + mg;
-  int cy0 = (j0 == 0) ? 0 : cell_of((double)j0 * T.hy, T.cs, g.ny) - mg;
-  int cy1 = (j0 == ncy - 1) ? g.ny - 1 : cell_of((double)(j0 + 1) * T.hy, T.cs, g.ny) + mg;
-  cx0 = cx0 < 0 ? 0 : cx0;
-  cy0 = cy0 < 0 ? 0 : cy0;
-  cx1 = cx1 > g.nx - 1 ? g.nx - 1 : cx1;
-  cy1 = cy1 > g.ny - 1 ? g.ny - 1 : cy1;
-  // cheap pre-filter in grid units (generous margin; the exact test follows)
-  const double lox = (i0 == 0) ? -1e300 : (double)i0 - 0.01;
-  const double hix = (i0 == ncx - 1) ? 1e300 : (double)(i0 + 1) + 0.01;
-  const double loy = (j0 == 0) ? -1e300 : (double)j0 - 0.01;
-  const double hiy = (j0 == ncy - 1) ? 1e300 : (double)(j0 + 1) + 0.01;
+  const int2 *tr = lv >= 1 ? trange : trange + (ncx + ncy);
+  const int wid = lv > 1 ? lv - 1 : 0;
+  const int2 rxw = tr[i0], ryw = tr[ncx + j0];
+  const int cx0 = max(rxw.x - wid, 0), cx1 = min(rxw.y + wid, g.nx - 1);
+  const int cy0 = max(ryw.x - wid, 0), cy1 = min(ryw.y + wid, g.ny - 1);
+  const int mykey = j0 + i0 * T.ngy;
   double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
-  for (int cy = cy0; cy <= cy1; ++cy) {
-    const int b = cstart[cy * g.nx + cx0], e = cstart[cy * g.nx + cx1 + 1];
-    for (int p = b + lane; p < e; p += kLanes) {
-      const double2 x = X[p];
-      const double ax = x.x * T.inv_hx, ay = x.y * T.inv_hy;
-      if (ax < lox || ax >= hix || ay < loy || ay >= hiy) continue;
-      // interp_cell (src/params.jl:53-68), exactly
-      double gx = x.x / T.hx;
-      double gy = x.y / T.hy;
-      double gxc = (gx > T.gxclamp) ? T.gxclamp : ((gx < 0.0) ? 0.0 : gx);
-      double gyc = (gy > T.gyclamp) ? T.gyclamp : ((gy < 0.0) ? 0.0 : gy);
-      double fxf = floor(gxc), fyf = floor(gyc);
-      int pi = (fxf > -1e9 && fxf < 1e9) ? (int)fxf : -1000000000;
-      int pj = (fyf > -1e9 && fyf < 1e9) ? (int)fyf : -1000000000;
-      pi = pi < 0 ? 0 : (pi > ncx - 1 ? ncx - 1 : pi);
-      pj = pj < 0 ? 0 : (pj > ncy - 1 ? ncy - 1 : pj);
-      if (pi != i0 || pj != j0) continue;
-      const double fx = gxc - (double)pi;
-      const double fy = gyc - (double)pj;
-      const double ga = (double)galpha[p];
-      s00 += (ga * (1.0 - fx)) * (1.0 - fy);
-      s10 += (ga * fx) * (1.0 - fy);
-      s01 += (ga * (1.0 - fx)) * fy;
-      s11 += (ga * fx) * fy;
+  // the lane's candidates of kKeyRows rows, in sweep order: candidate u of
+  // row q is slot rb[q] + kLanes u (u < its count), numbered k = pre[q] + u
+  for (int ya = cy0; ya <= cy1; ya += kKeyRows) {
+    int rb[kKeyRows], pre[kKeyRows];
+    int total = 0;
+#pragma unroll
+    for (int q = 0; q < kKeyRows; ++q) {
+      const int y = ya + q;
+      int b = 0, e = 0;
+      if (y <= cy1) {
+        b = cstart[y * g.nx + cx0];
+        e = cstart[y * g.nx + cx1 + 1];
+      }
+      rb[q] = b + lane;
+      pre[q] = total;
+      total += e > rb[q] ? (e - rb[q] + kLanes - 1) / kLanes : 0;
+    }
+    auto slot_of = [&](int k) {   // slot of candidate k < total
+      int q = 0;
+#pragma unroll
+      for (int r = 1; r < kKeyRows; ++r) q += (k >= pre[r]) ? 1 : 0;
+      return rb[q] + kLanes * (k - pre[q]);
+    };
+    for (int k0 = 0; k0 < total; k0 += kKeyBatch) {
+      int kv[kKeyBatch];
+#pragma unroll
+      for (int u = 0; u < kKeyBatch; ++u) kv[u] = k0 + u < total ? key[slot_of(k0 + u)] : -1;
+      uint32_t hits = 0;
+#pragma unroll
+      for (int u = 0; u < kKeyBatch; ++u) hits |= (kv[u] == mykey ? 1u : 0u) << u;
+      while (hits != 0) {
+        bool ok[kHitBatch];
+        double2 x[kHitBatch];
+        double ga[kHitBatch];
+#pragma unroll
+        for (int u = 0; u < kHitBatch; ++u) {
+          ok[u] = hits != 0;
+          if (ok[u]) {
+            const int p = slot_of(k0 + __builtin_ctz(hits));
+            hits &= hits - 1;
+            x[u] = X[p];
+            ga[u] = galpha[p];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kHitBatch; ++u) {
+          if (!ok[u]) continue;
+          // the fractions of interp_cell (src/params.jl:53-68), exactly
+          const double gx = x[u].x / T.hx, gy = x[u].y / T.hy;
+          const double gxc = (gx > T.gxclamp) ? T.gxclamp : ((gx < 0.0) ? 0.0 : gx);
+          const double gyc = (gy > T.gyclamp) ? T.gyclamp : ((gy < 0.0) ? 0.0 : gy);
+          const double fx = gxc - (double)i0;
+          const double fy = gyc - (double)j0;
+          s00 += (ga[u] * (1.0 - fx)) * (1.0 - fy);
+          s10 += (ga[u] * fx) * (1.0 - fy);
+          s01 += (ga[u] * (1.0 - fx)) * fy;
+          s11 += (ga[u] * fx) * fy;
+        }
+      }
     }
   }
-  // fixed butterfly over the group's lanes: the same order on every run
 #pragma unroll
   for (int m = kLanes / 2; m > 0; m >>= 1) {
     s00 += __shfl_xor(s00, m, kLanes);
@@ -146,13 +186,13 @@ __global__ void k_theta_nodes(const double4 *__restrict__ part, ThetaGrid T,
 
 }  // namespace
 
-void launch_theta_cells_f64(hipStream_t s, const double2 *X, const double *galpha,
-                            const int *cstart, Grid g, ThetaGrid T, bool first, double4 *part,
-                            const unsigned long long *lev, int stamp) {
+void launch_theta_cells_keyed_f64(hipStream_t s, const double2 *X, const double *galpha,
+                                  const int *key, const int *cstart, const int2 *trange, Grid g,
+                                  ThetaGrid T, bool first, double4 *part,
+                                  const unsigned long long *lev, int stamp) {
   const long long ncells = (long long)(T.ngx - 1) * (T.ngy - 1);
-  k_theta_cells<double><<<nblocks(ncells * kLanes), kBlock, 0, s>>>(X, galpha, cstart, g, T,
-                                                                    first ? 1 : 0, part, lev,
-                                                                    stamp);
+  k_theta_cells_keyed<<<nblocks(ncells * kLanes), kBlock, 0, s>>>(
+      X, galpha, key, cstart, trange, g, T, first ? 1 : 0, part, lev, stamp);
 }
 
 void launch_theta_nodes_f64(hipStream_t s, const double4 *part, ThetaGrid T, double *gtheta) {
